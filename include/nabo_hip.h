@@ -1,0 +1,151 @@
+/* nabo_hip.h - C ABI of the MI355X (gfx950) kNN library.
+ *
+ * Plain C types only: pointers, sizes and status codes. Host C++
+ * (include/nabo_amd/nabo.h), Python (libnabo_amd) and any FFI bind to these.
+ *
+ * Layout conventions are libnabo's (nabo/nabo.h:260-272): a cloud or query
+ * set is a column-major `rows x count` float matrix, one point per column,
+ * consecutive points `stride` floats apart (stride >= rows); results are
+ * column-major `k x m` (indices int32, dists2 float): the k results of query
+ * i are contiguous at [i*k, i*k+k). Missing results are (-1, +inf)
+ * (nabo/nabo.h:216-226, nabo/index_heap.h:299).
+ *
+ * Every function returns NABO_HIP_OK or an error code; the message of the
+ * last error on the calling thread is nabo_hip_last_error().
+ */
+#ifndef NABO_HIP_H
+#define NABO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NABO_HIP_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+  NABO_HIP_OK = 0,
+  NABO_HIP_ERR_ARGUMENT = 1,   /* the reference throws Nabo::runtime_error */
+  NABO_HIP_ERR_HIP = 2,        /* HIP runtime failure */
+  NABO_HIP_ERR_MEMORY = 3,     /* device allocation failed */
+  NABO_HIP_ERR_UNSUPPORTED = 4 /* valid for the reference, not offered here */
+};
+
+/* Search types: the reference's SearchType values 0..5 (nabo/nabo.h:290-299)
+ * keep their numbers; the HIP types are appended. */
+enum {
+  NABO_HIP_SEARCH_KDTREE = 6,     /* HIP kd-tree, KDTREE_LINEAR_HEAP semantics */
+  NABO_HIP_SEARCH_BRUTE_FORCE = 7 /* HIP brute force, BRUTE_FORCE semantics */
+};
+
+/* Creation flags (nabo/nabo.h:302-305) and search flags (nabo/nabo.h:308-312). */
+enum { NABO_HIP_TOUCH_STATISTICS = 1 };
+enum { NABO_HIP_ALLOW_SELF_MATCH = 1, NABO_HIP_SORT_RESULTS = 2 };
+
+/* Brute-force distance precisions (nabo_hip_params.bf_precision). */
+enum {
+  NABO_HIP_BF_FP32 = 0, /* exact FP32 sum of squared differences */
+  NABO_HIP_BF_BF16 = 1, /* MFMA bf16 shortlist of k+m, exact FP32 rescoring */
+  NABO_HIP_BF_FP16 = 2  /* MFMA fp16 shortlist of k+m, exact FP32 rescoring */
+};
+
+typedef struct nabo_hip_index nabo_hip_index;
+
+/* Creation parameters. The reference's Parameters map carries only
+ * "bucketSize" (nabo/kdtree_cpu.cpp:230); the other fields are this
+ * library's own keys. Initialise with nabo_hip_params_default(). */
+typedef struct nabo_hip_params {
+  uint32_t bucket_size;  /* "bucketSize", default 8, must be >= 2 */
+  int32_t query_sort;    /* 1: reorder queries by home leaf before searching */
+  int32_t bf_precision;  /* NABO_HIP_BF_* */
+  int32_t bf_extra;      /* m: shortlist size is k + m for bf16/fp16 */
+  int32_t device;        /* HIP device ordinal, -1 = current */
+  int32_t reserved[11];
+} nabo_hip_params;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* nabo_hip_last_error(void);
+
+/* Fills p with the defaults. */
+void nabo_hip_params_default(nabo_hip_params* p);
+
+/* Creates a search over a host cloud (rows x n, column stride `stride`).
+ * Replaces NearestNeighbourSearch::create (nabo/nabo.cpp:134-155) and the
+ * kd-tree / brute-force constructors (nabo/kdtree_cpu.cpp:227-272,
+ * nabo/brute_force_cpu.cpp:45-60). dim is clamped to rows (nabo/nabo.cpp:71).
+ * The tree is built on the GPU; the cloud is copied to HBM, so the host
+ * buffer need not outlive the call. */
+int nabo_hip_create(const float* cloud, int64_t n, int32_t rows, int64_t stride, int32_t dim,
+                    int32_t search_type, uint32_t creation_flags, const nabo_hip_params* params,
+                    nabo_hip_index** out);
+
+/* Same, for a cloud already resident in device memory (not retained). */
+int nabo_hip_create_device(const float* d_cloud, int64_t n, int32_t rows, int64_t stride,
+                           int32_t dim, int32_t search_type, uint32_t creation_flags,
+                           const nabo_hip_params* params, void* hip_stream,
+                           nabo_hip_index** out);
+
+/* Releases an index (the reference's `delete nns`). NULL is accepted. */
+int nabo_hip_destroy(nabo_hip_index* index);
+
+/* kNN for host buffers. Replaces both virtual knn overloads
+ * (nabo/nabo.h:338 and :351; nabo/kdtree_cpu.cpp:274-336,
+ * nabo/brute_force_cpu.cpp:63-103): max_radii == NULL selects the scalar
+ * max_radius form, otherwise one radius per query. query is qrows x m with
+ * column stride qstride; indices/dists2 are k x m. *touched (nullable)
+ * receives the TOUCH_STATISTICS count (the return value of the reference's
+ * knn, nabo/kdtree_cpu.cpp:303), 0 when the index was created without it.
+ * Argument checks follow checkSizesKnn (nabo/nabo.cpp:102-131). */
+int nabo_hip_knn(const nabo_hip_index* index, const float* query, int32_t qrows, int64_t qstride,
+                 int64_t m, int32_t k, float epsilon, uint32_t option_flags, float max_radius,
+                 const float* max_radii, int32_t* indices, float* dists2, uint64_t* touched);
+
+/* kNN for device buffers, enqueued on hip_stream (NULL = default stream);
+ * returns once enqueued. d_touched (nullable) is a device uint64 that
+ * receives the touched count. Stream-ordered scratch only: concurrent calls
+ * on one index from different streams are safe (knn is const and
+ * reentrant, nabo/nabo.h:179). */
+int nabo_hip_knn_device(const nabo_hip_index* index, const float* d_query, int32_t qrows,
+                        int64_t qstride, int64_t m, int32_t k, float epsilon,
+                        uint32_t option_flags, float max_radius, const float* d_max_radii,
+                        int32_t* d_indices, float* d_dists2, uint64_t* d_touched,
+                        void* hip_stream);
+
+/* Read-only properties of an index. */
+typedef struct nabo_hip_info {
+  int64_t n;            /* points */
+  int32_t rows;         /* cloud rows */
+  int32_t dim;          /* searched dimensions */
+  int32_t search_type;  /* NABO_HIP_SEARCH_* */
+  uint32_t bucket_size;
+  uint32_t dim_bits;    /* bits of the node word holding the cut dimension */
+  int32_t depth;        /* max split nodes on a root-leaf path (kd-tree) */
+  int64_t node_count;   /* kd-tree nodes */
+  int64_t device_bytes; /* HBM held by the index */
+  uint32_t creation_flags;
+  int32_t device;
+} nabo_hip_info;
+
+int nabo_hip_get_info(const nabo_hip_index* index, nabo_hip_info* info);
+
+/* Copies the bounding box (minBound/maxBound of nabo/nabo.h:276-279, with the
+ * reference's initial values when the tree is a single bucket) to host. */
+int nabo_hip_get_bounds(const nabo_hip_index* index, float* min_bound, float* max_bound);
+
+/* Copies the kd-tree to host: nodes as 2 x node_count uint32 (the node word
+ * `dim | (child or count) << dim_bits` and the cut value bits or the bucket
+ * start, nabo/nabo_private.h:138-153) and the n point indices of the buckets
+ * in leaf order (BucketEntry::index, nabo/nabo_private.h:158-168). */
+int nabo_hip_get_tree(const nabo_hip_index* index, uint32_t* nodes, int32_t* bucket_indices);
+
+/* Number of HIP devices visible (0 when none). */
+int nabo_hip_device_count(int32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NABO_HIP_H */

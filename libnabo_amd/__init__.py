@@ -1,0 +1,291 @@
+"""libnabo_amd: libnabo's kNN search on MI355X, behind libnabo's own interface.
+
+Python front-end of the C ABI in include/nabo_hip.h (libnabo_hip.so, built
+in-tree by libnabo_amd.build). It mirrors pynabo (python/nabo.cpp:95-217 of
+libnabo): ``NearestNeighbourSearch(cloud, searchType, dim, params)`` and
+``knn(query, k, epsilon, optionFlags, maxRadius) -> (indices, dists2)``, with
+the same layout rule: the contiguous axis of a 2-D array holds the
+coordinates of one point, and results keep the query's order with the k
+results of a query contiguous. Unlike pynabo (double only, python/nabo.cpp:10)
+the arithmetic is float32, as in ``Nabo::NNSearchF``.
+
+There is no CPU fallback: without libnabo_hip.so, or without a GPU, calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+
+import numpy as np
+
+__all__ = [
+    "SearchType", "SearchOptionFlags", "CreationOptionFlags", "NearestNeighbourSearch",
+    "NaboError", "load_library", "LIB_PATH", "Params",
+]
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libnabo_hip.so")
+
+
+class SearchType(enum.IntEnum):
+    """nabo/nabo.h:290-299, with the HIP types appended."""
+    BRUTE_FORCE = 0
+    KDTREE_LINEAR_HEAP = 1
+    KDTREE_TREE_HEAP = 2
+    KDTREE_CL_PT_IN_NODES = 3
+    KDTREE_CL_PT_IN_LEAVES = 4
+    BRUTE_FORCE_CL = 5
+    KDTREE_HIP = 6
+    BRUTE_FORCE_HIP = 7
+    SEARCH_TYPE_COUNT = 8
+
+
+class CreationOptionFlags(enum.IntFlag):
+    """nabo/nabo.h:302-305."""
+    TOUCH_STATISTICS = 1
+
+
+class SearchOptionFlags(enum.IntFlag):
+    """nabo/nabo.h:308-312."""
+    ALLOW_SELF_MATCH = 1
+    SORT_RESULTS = 2
+
+
+STATUS = {0: "ok", 1: "argument", 2: "hip", 3: "memory", 4: "unsupported"}
+BF_PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2}
+
+
+class NaboError(RuntimeError):
+    """Raised where the reference throws Nabo::runtime_error (nabo/nabo.cpp:50-66)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+        self.kind = STATUS.get(code, "unknown")
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("bucket_size", C.c_uint32),
+        ("query_sort", C.c_int32),
+        ("bf_precision", C.c_int32),
+        ("bf_extra", C.c_int32),
+        ("device", C.c_int32),
+        ("reserved", C.c_int32 * 11),
+    ]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("rows", C.c_int32),
+        ("dim", C.c_int32),
+        ("search_type", C.c_int32),
+        ("bucket_size", C.c_uint32),
+        ("dim_bits", C.c_uint32),
+        ("depth", C.c_int32),
+        ("node_count", C.c_int64),
+        ("device_bytes", C.c_int64),
+        ("creation_flags", C.c_uint32),
+        ("device", C.c_int32),
+    ]
+
+
+EXPORTED = [
+    "nabo_hip_last_error", "nabo_hip_params_default", "nabo_hip_create",
+    "nabo_hip_create_device", "nabo_hip_destroy", "nabo_hip_knn", "nabo_hip_knn_device",
+    "nabo_hip_get_info", "nabo_hip_get_bounds", "nabo_hip_get_tree", "nabo_hip_device_count",
+]
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libnabo_hip.so (raises when it has not been built)."""
+    global _LIB
+    if _LIB is not None and path == LIB_PATH:
+        return _LIB
+    if not os.path.exists(path):
+        raise NaboError(2, f"{path} is missing: build it with `python -m libnabo_amd.build`")
+    lib = C.CDLL(path)
+    vp, i32, i64, u32, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_float
+    lib.nabo_hip_last_error.restype = C.c_char_p
+    lib.nabo_hip_params_default.argtypes = [C.POINTER(Params)]
+    lib.nabo_hip_create.argtypes = [vp, i64, i32, i64, i32, i32, u32, C.POINTER(Params),
+                                    C.POINTER(vp)]
+    lib.nabo_hip_create_device.argtypes = [vp, i64, i32, i64, i32, i32, u32, C.POINTER(Params),
+                                           vp, C.POINTER(vp)]
+    lib.nabo_hip_destroy.argtypes = [vp]
+    lib.nabo_hip_knn.argtypes = [vp, vp, i32, i64, i64, i32, f32, u32, f32, vp, vp, vp,
+                                 C.POINTER(C.c_uint64)]
+    lib.nabo_hip_knn_device.argtypes = [vp, vp, i32, i64, i64, i32, f32, u32, f32, vp, vp, vp,
+                                        vp, vp]
+    lib.nabo_hip_get_info.argtypes = [vp, C.POINTER(Info)]
+    lib.nabo_hip_get_bounds.argtypes = [vp, vp, vp]
+    lib.nabo_hip_get_tree.argtypes = [vp, vp, vp]
+    lib.nabo_hip_device_count.argtypes = [C.POINTER(C.c_int32)]
+    for name in EXPORTED:
+        getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
+    lib.nabo_hip_last_error.restype = C.c_char_p
+    if path == LIB_PATH:
+        _LIB = lib
+    return lib
+
+
+def _check(lib, rc: int):
+    if rc != 0:
+        raise NaboError(rc, lib.nabo_hip_last_error().decode())
+
+
+def device_count() -> int:
+    lib = load_library()
+    c = C.c_int32(0)
+    _check(lib, lib.nabo_hip_device_count(C.byref(c)))
+    return c.value
+
+
+def _as_points(a, name: str) -> np.ndarray:
+    """(count, rows) float32 C-order view: pynabo's layout rule (python/nabo.cpp:35-49)."""
+    a = np.asarray(a)
+    if a.ndim != 2:
+        raise NaboError(1, f'Argument "{name}" must be a two-dimensional array')
+    if a.flags.f_contiguous and not a.flags.c_contiguous:
+        a = a.T  # contiguous axis = coordinates
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def make_params(bucket_size: int = 8, query_sort: bool = True, bf_precision: str = "fp32",
+                bf_extra: int = 32, device: int = -1, **unknown) -> Params:
+    """Parameters map (nabo/nabo.h:231-254): only "bucketSize" is a reference key."""
+    p = Params()
+    load_library().nabo_hip_params_default(C.byref(p))
+    p.bucket_size = int(bucket_size)
+    p.query_sort = 1 if query_sort else 0
+    p.bf_precision = BF_PRECISION[bf_precision]
+    p.bf_extra = int(bf_extra)
+    p.device = int(device)
+    return p
+
+
+_PARAM_KEYS = {"bucketSize": "bucket_size", "querySort": "query_sort",
+               "bfPrecision": "bf_precision", "bfExtra": "bf_extra", "device": "device"}
+
+
+class NearestNeighbourSearch:
+    """pynabo.NearestNeighbourSearch on the GPU (python/nabo.cpp:95-163).
+
+    ``cloud``: array of points (contiguous axis = coordinates); ``searchType``:
+    KDTREE_HIP (default; KDTREE_LINEAR_HEAP semantics) or BRUTE_FORCE_HIP;
+    ``params``: {"bucketSize": 8, ...}.
+    """
+
+    def __init__(self, cloud, searchType=SearchType.KDTREE_HIP, dim: int = 2**31 - 1,
+                 creationOptionFlags: int = 0, params: dict | None = None):
+        self._lib = load_library()
+        self._h = None
+        kw = {}
+        for key, val in (params or {}).items():
+            if key not in _PARAM_KEYS:
+                raise NaboError(1, f"unknown parameter {key!r}")
+            kw[_PARAM_KEYS[key]] = val
+        p = make_params(**kw)
+        if int(searchType) in (SearchType.KDTREE_LINEAR_HEAP, SearchType.KDTREE_TREE_HEAP):
+            searchType = SearchType.KDTREE_HIP
+        elif int(searchType) == SearchType.BRUTE_FORCE:
+            searchType = SearchType.BRUTE_FORCE_HIP
+        pts = _as_points(cloud, "cloud")
+        n, rows = pts.shape
+        h = C.c_void_p()
+        _check(self._lib, self._lib.nabo_hip_create(_ptr(pts), n, rows, rows, int(dim),
+                                                    int(searchType), int(creationOptionFlags),
+                                                    C.byref(p), C.byref(h)))
+        self._h = h
+        self.cloud = pts
+        self.searchType = SearchType(int(searchType))
+        self.creationOptionFlags = int(creationOptionFlags)
+        self.last_touched = 0
+
+    def close(self):
+        if self._h:
+            self._lib.nabo_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- properties -------------------------------------------------------
+    def info(self) -> Info:
+        inf = Info()
+        _check(self._lib, self._lib.nabo_hip_get_info(self._h, C.byref(inf)))
+        return inf
+
+    @property
+    def dim(self) -> int:
+        return self.info().dim
+
+    @property
+    def bounds(self):
+        d = self.dim
+        lo = np.empty(d, np.float32)
+        hi = np.empty(d, np.float32)
+        _check(self._lib, self._lib.nabo_hip_get_bounds(self._h, _ptr(lo), _ptr(hi)))
+        return lo, hi
+
+    def tree(self):
+        """(nodes[count, 2] uint32, bucket point indices[n]) as built on the GPU."""
+        inf = self.info()
+        nodes = np.empty((inf.node_count, 2), np.uint32)
+        bucket = np.empty(inf.n, np.int32)
+        _check(self._lib, self._lib.nabo_hip_get_tree(self._h, _ptr(nodes), _ptr(bucket)))
+        return nodes, bucket
+
+    # -- search -----------------------------------------------------------
+    def knn(self, query, k: int = 1, epsilon: float = 0.0, optionFlags: int = 0,
+            maxRadius: float = np.inf, maxRadii=None):
+        """Returns (indices, dists2), each (query count, k); see nabo/nabo.h:338,351."""
+        q = _as_points(query, "query")
+        m, qrows = q.shape
+        k = int(k)
+        idx = np.empty((m, max(k, 0)), np.int32)
+        d2 = np.empty((m, max(k, 0)), np.float32)
+        radii = None
+        if maxRadii is not None:
+            radii = np.ascontiguousarray(maxRadii, dtype=np.float32).ravel()
+            if radii.size != m:
+                raise NaboError(1, f"Maximum radii vector has not the same length ({radii.size}) "
+                                   f"than query has columns ({m})")
+        touched = C.c_uint64(0)
+        _check(self._lib, self._lib.nabo_hip_knn(self._h, _ptr(q), qrows, qrows, m, k,
+                                                 float(epsilon), int(optionFlags),
+                                                 float(maxRadius), _ptr(radii), _ptr(idx),
+                                                 _ptr(d2), C.byref(touched)))
+        self.last_touched = touched.value
+        return idx, d2
+
+    def knn_device(self, query, k: int, indices, dists2, epsilon: float = 0.0,
+                   optionFlags: int = 0, maxRadius: float = np.inf, maxRadii=None,
+                   touched=None, stream=None):
+        """Enqueues a search on torch device tensors (query: (m, rows) float32
+        contiguous; indices/dists2: (m, k) int32/float32) on ``stream``
+        (a torch.cuda.Stream, default: the current one)."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(query.device)
+        m, qrows = query.shape
+        _check(self._lib, self._lib.nabo_hip_knn_device(
+            self._h, C.c_void_p(query.data_ptr()), qrows, query.stride(0), m, int(k),
+            float(epsilon), int(optionFlags), float(maxRadius),
+            None if maxRadii is None else C.c_void_p(maxRadii.data_ptr()),
+            C.c_void_p(indices.data_ptr()), C.c_void_p(dists2.data_ptr()),
+            None if touched is None else C.c_void_p(touched.data_ptr()),
+            C.c_void_p(stream.cuda_stream)))

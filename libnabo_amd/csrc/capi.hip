@@ -1,0 +1,439 @@
+// C ABI of libnabo_hip (declared in include/nabo_hip.h).
+//
+// Argument checks and their messages follow the reference:
+//   create      nabo/nabo.cpp:68-80, 134-155; nabo/kdtree_cpu.cpp:227-251
+//   knn         nabo/nabo.cpp:102-131 (checkSizesKnn)
+// Exceptions never cross the ABI: each entry point maps them to a status
+// code and keeps the message for nabo_hip_last_error().
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+struct nabo_hip_index {
+  nabo_hip::Index ix;
+};
+
+namespace nabo_hip {
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return NABO_HIP_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return NABO_HIP_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return NABO_HIP_ERR_ARGUMENT;
+  }
+}
+
+[[noreturn]] void arg_error(const std::string& msg) { throw Error(NABO_HIP_ERR_ARGUMENT, msg); }
+
+uint32_t storage_bits(uint32_t v) {  // getStorageBitCount, nabo/kdtree_cpu.cpp:60-70
+  uint32_t b = 0;
+  while (v) {
+    ++b;
+    v >>= 1;
+  }
+  return b;
+}
+
+// Stream-ordered allocations are served from the device's default pool; keep
+// freed blocks in the pool instead of returning them at every synchronize.
+void prepare_device(int dev) {
+  static std::mutex mu;
+  static std::vector<int> done;
+  std::lock_guard<std::mutex> lock(mu);
+  if (std::find(done.begin(), done.end(), dev) != done.end()) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+    uint64_t threshold = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+  }
+  done.push_back(dev);
+}
+
+// One non-blocking stream per host thread and device for the host-buffer
+// entry points (default priority).
+hipStream_t thread_stream(int dev) {
+  thread_local std::vector<std::pair<int, hipStream_t>> streams;
+  for (auto& e : streams)
+    if (e.first == dev) return e.second;
+  hipStream_t s;
+  NH_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  streams.emplace_back(dev, s);
+  return s;
+}
+
+struct DeviceScope {
+  int prev = -1;
+  explicit DeviceScope(int dev) {
+    NH_CHECK(hipGetDevice(&prev));
+    if (dev != prev) NH_CHECK(hipSetDevice(dev));
+  }
+  ~DeviceScope() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+void check_create(int64_t n, int32_t rows, int64_t stride, int32_t dim, int32_t type,
+                  const nabo_hip_params& p) {
+  if (dim <= 0) arg_error("Your space must have at least one dimension");
+  if (n == 0) arg_error("Cloud has no points");
+  if (rows == 0) arg_error("Cloud has 0 dimensions");
+  if (n < 0 || rows < 0) arg_error("Cloud has a negative size");
+  if (stride < rows) arg_error("Cloud column stride is smaller than its number of rows");
+  if (type >= 0 && type <= 5)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                "search type " + std::to_string(type) +
+                    " is a CPU search of the reference; the HIP library offers KDTREE_HIP (6) "
+                    "and BRUTE_FORCE_HIP (7)");
+  if (type != NABO_HIP_SEARCH_KDTREE && type != NABO_HIP_SEARCH_BRUTE_FORCE)
+    arg_error("Unknown search type");
+  if (n >= (int64_t(1) << 30))
+    throw Error(NABO_HIP_ERR_UNSUPPORTED, "clouds of 2^30 points or more are not supported");
+  if (type == NABO_HIP_SEARCH_KDTREE) {
+    const uint32_t bucket = p.bucket_size;
+    if (bucket < 2)
+      arg_error("Requested bucket size " + std::to_string(bucket) + ", but must be larger than 2");
+    const uint32_t used_dim = uint32_t(std::min(dim, rows));
+    const uint32_t bits = storage_bits(used_dim);
+    if (n > int64_t(bucket)) {
+      const uint64_t max_nodes = (uint64_t(1) << (32 - bits)) - 1;
+      const uint64_t est = uint64_t(n) / (bucket / 2);
+      if (est > max_nodes)
+        arg_error("Cloud has a risk to have more nodes (" + std::to_string(est) +
+                  ") than the kd-tree allows (" + std::to_string(max_nodes) +
+                  "). The kd-tree has " + std::to_string(bits) + " bits for dimensions and " +
+                  std::to_string(32 - bits) + " bits for node indices");
+    }
+  }
+}
+
+void build_index(nabo_hip_index* h, const float* d_cloud, int64_t n, int32_t rows, int64_t stride,
+                 int32_t dim, int32_t type, uint32_t cflags, const nabo_hip_params& p,
+                 hipStream_t s) {
+  Index& ix = h->ix;
+  ix.search_type = type;
+  ix.rows = rows;
+  ix.dim = std::min(dim, rows);
+  ix.n = n;
+  ix.creation_flags = cflags;
+  ix.params = p;
+  ix.bucket_size = p.bucket_size;
+  ix.dim_bits = storage_bits(uint32_t(ix.dim));
+  ix.dim_mask = (1u << ix.dim_bits) - 1u;
+  if (type == NABO_HIP_SEARCH_KDTREE) {
+    kd_build(ix, d_cloud, stride, s);
+    ix.device_bytes = int64_t(sizeof(uint2)) * ix.node_count +
+                      int64_t(sizeof(float)) * ix.pts_stride * ix.n;
+  } else {
+    bf_prepare(ix, d_cloud, stride, s);
+    ix.device_bytes = int64_t(sizeof(float)) * ix.n * ix.dim;
+  }
+  NH_CHECK(hipStreamSynchronize(s));
+}
+
+void check_knn(const Index& ix, int32_t qrows, int64_t qstride, int64_t m, int32_t k,
+               uint32_t flags) {
+  const bool allow_self = flags & NABO_HIP_ALLOW_SELF_MATCH;
+  if (allow_self) {
+    if (k > ix.n)
+      arg_error("Requesting more points (" + std::to_string(k) + ") than available in cloud (" +
+                std::to_string(ix.n) + ")");
+  } else if (k > ix.n - 1) {
+    arg_error("Requesting more points (" + std::to_string(k) +
+              ") than available in cloud minus 1 (" + std::to_string(ix.n - 1) +
+              ") (as self match is forbidden)");
+  }
+  if (qrows < ix.dim)
+    arg_error("Query has less dimensions (" + std::to_string(qrows) +
+              ") than requested for cloud (" + std::to_string(ix.dim) + ")");
+  if (flags > 3u)
+    arg_error("OR-ed value of option flags (" + std::to_string(flags) +
+              ") is larger than maximal valid value (3)");
+  if (k < 0) arg_error("k must not be negative");
+  if (m < 0) arg_error("Query has a negative number of columns");
+  if (m > 0 && qstride < qrows) arg_error("Query column stride is smaller than its number of rows");
+}
+
+__global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
+
+// The touched counter is cleared by a kernel rather than a memset node so
+// that callers may capture the call in a HIP graph.
+void run_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
+  if (a.touched) {
+    k_set_u64<<<1, 1, 0, s>>>(a.touched, 0);
+    NH_LAUNCH_CHECK();
+  }
+  if (a.k == 0 || a.m == 0) return;
+  if (ix.search_type == NABO_HIP_SEARCH_KDTREE) {
+    KnnArgs b = a;
+    if (!(ix.creation_flags & NABO_HIP_TOUCH_STATISTICS)) b.touched = nullptr;
+    kd_knn(ix, b, s);
+  } else {
+    bf_knn(ix, a, s);
+  }
+}
+
+}  // namespace
+
+void throw_hip(hipError_t e, const char* what, const char* file, int line) {
+  const int code = (e == hipErrorOutOfMemory) ? NABO_HIP_ERR_MEMORY : NABO_HIP_ERR_HIP;
+  throw Error(code, std::string(what) + " failed: " + hipGetErrorString(e) + " (" + file + ":" +
+                        std::to_string(line) + ")");
+}
+
+Index::~Index() {
+  if (d_nodes) (void)hipFree(d_nodes);
+  if (d_pts) (void)hipFree(d_pts);
+  if (d_cloud) (void)hipFree(d_cloud);
+  if (d_cloud_lp) (void)hipFree(d_cloud_lp);
+  if (d_norms) (void)hipFree(d_norms);
+}
+
+}  // namespace nabo_hip
+
+using namespace nabo_hip;
+
+extern "C" {
+
+const char* nabo_hip_last_error(void) { return g_last_error.c_str(); }
+
+void nabo_hip_params_default(nabo_hip_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->bucket_size = 8;
+  p->query_sort = 1;
+  p->bf_precision = NABO_HIP_BF_FP32;
+  p->bf_extra = 32;
+  p->device = -1;
+}
+
+int nabo_hip_device_count(int32_t* count) {
+  return guarded([&] {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    if (count) *count = c;
+  });
+}
+
+int nabo_hip_create(const float* cloud, int64_t n, int32_t rows, int64_t stride, int32_t dim,
+                    int32_t search_type, uint32_t creation_flags, const nabo_hip_params* params,
+                    nabo_hip_index** out) {
+  return guarded([&] {
+    if (!out) arg_error("output handle pointer is null");
+    *out = nullptr;
+    nabo_hip_params p;
+    nabo_hip_params_default(&p);
+    if (params) p = *params;
+    check_create(n, rows, stride, dim, search_type, p);
+    if (!cloud) arg_error("cloud pointer is null");
+    int dev = p.device;
+    if (dev < 0) NH_CHECK(hipGetDevice(&dev));
+    DeviceScope scope(dev);
+    prepare_device(dev);
+    hipStream_t s = thread_stream(dev);
+    const int64_t floats = stride * (n - 1) + rows;
+    DevBuf<float> d_cloud(size_t(floats), s);
+    NH_CHECK(hipMemcpyAsync(d_cloud.p, cloud, sizeof(float) * size_t(floats),
+                            hipMemcpyHostToDevice, s));
+    auto* h = new nabo_hip_index();
+    h->ix.device = dev;
+    try {
+      build_index(h, d_cloud.p, n, rows, stride, dim, search_type, creation_flags, p, s);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int nabo_hip_create_device(const float* d_cloud, int64_t n, int32_t rows, int64_t stride,
+                           int32_t dim, int32_t search_type, uint32_t creation_flags,
+                           const nabo_hip_params* params, void* hip_stream,
+                           nabo_hip_index** out) {
+  return guarded([&] {
+    if (!out) arg_error("output handle pointer is null");
+    *out = nullptr;
+    nabo_hip_params p;
+    nabo_hip_params_default(&p);
+    if (params) p = *params;
+    check_create(n, rows, stride, dim, search_type, p);
+    if (!d_cloud) arg_error("cloud pointer is null");
+    int dev = p.device;
+    if (dev < 0) NH_CHECK(hipGetDevice(&dev));
+    DeviceScope scope(dev);
+    prepare_device(dev);
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    auto* h = new nabo_hip_index();
+    h->ix.device = dev;
+    try {
+      build_index(h, d_cloud, n, rows, stride, dim, search_type, creation_flags, p, s);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int nabo_hip_destroy(nabo_hip_index* index) {
+  return guarded([&] {
+    if (!index) return;
+    DeviceScope scope(index->ix.device);
+    delete index;
+  });
+}
+
+int nabo_hip_knn_device(const nabo_hip_index* index, const float* d_query, int32_t qrows,
+                        int64_t qstride, int64_t m, int32_t k, float epsilon,
+                        uint32_t option_flags, float max_radius, const float* d_max_radii,
+                        int32_t* d_indices, float* d_dists2, uint64_t* d_touched,
+                        void* hip_stream) {
+  return guarded([&] {
+    if (!index) arg_error("index is null");
+    const Index& ix = index->ix;
+    check_knn(ix, qrows, qstride, m, k, option_flags);
+    if (m > 0 && k > 0 && (!d_query || !d_indices || !d_dists2))
+      arg_error("query or output pointer is null");
+    DeviceScope scope(ix.device);
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    KnnArgs a;
+    a.query = d_query;
+    a.qrows = qrows;
+    a.qstride = qstride;
+    a.m = m;
+    a.k = k;
+    a.epsilon = epsilon;
+    a.flags = option_flags;
+    a.max_radius = max_radius;
+    a.radii = d_max_radii;
+    a.out_idx = d_indices;
+    a.out_d2 = d_dists2;
+    a.touched = d_touched;
+    run_knn(ix, a, s);
+    if (d_touched && ix.search_type == NABO_HIP_SEARCH_BRUTE_FORCE &&
+        (ix.creation_flags & NABO_HIP_TOUCH_STATISTICS)) {
+      k_set_u64<<<1, 1, 0, s>>>(d_touched, uint64_t(m) * uint64_t(ix.n));
+      NH_LAUNCH_CHECK();
+    }
+  });
+}
+
+int nabo_hip_knn(const nabo_hip_index* index, const float* query, int32_t qrows, int64_t qstride,
+                 int64_t m, int32_t k, float epsilon, uint32_t option_flags, float max_radius,
+                 const float* max_radii, int32_t* indices, float* dists2, uint64_t* touched) {
+  return guarded([&] {
+    if (!index) arg_error("index is null");
+    const Index& ix = index->ix;
+    check_knn(ix, qrows, qstride, m, k, option_flags);
+    if (touched) *touched = 0;
+    if (m == 0 || k == 0) return;
+    if (!query || !indices || !dists2) arg_error("query or output pointer is null");
+    DeviceScope scope(ix.device);
+    hipStream_t s = thread_stream(ix.device);
+    const int64_t qfloats = qstride * (m - 1) + qrows;
+    DevBuf<float> dq(size_t(qfloats), s), dr(max_radii ? size_t(m) : 0, s), dd(size_t(m) * k, s);
+    DevBuf<int32_t> di(size_t(m) * k, s);
+    DevBuf<uint64_t> dt(1, s);
+    NH_CHECK(hipMemcpyAsync(dq.p, query, sizeof(float) * size_t(qfloats), hipMemcpyHostToDevice, s));
+    if (max_radii)
+      NH_CHECK(hipMemcpyAsync(dr.p, max_radii, sizeof(float) * size_t(m), hipMemcpyHostToDevice, s));
+    KnnArgs a;
+    a.query = dq.p;
+    a.qrows = qrows;
+    a.qstride = qstride;
+    a.m = m;
+    a.k = k;
+    a.epsilon = epsilon;
+    a.flags = option_flags;
+    a.max_radius = max_radius;
+    a.radii = max_radii ? dr.p : nullptr;
+    a.out_idx = di.p;
+    a.out_d2 = dd.p;
+    a.touched = dt.p;
+    run_knn(ix, a, s);
+    NH_CHECK(hipMemcpyAsync(indices, di.p, sizeof(int32_t) * size_t(m) * k, hipMemcpyDeviceToHost, s));
+    NH_CHECK(hipMemcpyAsync(dists2, dd.p, sizeof(float) * size_t(m) * k, hipMemcpyDeviceToHost, s));
+    uint64_t t = 0;
+    NH_CHECK(hipMemcpyAsync(&t, dt.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    NH_CHECK(hipStreamSynchronize(s));
+    if (touched) {
+      if (!(ix.creation_flags & NABO_HIP_TOUCH_STATISTICS))
+        *touched = 0;
+      else if (ix.search_type == NABO_HIP_SEARCH_BRUTE_FORCE)
+        *touched = uint64_t(m) * uint64_t(ix.n);
+      else
+        *touched = t;
+    }
+  });
+}
+
+int nabo_hip_get_info(const nabo_hip_index* index, nabo_hip_info* info) {
+  return guarded([&] {
+    if (!index || !info) arg_error("null argument");
+    const Index& ix = index->ix;
+    std::memset(info, 0, sizeof(*info));
+    info->n = ix.n;
+    info->rows = ix.rows;
+    info->dim = ix.dim;
+    info->search_type = ix.search_type;
+    info->bucket_size = ix.bucket_size;
+    info->dim_bits = ix.dim_bits;
+    info->depth = ix.depth;
+    info->node_count = ix.node_count;
+    info->device_bytes = ix.device_bytes;
+    info->creation_flags = ix.creation_flags;
+    info->device = ix.device;
+  });
+}
+
+int nabo_hip_get_bounds(const nabo_hip_index* index, float* min_bound, float* max_bound) {
+  return guarded([&] {
+    if (!index || !min_bound || !max_bound) arg_error("null argument");
+    const Index& ix = index->ix;
+    std::copy(ix.min_bound.begin(), ix.min_bound.end(), min_bound);
+    std::copy(ix.max_bound.begin(), ix.max_bound.end(), max_bound);
+  });
+}
+
+int nabo_hip_get_tree(const nabo_hip_index* index, uint32_t* nodes, int32_t* bucket_indices) {
+  return guarded([&] {
+    if (!index || !nodes || !bucket_indices) arg_error("null argument");
+    const Index& ix = index->ix;
+    if (ix.search_type != NABO_HIP_SEARCH_KDTREE) arg_error("index is not a kd-tree");
+    DeviceScope scope(ix.device);
+    hipStream_t s = thread_stream(ix.device);
+    NH_CHECK(hipMemcpyAsync(nodes, ix.d_nodes, sizeof(uint2) * size_t(ix.node_count),
+                            hipMemcpyDeviceToHost, s));
+    std::vector<float> pts(size_t(ix.n) * ix.pts_stride);
+    NH_CHECK(hipMemcpyAsync(pts.data(), ix.d_pts, sizeof(float) * pts.size(), hipMemcpyDeviceToHost,
+                            s));
+    NH_CHECK(hipStreamSynchronize(s));
+    for (int64_t p = 0; p < ix.n; ++p) {
+      int32_t v;
+      std::memcpy(&v, &pts[size_t(p * ix.pts_stride + ix.dim)], 4);
+      bucket_indices[p] = v;
+    }
+  });
+}
+
+}  // extern "C"

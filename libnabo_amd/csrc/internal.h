@@ -1,0 +1,122 @@
+// Internal declarations shared by the HIP translation units of libnabo_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nabo_hip.h"
+
+namespace nabo_hip {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line);
+
+#define NH_CHECK(expr)                                              \
+  do {                                                              \
+    hipError_t nh_e_ = (expr);                                      \
+    if (nh_e_ != hipSuccess) throw_hip(nh_e_, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+#define NH_LAUNCH_CHECK() NH_CHECK(hipGetLastError())
+
+// Stream-ordered device buffer (hipMallocAsync / hipFreeAsync).
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t count = 0;
+  hipStream_t s = nullptr;
+  DevBuf() = default;
+  DevBuf(size_t n, hipStream_t stream) { alloc(n, stream); }
+  void alloc(size_t n, hipStream_t stream) {
+    release();
+    s = stream;
+    count = n;
+    if (n) {
+      hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&p), n * sizeof(T), stream);
+      if (e != hipSuccess) {
+        p = nullptr;
+        throw Error(NABO_HIP_ERR_MEMORY, "device allocation of " + std::to_string(n * sizeof(T)) +
+                                             " bytes failed: " + hipGetErrorString(e));
+      }
+    }
+  }
+  void release() {
+    if (p) (void)hipFreeAsync(p, s);
+    p = nullptr;
+    count = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// A built search structure resident in HBM.
+struct Index {
+  int search_type = NABO_HIP_SEARCH_KDTREE;
+  int device = 0;
+  int rows = 0;
+  int dim = 0;
+  int64_t n = 0;
+  uint32_t creation_flags = 0;
+  nabo_hip_params params{};
+
+  // kd-tree (points in leaves, implicit bounds)
+  uint32_t bucket_size = 8;
+  uint32_t dim_bits = 0;
+  uint32_t dim_mask = 0;
+  int depth = 0;
+  int64_t node_count = 0;
+  uint2* d_nodes = nullptr;   // {dim | child << dim_bits, cut bits or bucket start}
+  float* d_pts = nullptr;     // leaf-ordered points, pts_stride floats each,
+                              // slot `dim` holds the point index bits
+  int pts_stride = 4;
+
+  // brute force
+  float* d_cloud = nullptr;   // dim x n, column stride bf_stride
+  int bf_stride = 0;
+  uint16_t* d_cloud_lp = nullptr;  // low-precision copy for the MFMA shortlist
+  float* d_norms = nullptr;        // |p|^2 per point (low-precision path)
+
+  std::vector<float> min_bound, max_bound;
+  int64_t device_bytes = 0;
+
+  ~Index();
+};
+
+struct KnnArgs {
+  const float* query = nullptr;  // device
+  int qrows = 0;
+  int64_t qstride = 0;
+  int64_t m = 0;
+  int k = 1;
+  float epsilon = 0.f;
+  uint32_t flags = 0;
+  float max_radius = 0.f;
+  const float* radii = nullptr;  // device, nullable
+  int32_t* out_idx = nullptr;    // device
+  float* out_d2 = nullptr;       // device
+  uint64_t* touched = nullptr;   // device, nullable
+};
+
+// Builds the kd-tree of d_cloud on the GPU (kdtree_build.hip).
+void kd_build(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s);
+// Searches a built kd-tree (kdtree_search.hip).
+void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s);
+// Prepares and searches a brute-force index (brute_force.hip).
+void bf_prepare(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s);
+void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s);
+
+// Largest k the register-resident k-best arrays support.
+constexpr int kMaxK = 128;
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace nabo_hip

@@ -1,0 +1,553 @@
+// kd-tree build on the GPU: the unbalanced points-in-leaves tree of
+// KDTreeUnbalancedPtInLeavesImplicitBoundsStackOpt (nabo/kdtree_cpu.cpp:109-272),
+// node for node and bucket entry for bucket entry.
+//
+// The recursive build is run breadth-first, one level of segments at a time:
+//   1. classify each segment: leaf if count <= bucketSize, else cut dimension
+//      = first widest cell extent, ideal cut = cell midpoint
+//      (nabo/kdtree_cpu.cpp:117-135, argMax :76-90);
+//   2. min/max of the points on the cut dimension (getBounds :93-107), cut
+//      value clamped to them (:141-147);
+//   3. the two Hoare passes (:149-177). A pass swaps the i-th out-of-place
+//      element counted from the left with the i-th out-of-place element
+//      counted from the right, so its result is a pure function of ranks:
+//      a global exclusive scan of the out-of-place flags gives every rank and
+//      the swap pairs follow without any sequential scan;
+//   4. leftCount rule (:180-190) and the children's clipped cells (:207-212).
+// Nodes are recorded in BFS order; subtree sizes (bottom-up) then give the
+// reference's pre-order numbering (left child = parent + 1, :215-223).
+// Bucket starts are the positions of the leaf segments in the final
+// permutation, which is the reference's bucket vector order.
+
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <cfloat>
+
+#include "device_common.h"
+#include "internal.h"
+
+namespace nabo_hip {
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Seg {  // one level of segments
+  int32_t* start;
+  int32_t* count;
+  float* lo;  // S x dim
+  float* hi;  // S x dim
+};
+
+struct SegStat {
+  uint32_t* cd;
+  float* ideal;
+  float* cut;
+  uint32_t* minkey;
+  uint32_t* maxkey;
+  uint32_t* lt;
+  uint32_t* eq;
+  uint32_t* split;
+  uint32_t* childbase;
+  int32_t* left;
+};
+
+struct Bfs {  // all nodes in BFS order
+  uint32_t* cd;     // cut dimension, or dim for a leaf
+  uint32_t* aux;    // cut value bits, or bucket start
+  uint32_t* count;  // leaf point count
+  int32_t* left;
+  int32_t* right;
+  uint32_t* size;   // nodes in the subtree
+  uint32_t* pre;    // pre-order id
+};
+
+__global__ void k_bounds(int64_t n, const float* __restrict__ cloud, int64_t stride, int dim,
+                         uint32_t* __restrict__ keys /* 2*dim: min keys, max keys */) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool act = p < n;
+  for (int d = 0; d < dim; ++d) {
+    const float v = act ? cloud[p * stride + d] : 0.f;
+    const uint32_t kv = float_key(v);
+    uint32_t mn = wave_min_u32(act ? kv : 0xffffffffu);
+    uint32_t mx = wave_max_u32(act ? kv : 0u);
+    if (__lane_id() == 0) {
+      atomicMin(&keys[d], mn);
+      atomicMax(&keys[dim + d], mx);
+    }
+  }
+}
+
+__global__ void k_init_keys(int dim, uint32_t* keys) {
+  for (int d = threadIdx.x; d < dim; d += blockDim.x) {
+    keys[d] = float_key(FLT_MAX);        // minBound starts at max() (nabo.cpp:73)
+    keys[dim + d] = float_key(FLT_MIN);  // maxBound starts at min() > 0 (nabo.cpp:74)
+  }
+}
+
+__global__ void k_root(int dim, int32_t n, const uint32_t* keys, Seg seg, float* bounds_out) {
+  for (int d = threadIdx.x; d < dim; d += blockDim.x) {
+    const float lo = key_float(keys[d]);
+    const float hi = key_float(keys[dim + d]);
+    seg.lo[d] = lo;
+    seg.hi[d] = hi;
+    bounds_out[d] = lo;
+    bounds_out[dim + d] = hi;
+  }
+  if (threadIdx.x == 0) {
+    seg.start[0] = 0;
+    seg.count[0] = n;
+  }
+}
+
+__global__ void k_iota(int64_t n, int32_t* perm, int32_t* pseg) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p < n) {
+    perm[p] = int32_t(p);
+    pseg[p] = 0;
+  }
+}
+
+__global__ void k_seg_classify(int32_t S, Seg seg, SegStat st, Bfs bfs, int64_t node_base,
+                               uint32_t bucket, int dim) {
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int32_t cnt = seg.count[s];
+  const int64_t node = node_base + s;
+  if (cnt <= int32_t(bucket)) {
+    st.split[s] = 0;
+    bfs.cd[node] = uint32_t(dim);
+    bfs.aux[node] = uint32_t(seg.start[s]);
+    bfs.count[node] = uint32_t(cnt);
+    return;
+  }
+  st.split[s] = 1;
+  const float* lo = seg.lo + int64_t(s) * dim;
+  const float* hi = seg.hi + int64_t(s) * dim;
+  float best = 0.f;
+  uint32_t axis = 0;
+  for (int d = 0; d < dim; ++d) {
+    const float extent = hi[d] - lo[d];
+    if (extent > best) {
+      best = extent;
+      axis = uint32_t(d);
+    }
+  }
+  st.cd[s] = axis;
+  st.ideal[s] = (hi[axis] + lo[axis]) * 0.5f;  // exact: same value as /2
+  st.minkey[s] = 0xffffffffu;
+  st.maxkey[s] = 0u;
+  st.lt[s] = 0u;
+  st.eq[s] = 0u;
+}
+
+__global__ void k_split_total(int32_t S, const uint32_t* split, const uint32_t* childbase,
+                              int32_t* total) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *total = int32_t(childbase[S - 1] + split[S - 1]);
+}
+
+// Gathers the cut coordinate of every point still inside a splitting segment
+// and reduces its min/max per segment.
+__global__ void k_pt_gather(int64_t n, int32_t* __restrict__ pseg, const int32_t* __restrict__ perm,
+                            const float* __restrict__ cloud, int64_t stride, SegStat st,
+                            float* __restrict__ xs) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int s = p < n ? pseg[p] : -1;
+  if (s >= 0 && st.split[s] == 0) {
+    pseg[p] = -1;  // its segment became a leaf
+    s = -1;
+  }
+  const bool act = s >= 0;
+  uint32_t key = 0;
+  if (act) {
+    const float x = cloud[int64_t(perm[p]) * stride + st.cd[s]];
+    xs[p] = x;
+    key = float_key(x);
+  }
+  wave_segmented_atomic<0>(act, s, key, st.minkey);
+  wave_segmented_atomic<1>(act, s, key, st.maxkey);
+}
+
+__global__ void k_seg_cut(int32_t S, SegStat st) {
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S || st.split[s] == 0) return;
+  const float pmin = key_float(st.minkey[s]);
+  const float pmax = key_float(st.maxkey[s]);
+  const float ideal = st.ideal[s];
+  float cut;
+  if (ideal < pmin)
+    cut = pmin;
+  else if (ideal > pmax)
+    cut = pmax;
+  else
+    cut = ideal;
+  st.cut[s] = cut;
+}
+
+__global__ void k_pt_count(int64_t n, const int32_t* __restrict__ pseg,
+                           const float* __restrict__ xs, SegStat st) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int s = p < n ? pseg[p] : -1;
+  const bool act = s >= 0;
+  uint32_t lt = 0, eq = 0;
+  if (act) {
+    const float x = xs[p];
+    const float cut = st.cut[s];
+    lt = x < cut;
+    eq = x == cut;
+  }
+  wave_segmented_atomic<2>(act, s, lt, st.lt);
+  wave_segmented_atomic<2>(act, s, eq, st.eq);
+}
+
+// Out-of-place flags of one Hoare pass: low word = belongs right but sits
+// left, high word = belongs left but sits right.
+__global__ void k_pt_flags(int64_t n, int pass, const int32_t* __restrict__ pseg,
+                           const float* __restrict__ xs, Seg seg, SegStat st,
+                           uint64_t* __restrict__ flags) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int s = pseg[p];
+  uint64_t f = 0;
+  if (s >= 0) {
+    const int32_t pos = int32_t(p) - seg.start[s];
+    const float x = xs[p];
+    const float cut = st.cut[s];
+    const int32_t br1 = int32_t(st.lt[s]);
+    bool fl, fr;
+    if (pass == 1) {
+      fl = pos < br1 && !(x < cut);
+      fr = pos >= br1 && x < cut;
+    } else {
+      // After pass 1 every x in [br1, count) is >= cut, so "> cut" is
+      // "!= cut" and "<= cut" is "== cut"; written with == the two counts
+      // match by construction.
+      const int32_t br2 = br1 + int32_t(st.eq[s]);
+      fl = pos >= br1 && pos < br2 && !(x == cut);
+      fr = pos >= br2 && x == cut;
+    }
+    f = uint64_t(fl) | (uint64_t(fr) << 32);
+  }
+  flags[p] = f;
+}
+
+__device__ __forceinline__ void pass_ranks(int64_t p, int s, Seg seg,
+                                           const uint64_t* __restrict__ flags,
+                                           const uint64_t* __restrict__ excl, uint64_t f,
+                                           int32_t* start_out, uint32_t* rank_l,
+                                           uint32_t* rank_r) {
+  const int32_t start = seg.start[s];
+  const int32_t last = start + seg.count[s] - 1;
+  const uint64_t base = excl[start];
+  const uint64_t mine = excl[p];
+  *start_out = start;
+  *rank_l = uint32_t(mine) - uint32_t(base);
+  if (f >> 32) {
+    const uint64_t end = excl[last] + flags[last];
+    const uint32_t total_r = uint32_t(end >> 32) - uint32_t(base >> 32);
+    const uint32_t from_left = uint32_t(mine >> 32) - uint32_t(base >> 32);
+    *rank_r = total_r - 1u - from_left;
+  } else {
+    *rank_r = 0;
+  }
+}
+
+__global__ void k_pt_slots(int64_t n, const int32_t* __restrict__ pseg, Seg seg,
+                           const uint64_t* __restrict__ flags, const uint64_t* __restrict__ excl,
+                           int32_t* __restrict__ slot_l, int32_t* __restrict__ slot_r) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint64_t f = flags[p];
+  if (f == 0) return;
+  const int s = pseg[p];
+  int32_t start;
+  uint32_t rl, rr;
+  pass_ranks(p, s, seg, flags, excl, f, &start, &rl, &rr);
+  if (uint32_t(f))
+    slot_l[start + rl] = int32_t(p);
+  else
+    slot_r[start + rr] = int32_t(p);
+}
+
+__global__ void k_pt_move(int64_t n, const int32_t* __restrict__ pseg, Seg seg,
+                          const uint64_t* __restrict__ flags, const uint64_t* __restrict__ excl,
+                          const int32_t* __restrict__ slot_l, const int32_t* __restrict__ slot_r,
+                          const int32_t* __restrict__ perm, const float* __restrict__ xs,
+                          int32_t* __restrict__ perm_out, float* __restrict__ xs_out) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint64_t f = flags[p];
+  int64_t dst = p;
+  if (f) {
+    const int s = pseg[p];
+    int32_t start;
+    uint32_t rl, rr;
+    pass_ranks(p, s, seg, flags, excl, f, &start, &rl, &rr);
+    dst = uint32_t(f) ? slot_r[start + rl] : slot_l[start + rr];
+  }
+  perm_out[dst] = perm[p];
+  xs_out[dst] = xs[p];
+}
+
+__global__ void k_seg_children(int32_t S, Seg seg, SegStat st, Bfs bfs, int64_t node_base,
+                               int64_t next_base, Seg next, int dim) {
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S || st.split[s] == 0) return;
+  const int32_t cnt = seg.count[s];
+  const int32_t start = seg.start[s];
+  const float ideal = st.ideal[s];
+  const float pmin = key_float(st.minkey[s]);
+  const float pmax = key_float(st.maxkey[s]);
+  const int32_t br1 = int32_t(st.lt[s]);
+  const int32_t br2 = br1 + int32_t(st.eq[s]);
+  int32_t left;
+  if (ideal < pmin)
+    left = 1;
+  else if (ideal > pmax)
+    left = cnt - 1;
+  else if (br1 > cnt / 2)
+    left = br1;
+  else if (br2 < cnt / 2)
+    left = br2;
+  else
+    left = cnt / 2;
+  left = min(max(left, 1), cnt - 1);  // no-op for finite coordinates
+  st.left[s] = left;
+  const uint32_t cd = st.cd[s];
+  const float cut = st.cut[s];
+  const int32_t c = int32_t(2 * st.childbase[s]);
+  next.start[c] = start;
+  next.count[c] = left;
+  next.start[c + 1] = start + left;
+  next.count[c + 1] = cnt - left;
+  for (int d = 0; d < dim; ++d) {
+    const float lo = seg.lo[int64_t(s) * dim + d];
+    const float hi = seg.hi[int64_t(s) * dim + d];
+    next.lo[int64_t(c) * dim + d] = lo;
+    next.hi[int64_t(c) * dim + d] = (uint32_t(d) == cd) ? cut : hi;
+    next.lo[int64_t(c + 1) * dim + d] = (uint32_t(d) == cd) ? cut : lo;
+    next.hi[int64_t(c + 1) * dim + d] = hi;
+  }
+  const int64_t node = node_base + s;
+  bfs.cd[node] = cd;
+  bfs.aux[node] = __float_as_uint(cut);
+  bfs.left[node] = int32_t(next_base + c);
+  bfs.right[node] = int32_t(next_base + c + 1);
+}
+
+__global__ void k_pt_reseg(int64_t n, int32_t* __restrict__ pseg, Seg seg, SegStat st) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int s = pseg[p];
+  if (s < 0) return;
+  const int32_t pos = int32_t(p) - seg.start[s];
+  pseg[p] = int32_t(2 * st.childbase[s]) + (pos < st.left[s] ? 0 : 1);
+}
+
+__global__ void k_subtree_size(int64_t base, int32_t S, Bfs bfs, int dim) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S) return;
+  const int64_t node = base + i;
+  if (bfs.cd[node] == uint32_t(dim))
+    bfs.size[node] = 1;
+  else
+    bfs.size[node] = 1 + bfs.size[bfs.left[node]] + bfs.size[bfs.right[node]];
+}
+
+__global__ void k_preorder(int64_t base, int32_t S, Bfs bfs, int dim) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S) return;
+  const int64_t node = base + i;
+  if (node == 0) bfs.pre[0] = 0;
+  if (bfs.cd[node] == uint32_t(dim)) return;
+  const uint32_t pre = bfs.pre[node];
+  const int32_t l = bfs.left[node];
+  bfs.pre[l] = pre + 1;
+  bfs.pre[bfs.right[node]] = pre + 1 + bfs.size[l];
+}
+
+__global__ void k_encode(int64_t count, Bfs bfs, uint32_t dim_bits, int dim, uint2* nodes) {
+  const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (node >= count) return;
+  const uint32_t cd = bfs.cd[node];
+  uint2 out;
+  if (cd == uint32_t(dim)) {
+    out.x = uint32_t(dim) | (bfs.count[node] << dim_bits);
+    out.y = bfs.aux[node];
+  } else {
+    out.x = cd | (bfs.pre[bfs.right[node]] << dim_bits);
+    out.y = bfs.aux[node];
+  }
+  nodes[bfs.pre[node]] = out;
+}
+
+// Leaf-ordered copy of the points: pts_stride floats per entry, coordinates
+// then the point index (as bits) in slot dim.
+__global__ void k_pack_points(int64_t n, const int32_t* __restrict__ perm,
+                              const float* __restrict__ cloud, int64_t stride, int dim,
+                              int pts_stride, float* __restrict__ pts) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int32_t idx = perm[p];
+  float* dst = pts + p * pts_stride;
+  const float* src = cloud + int64_t(idx) * stride;
+  for (int d = 0; d < dim; ++d) dst[d] = src[d];
+  dst[dim] = __int_as_float(idx);
+  for (int d = dim + 1; d < pts_stride; ++d) dst[d] = 0.f;
+}
+
+inline unsigned grid_for(int64_t n) { return unsigned(ceil_div(n, kBlock)); }
+
+}  // namespace
+
+void kd_build(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s) {
+  const int64_t n = ix.n;
+  const int dim = ix.dim;
+  ix.pts_stride = int((dim + 1 + 3) / 4 * 4);
+
+  NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_pts), sizeof(float) * size_t(n) * ix.pts_stride));
+  float* pts = ix.d_pts;
+  ix.min_bound.assign(size_t(dim), FLT_MAX);
+  ix.max_bound.assign(size_t(dim), FLT_MIN);
+
+  if (n <= int64_t(ix.bucket_size)) {
+    // Single bucket holding every point in index order (kdtree_cpu.cpp:236-243);
+    // the bounds keep their initial values there.
+    DevBuf<int32_t> perm(size_t(n), s), pseg(size_t(n), s);
+    k_iota<<<grid_for(n), kBlock, 0, s>>>(n, perm.p, pseg.p);
+    k_pack_points<<<grid_for(n), kBlock, 0, s>>>(n, perm.p, d_cloud, stride, dim, ix.pts_stride,
+                                                  pts);
+    NH_LAUNCH_CHECK();
+    uint2 root;
+    root.x = uint32_t(dim) | (uint32_t(n) << ix.dim_bits);
+    root.y = 0;
+    NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_nodes), sizeof(uint2)));
+    NH_CHECK(hipMemcpyAsync(ix.d_nodes, &root, sizeof(uint2), hipMemcpyHostToDevice, s));
+    NH_CHECK(hipStreamSynchronize(s));
+    ix.node_count = 1;
+    ix.depth = 0;
+    return;
+  }
+
+  const int64_t max_nodes = 2 * n;
+  // Level buffers (double-buffered), per-segment stats, BFS node records.
+  DevBuf<int32_t> seg_start[2], seg_count[2];
+  DevBuf<float> seg_lo[2], seg_hi[2];
+  for (int b = 0; b < 2; ++b) {
+    seg_start[b].alloc(size_t(n), s);
+    seg_count[b].alloc(size_t(n), s);
+    seg_lo[b].alloc(size_t(n) * dim, s);
+    seg_hi[b].alloc(size_t(n) * dim, s);
+  }
+  DevBuf<uint32_t> st_cd(n, s), st_minkey(n, s), st_maxkey(n, s), st_lt(n, s), st_eq(n, s),
+      st_split(n, s), st_childbase(n, s);
+  DevBuf<float> st_ideal(n, s), st_cut(n, s);
+  DevBuf<int32_t> st_left(n, s);
+  DevBuf<uint32_t> b_cd(max_nodes, s), b_aux(max_nodes, s), b_count(max_nodes, s),
+      b_size(max_nodes, s), b_pre(max_nodes, s);
+  DevBuf<int32_t> b_left(max_nodes, s), b_right(max_nodes, s);
+
+  DevBuf<int32_t> perm(n, s), perm2(n, s), pseg(n, s), slot_l(n, s), slot_r(n, s);
+  DevBuf<float> xs(n, s), xs2(n, s);
+  DevBuf<uint64_t> flags(n, s), excl(n, s);
+  DevBuf<uint32_t> keys(size_t(2 * dim), s);
+  DevBuf<float> bounds(size_t(2 * dim), s);
+  DevBuf<int32_t> d_total(1, s);
+
+  // Scan scratch, sized for the largest scan (n uint64).
+  size_t scan_bytes64 = 0, scan_bytes32 = 0;
+  NH_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes64, flags.p, excl.p, uint64_t(0), size_t(n),
+                                   rocprim::plus<uint64_t>(), s));
+  NH_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes32, st_split.p, st_childbase.p, uint32_t(0),
+                                   size_t(n), rocprim::plus<uint32_t>(), s));
+  DevBuf<uint8_t> scan_tmp(std::max(scan_bytes64, scan_bytes32) + 16, s);
+
+  SegStat st{st_cd.p, st_ideal.p, st_cut.p, st_minkey.p, st_maxkey.p, st_lt.p, st_eq.p,
+             st_split.p, st_childbase.p, st_left.p};
+  Bfs bfs{b_cd.p, b_aux.p, b_count.p, b_left.p, b_right.p, b_size.p, b_pre.p};
+  auto level = [&](int b) {
+    return Seg{seg_start[b].p, seg_count[b].p, seg_lo[b].p, seg_hi[b].p};
+  };
+
+  // Bounds pass (kdtree_cpu.cpp:256-267) and the root segment.
+  k_init_keys<<<1, 64, 0, s>>>(dim, keys.p);
+  k_bounds<<<grid_for(n), kBlock, 0, s>>>(n, d_cloud, stride, dim, keys.p);
+  k_root<<<1, 64, 0, s>>>(dim, int32_t(n), keys.p, level(0), bounds.p);
+  k_iota<<<grid_for(n), kBlock, 0, s>>>(n, perm.p, pseg.p);
+  NH_LAUNCH_CHECK();
+  std::vector<float> hb(size_t(2 * dim));
+  NH_CHECK(hipMemcpyAsync(hb.data(), bounds.p, sizeof(float) * 2 * dim, hipMemcpyDeviceToHost, s));
+
+  std::vector<int64_t> level_base;
+  std::vector<int32_t> level_size;
+  int32_t S = 1;
+  int64_t node_base = 0;
+  int cur = 0;
+  int32_t host_total = 0;
+  while (S > 0) {
+    if (node_base + S > max_nodes || int64_t(level_base.size()) > n)
+      throw Error(NABO_HIP_ERR_HIP, "kd-tree build overflow");
+    level_base.push_back(node_base);
+    level_size.push_back(S);
+    Seg sg = level(cur);
+    Seg nx = level(cur ^ 1);
+    const unsigned gs = grid_for(S);
+    k_seg_classify<<<gs, kBlock, 0, s>>>(S, sg, st, bfs, node_base, ix.bucket_size, dim);
+    size_t bytes = scan_tmp.count;
+    NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, st.split, st.childbase, uint32_t(0),
+                                     size_t(S), rocprim::plus<uint32_t>(), s));
+    k_split_total<<<1, 64, 0, s>>>(S, st.split, st.childbase, d_total.p);
+    NH_LAUNCH_CHECK();
+    NH_CHECK(hipMemcpyAsync(&host_total, d_total.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NH_CHECK(hipStreamSynchronize(s));
+    const int32_t T = host_total;
+    if (T > 0) {
+      const unsigned gp = grid_for(n);
+      k_pt_gather<<<gp, kBlock, 0, s>>>(n, pseg.p, perm.p, d_cloud, stride, st, xs.p);
+      k_seg_cut<<<gs, kBlock, 0, s>>>(S, st);
+      k_pt_count<<<gp, kBlock, 0, s>>>(n, pseg.p, xs.p, st);
+      for (int pass = 1; pass <= 2; ++pass) {
+        k_pt_flags<<<gp, kBlock, 0, s>>>(n, pass, pseg.p, xs.p, sg, st, flags.p);
+        bytes = scan_tmp.count;
+        NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, flags.p, excl.p, uint64_t(0),
+                                         size_t(n), rocprim::plus<uint64_t>(), s));
+        k_pt_slots<<<gp, kBlock, 0, s>>>(n, pseg.p, sg, flags.p, excl.p, slot_l.p, slot_r.p);
+        k_pt_move<<<gp, kBlock, 0, s>>>(n, pseg.p, sg, flags.p, excl.p, slot_l.p, slot_r.p,
+                                        perm.p, xs.p, perm2.p, xs2.p);
+        std::swap(perm.p, perm2.p);
+        std::swap(xs.p, xs2.p);
+      }
+      k_seg_children<<<gs, kBlock, 0, s>>>(S, sg, st, bfs, node_base, node_base + S, nx, dim);
+      k_pt_reseg<<<gp, kBlock, 0, s>>>(n, pseg.p, sg, st);
+      NH_LAUNCH_CHECK();
+    }
+    node_base += S;
+    S = 2 * T;
+    cur ^= 1;
+  }
+  const int64_t count = node_base;
+  const int levels = int(level_base.size());
+
+  for (int L = levels - 1; L >= 0; --L)
+    k_subtree_size<<<grid_for(level_size[L]), kBlock, 0, s>>>(level_base[L], level_size[L], bfs,
+                                                              dim);
+  for (int L = 0; L < levels; ++L)
+    k_preorder<<<grid_for(level_size[L]), kBlock, 0, s>>>(level_base[L], level_size[L], bfs, dim);
+  NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_nodes), sizeof(uint2) * count));
+  k_encode<<<grid_for(count), kBlock, 0, s>>>(count, bfs, ix.dim_bits, dim, ix.d_nodes);
+  k_pack_points<<<grid_for(n), kBlock, 0, s>>>(n, perm.p, d_cloud, stride, dim, ix.pts_stride,
+                                                pts);
+  NH_LAUNCH_CHECK();
+  NH_CHECK(hipStreamSynchronize(s));
+  for (int d = 0; d < dim; ++d) {
+    ix.min_bound[d] = hb[d];
+    ix.max_bound[d] = hb[dim + d];
+  }
+  ix.node_count = count;
+  ix.depth = levels - 1;
+}
+
+}  // namespace nabo_hip

@@ -1,0 +1,390 @@
+// kd-tree kNN search on the GPU: the depth-first descent of
+// KDTreeUnbalancedPtInLeavesImplicitBoundsStackOpt::recurseKnn
+// (nabo/kdtree_cpu.cpp:367-447) with the sorted-array k-best of
+// IndexHeapBruteForceVector (nabo/index_heap.h:263-369).
+//
+// One query per lane. The recursion becomes a per-lane explicit stack with
+// one 8-byte entry per split ancestor of the current node:
+//   pending  {parent node, rd of its far child}: pushed when the lane
+//            descends into the near child; popped after the near subtree,
+//            when the reference tests `rd <= maxRadius2 && rd*maxError2 <
+//            head` (:415-417 / :433-435) against the k-best as it is then;
+//   restore  {cut dim | flag, previous off[cd]}: replaces the pending entry
+//            while the far subtree is visited and puts off[cd] back after
+//            it (:419-424 / :437-442).
+// The far child's rd is formed from the same operands, in the same order,
+// as the reference, so the visit order, the pruning decisions, ties and
+// dists2 are bit-identical, for epsilon > 0 as well. The stack lives in HBM,
+// slot-major ([slot][lane]), so lanes at the same depth touch one line.
+//
+// The k-best array is K = next supported size >= k entries in VGPRs; the
+// first K - k entries hold -inf and never move, so the head (largest kept
+// distance) is always entry K - 1 and no register is indexed dynamically.
+//
+// Queries are first ordered by the leaf their greedy descent reaches, so the
+// lanes of a wave walk neighbouring paths (a radix sort on the leaf's bucket
+// start); results are written to each query's own column.
+
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <cfloat>
+
+#include "device_common.h"
+#include "internal.h"
+
+namespace nabo_hip {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kRestore = 0x80000000u;
+constexpr int kMaxGenericDim = 64;
+
+// Coordinates / offsets of one query in VGPRs, indexed by a runtime cut
+// dimension through selects (DIM > 0), or a local array (DIM == 0).
+template <int DIM>
+struct Vec {
+  float v[DIM];
+  __device__ __forceinline__ float get(uint32_t i) const {
+    float r = v[0];
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) r = (i == uint32_t(d)) ? v[d] : r;
+    return r;
+  }
+  __device__ __forceinline__ void set(uint32_t i, float x) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) v[d] = (i == uint32_t(d)) ? x : v[d];
+  }
+};
+template <>
+struct Vec<0> {
+  float v[kMaxGenericDim];
+  __device__ __forceinline__ float get(uint32_t i) const { return v[i]; }
+  __device__ __forceinline__ void set(uint32_t i, float x) { v[i] = x; }
+};
+
+struct SearchParams {
+  const uint2* nodes;
+  const float* pts;
+  int pts_stride;
+  uint32_t dim_bits;
+  uint32_t dim_mask;
+  int dim;
+  const float* query;
+  int64_t qstride;
+  int64_t m;
+  const int32_t* order;
+  int k;
+  float max_err2;
+  float max_r2;
+  const float* radii;
+  int32_t* out_idx;
+  float* out_d2;
+  unsigned long long* touched;
+  uint2* stack;
+  int64_t stack_lanes;
+  int allow_self;
+};
+
+// Sorted insertion of (value, index): entries greater than value move up by
+// one, value lands after the entries equal to it (index_heap.h:322-334).
+template <int K>
+__device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], float value,
+                                            int32_t index) {
+#pragma unroll
+  for (int j = K - 1; j > 0; --j) {
+    const bool shift = hd[j - 1] > value;
+    const bool place = !shift && hd[j] > value;
+    hd[j] = shift ? hd[j - 1] : (place ? value : hd[j]);
+    hi[j] = shift ? hi[j - 1] : (place ? index : hi[j]);
+  }
+  const bool place0 = hd[0] > value;
+  hd[0] = place0 ? value : hd[0];
+  hi[0] = place0 ? index : hi[0];
+}
+
+template <int DIM>
+__device__ __forceinline__ float point_dist(const SearchParams& a, const Vec<DIM>& q, int64_t e,
+                                            int32_t* index) {
+  if constexpr (DIM > 0 && DIM <= 3) {
+    const float4 p = reinterpret_cast<const float4*>(a.pts)[e];
+    const float c[4] = {p.x, p.y, p.z, p.w};
+    float dist = 0.f;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const float diff = q.v[d] - c[d];
+      dist = dist + diff * diff;
+    }
+    *index = __float_as_int(c[DIM]);
+    return dist;
+  } else {
+    const float* p = a.pts + e * a.pts_stride;
+    const int dim = DIM > 0 ? DIM : a.dim;
+    float dist = 0.f;
+    for (int d = 0; d < dim; ++d) {
+      const float diff = q.v[d] - p[d];
+      dist = dist + diff * diff;
+    }
+    *index = __float_as_int(p[dim]);
+    return dist;
+  }
+}
+
+template <int DIM, int K>
+__global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
+  const int64_t lane_id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t lanes = int64_t(gridDim.x) * blockDim.x;
+  const int dim = DIM > 0 ? DIM : a.dim;
+  const uint32_t leaf_tag = uint32_t(dim);
+  uint2* stack = a.stack + lane_id;
+  const int64_t sstride = a.stack_lanes;
+  uint64_t touched = 0;
+
+  for (int64_t i = lane_id; i < a.m; i += lanes) {
+    const int64_t qi = a.order ? a.order[i] : i;
+    Vec<DIM> q, off;
+    const float* qp = a.query + qi * a.qstride;
+    for (int d = 0; d < dim; ++d) {
+      q.v[d] = qp[d];
+      off.v[d] = 0.f;
+    }
+    float hd[K];
+    int32_t hi[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      hd[j] = j < K - a.k ? -INFINITY : INFINITY;
+      hi[j] = -1;
+    }
+    float r2 = a.max_r2;
+    if (a.radii) {
+      const float r = a.radii[qi];
+      r2 = r * r;
+    }
+
+    uint32_t n = 0;
+    float rd = 0.f;
+    int sp = 0;
+    for (;;) {
+      const uint2 nd = a.nodes[n];
+      const uint32_t cd = nd.x & a.dim_mask;
+      if (cd != leaf_tag) {
+        const float new_off = q.get(cd) - __uint_as_float(nd.y);
+        const float old_off = off.get(cd);
+        const float rd_far = rd + (-old_off * old_off + new_off * new_off);
+        stack[sp * sstride] = make_uint2(n, __float_as_uint(rd_far));
+        ++sp;
+        n = new_off > 0 ? (nd.x >> a.dim_bits) : n + 1;
+        continue;
+      }
+      const uint32_t count = nd.x >> a.dim_bits;
+      const int64_t start = nd.y;
+      for (uint32_t j = 0; j < count; ++j) {
+        int32_t index;
+        const float dist = point_dist<DIM>(a, q, start + j, &index);
+        if (dist <= r2 && dist < hd[K - 1] && (a.allow_self || dist > FLT_EPSILON))
+          heap_insert<K>(hd, hi, dist, index);
+      }
+      touched += count;
+
+      bool descend = false;
+      while (sp > 0) {
+        --sp;
+        const uint2 e = stack[sp * sstride];
+        if (e.x & kRestore) {
+          off.set(e.x & ~kRestore, __uint_as_float(e.y));
+          continue;
+        }
+        const float rdf = __uint_as_float(e.y);
+        if (rdf <= r2 && rdf * a.max_err2 < hd[K - 1]) {
+          const uint2 pn = a.nodes[e.x];
+          const uint32_t pcd = pn.x & a.dim_mask;
+          const float new_off = q.get(pcd) - __uint_as_float(pn.y);
+          const uint32_t far = new_off > 0 ? e.x + 1 : (pn.x >> a.dim_bits);
+          stack[sp * sstride] = make_uint2(kRestore | pcd, __float_as_uint(off.get(pcd)));
+          ++sp;
+          off.set(pcd, new_off);
+          n = far;
+          rd = rdf;
+          descend = true;
+          break;
+        }
+      }
+      if (!descend) break;
+    }
+
+    int32_t* oi = a.out_idx + qi * a.k;
+    float* od = a.out_d2 + qi * a.k;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      if (j >= K - a.k) {
+        oi[j - (K - a.k)] = hi[j];
+        od[j - (K - a.k)] = hd[j];
+      }
+    }
+  }
+
+  if (a.touched) {
+    uint32_t lo = uint32_t(touched), hi32 = uint32_t(touched >> 32);
+    uint64_t sum = 0;
+    // 64-bit wave sum in two halves with carry through a 64-bit accumulate.
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint32_t olo = (uint32_t)__shfl_xor((int)lo, o, kWave);
+      const uint32_t ohi = (uint32_t)__shfl_xor((int)hi32, o, kWave);
+      const uint64_t t = ((uint64_t(hi32) << 32) | lo) + ((uint64_t(ohi) << 32) | olo);
+      lo = uint32_t(t);
+      hi32 = uint32_t(t >> 32);
+    }
+    sum = (uint64_t(hi32) << 32) | lo;
+    if (__lane_id() == 0 && sum) atomicAdd(a.touched, (unsigned long long)sum);
+  }
+}
+
+// Greedy descent to the leaf a query falls into; its bucket start orders the
+// queries.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nodes,
+                                                   uint32_t dim_bits, uint32_t dim_mask, int dim_rt,
+                                                   const float* __restrict__ query,
+                                                   int64_t qstride, int64_t m,
+                                                   uint32_t* __restrict__ keys,
+                                                   int32_t* __restrict__ vals) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int dim = DIM > 0 ? DIM : dim_rt;
+  const float* qp = query + i * qstride;
+  uint32_t n = 0;
+  for (;;) {
+    const uint2 nd = nodes[n];
+    const uint32_t cd = nd.x & dim_mask;
+    if (cd == uint32_t(dim)) {
+      keys[i] = nd.y;
+      break;
+    }
+    const float new_off = qp[cd] - __uint_as_float(nd.y);
+    n = new_off > 0 ? (nd.x >> dim_bits) : n + 1;
+  }
+  vals[i] = int32_t(i);
+}
+
+template <int DIM, int K>
+void launch_search(const SearchParams& p, unsigned grid, hipStream_t s) {
+  k_kd_search<DIM, K><<<grid, kBlock, 0, s>>>(p);
+}
+
+template <int DIM>
+void dispatch_k(const SearchParams& p, unsigned grid, hipStream_t s) {
+  const int k = p.k;
+  if (k <= 1)
+    launch_search<DIM, 1>(p, grid, s);
+  else if (k <= 2)
+    launch_search<DIM, 2>(p, grid, s);
+  else if (k <= 4)
+    launch_search<DIM, 4>(p, grid, s);
+  else if (k <= 8)
+    launch_search<DIM, 8>(p, grid, s);
+  else if (k <= 16)
+    launch_search<DIM, 16>(p, grid, s);
+  else if (k <= 32)
+    launch_search<DIM, 32>(p, grid, s);
+  else if (k <= 64)
+    launch_search<DIM, 64>(p, grid, s);
+  else
+    launch_search<DIM, 128>(p, grid, s);
+}
+
+uint32_t bits_needed(uint64_t v) {
+  uint32_t b = 0;
+  while (v) {
+    ++b;
+    v >>= 1;
+  }
+  return b;
+}
+
+}  // namespace
+
+void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
+  if (a.m == 0) return;
+  if (a.k > kMaxK)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                "k = " + std::to_string(a.k) + " exceeds the HIP kd-tree limit of " +
+                    std::to_string(kMaxK));
+  if (ix.dim > kMaxGenericDim && ix.dim != 2 && ix.dim != 3)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                "dim = " + std::to_string(ix.dim) + " exceeds the HIP kd-tree limit of " +
+                    std::to_string(kMaxGenericDim));
+
+  SearchParams p{};
+  p.nodes = ix.d_nodes;
+  p.pts = ix.d_pts;
+  p.pts_stride = ix.pts_stride;
+  p.dim_bits = ix.dim_bits;
+  p.dim_mask = ix.dim_mask;
+  p.dim = ix.dim;
+  p.query = a.query;
+  p.qstride = a.qstride;
+  p.m = a.m;
+  p.k = a.k;
+  p.max_err2 = (1 + a.epsilon) * (1 + a.epsilon);
+  p.max_r2 = a.max_radius * a.max_radius;
+  p.radii = a.radii;
+  p.out_idx = a.out_idx;
+  p.out_d2 = a.out_d2;
+  p.touched = reinterpret_cast<unsigned long long*>(a.touched);
+  p.allow_self = (a.flags & NABO_HIP_ALLOW_SELF_MATCH) ? 1 : 0;
+
+  // Query order by home leaf.
+  DevBuf<uint32_t> keys, keys_sorted;
+  DevBuf<int32_t> vals, order;
+  DevBuf<uint8_t> sort_tmp;
+  if (ix.params.query_sort && a.m > 1 && ix.node_count > 1) {
+    keys.alloc(size_t(a.m), s);
+    keys_sorted.alloc(size_t(a.m), s);
+    vals.alloc(size_t(a.m), s);
+    order.alloc(size_t(a.m), s);
+    const unsigned g = unsigned(ceil_div(a.m, kBlock));
+    if (ix.dim == 3)
+      k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
+                                       a.qstride, a.m, keys.p, vals.p);
+    else if (ix.dim == 2)
+      k_locate<2><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
+                                       a.qstride, a.m, keys.p, vals.p);
+    else
+      k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
+                                       a.qstride, a.m, keys.p, vals.p);
+    NH_LAUNCH_CHECK();
+    const unsigned end_bit = std::max(1u, bits_needed(uint64_t(ix.n)));
+    size_t bytes = 0;
+    NH_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, keys.p, keys_sorted.p, vals.p, order.p,
+                                       size_t(a.m), 0, end_bit, s));
+    sort_tmp.alloc(bytes + 16, s);
+    NH_CHECK(rocprim::radix_sort_pairs(sort_tmp.p, bytes, keys.p, keys_sorted.p, vals.p, order.p,
+                                       size_t(a.m), 0, end_bit, s));
+    p.order = order.p;
+  }
+
+  // Persistent grid: enough lanes to fill the device, one stack per lane.
+  int dev = 0, cus = 256;
+  NH_CHECK(hipGetDevice(&dev));
+  NH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int64_t max_blocks = int64_t(cus) * 8;
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(a.m, kBlock), max_blocks));
+  const int64_t lanes = blocks * kBlock;
+  const int depth = std::max(1, ix.depth);
+  DevBuf<uint2> stack(size_t(lanes) * size_t(depth), s);
+  p.stack = stack.p;
+  p.stack_lanes = lanes;
+
+  if (ix.dim == 3)
+    dispatch_k<3>(p, unsigned(blocks), s);
+  else if (ix.dim == 2)
+    dispatch_k<2>(p, unsigned(blocks), s);
+  else
+    dispatch_k<0>(p, unsigned(blocks), s);
+  NH_LAUNCH_CHECK();
+}
+
+}  // namespace nabo_hip

@@ -1,0 +1,169 @@
+"""GPU parity of the HIP kd-tree (KDTREE_HIP) against the oracle.
+
+Bar: the GPU-built tree is the oracle's tree node for node (same cut
+dimensions, cut values, child links, bucket order); knn indices and dists2
+are bit-identical to the oracle's linear-heap search, for epsilon 0 and > 0,
+every flag combination, scalar and per-query radii, and the
+TOUCH_STATISTICS count matches.
+"""
+import numpy as np
+import pytest
+
+from libnabo_amd import datasets
+from tests.helpers import assert_bit_exact, exhaustive_queries, random_box_queries
+
+pytestmark = pytest.mark.gpu
+
+
+def make(nabo, cloud, bucket=8, stats=False, dim=None):
+    return nabo.NearestNeighbourSearch(
+        cloud, nabo.SearchType.KDTREE_HIP, dim if dim is not None else 2**31 - 1,
+        nabo.CreationOptionFlags.TOUCH_STATISTICS if stats else 0, {"bucketSize": bucket})
+
+
+def assert_same_tree(nns, otree):
+    nodes, bucket = nns.tree()
+    onodes, obucket, omin, omax = otree.export()
+    assert nodes.shape == onodes.shape, f"node count {nodes.shape[0]} vs {onodes.shape[0]}"
+    assert np.array_equal(nodes[:, 0], onodes[:, 0]), "node words differ"
+    # cut values compared as floats (+0 == -0), bucket starts as integers
+    dim_mask = (1 << otree.dim_bits) - 1
+    leaf = (onodes[:, 0] & dim_mask) == otree.dim
+    assert np.array_equal(nodes[leaf, 1], onodes[leaf, 1]), "bucket starts differ"
+    assert np.array_equal(nodes[~leaf, 1].view(np.float32), onodes[~leaf, 1].view(np.float32)), \
+        "cut values differ"
+    assert np.array_equal(bucket, obucket), "bucket order differs"
+    lo, hi = nns.bounds
+    assert np.array_equal(lo, omin) and np.array_equal(hi, omax), "bounds differ"
+    assert nns.info().depth == otree.depth
+
+
+CLOUDS = {
+    "uniform10k": lambda: datasets.uniform_cube(10_000, 10_000, seed=1),
+    "scan3d": lambda: (datasets.load_reference_scan("scan.3d"),
+                       random_box_queries(datasets.load_reference_scan("scan.3d"), 3000, 1)),
+    "scan2d": lambda: (datasets.load_reference_scan("scan.2d"),
+                       exhaustive_queries(datasets.load_reference_scan("scan.2d"))),
+    "lidar60k": lambda: datasets.lidar_scene(60_000, seed=2),
+}
+
+
+@pytest.mark.parametrize("name", list(CLOUDS))
+@pytest.mark.parametrize("bucket", [8, 2, 17])
+def test_tree_identical(nabo, oracle, name, bucket):
+    cloud, _ = CLOUDS[name]()
+    print(f"case tree {name} bucket={bucket}", flush=True)
+    nns = make(nabo, cloud, bucket)
+    assert_same_tree(nns, oracle.kdtree(cloud, bucket_size=bucket))
+
+
+def test_tree_duplicates_and_ties(nabo, oracle, rng):
+    # integer grid with many repeated coordinates: exercises the == cut block
+    cloud = rng.integers(0, 6, size=(5000, 3)).astype(np.float32)
+    print("case tree duplicates", flush=True)
+    nns = make(nabo, cloud, 8)
+    assert_same_tree(nns, oracle.kdtree(cloud, bucket_size=8))
+
+
+def test_tree_negative_coordinates(nabo, oracle, rng):
+    # all coordinates negative: maxBound stays at FLT_MIN (nabo.cpp:74)
+    cloud = (-1.0 - rng.random((3000, 3))).astype(np.float32)
+    print("case tree negative", flush=True)
+    nns = make(nabo, cloud, 8)
+    assert_same_tree(nns, oracle.kdtree(cloud, bucket_size=8))
+
+
+@pytest.mark.parametrize("n", [1, 5, 8, 9, 17])
+def test_tree_tiny(nabo, oracle, rng, n):
+    cloud = rng.random((n, 3)).astype(np.float32)
+    print(f"case tree tiny n={n}", flush=True)
+    nns = make(nabo, cloud, 8)
+    assert_same_tree(nns, oracle.kdtree(cloud, bucket_size=8))
+
+
+@pytest.mark.parametrize("name", list(CLOUDS))
+@pytest.mark.parametrize("k", [1, 4, 10, 16])
+@pytest.mark.parametrize("eps", [0.0, 0.1])
+def test_knn_bit_exact(nabo, oracle, name, k, eps):
+    cloud, query = CLOUDS[name]()
+    print(f"case knn {name} k={k} eps={eps}", flush=True)
+    nns = make(nabo, cloud, 8, stats=True)
+    otree = oracle.kdtree(cloud, bucket_size=8)
+    for flags in (0, 1, 2, 3):
+        for radius in (np.inf, 0.05):
+            gi, gd = nns.knn(query, k, eps, flags, radius)
+            oi, od, touched, _ = otree.knn(query, k, eps, flags, radius, stats=True)
+            assert_bit_exact(gi, gd, oi, od, f"{name} k={k} eps={eps} flags={flags} r={radius}")
+            assert nns.last_touched == int(touched.sum())
+
+
+def test_knn_self_queries(nabo, oracle):
+    cloud, _ = datasets.lidar_scene(60_000, seed=5)
+    nns = make(nabo, cloud, 8)
+    otree = oracle.kdtree(cloud, bucket_size=8)
+    print("case knn self", flush=True)
+    for flags in (0, 1):
+        gi, gd = nns.knn(cloud, 8, 0.0, flags)
+        oi, od = otree.knn(cloud, 8, 0.0, flags)
+        assert_bit_exact(gi, gd, oi, od, f"self flags={flags}")
+    # examples/usage.cpp:37-57: self match allowed -> first neighbour is the point
+    gi, _ = nns.knn(cloud, 3, 0.0, 1 | 2)
+    assert np.mean(gi[:, 0] == np.arange(len(cloud))) > 0.999
+
+
+def test_knn_per_query_radii(nabo, oracle, rng):
+    cloud, query = datasets.uniform_cube(20_000, 5_000, seed=9)
+    radii = rng.uniform(0.0, 0.08, size=len(query)).astype(np.float32)
+    nns = make(nabo, cloud, 8)
+    otree = oracle.kdtree(cloud, bucket_size=8)
+    print("case knn radii", flush=True)
+    gi, gd = nns.knn(query, 12, 0.0, 0, maxRadii=radii)
+    oi, od = otree.knn(query, 12, 0.0, 0, radii=radii)
+    assert_bit_exact(gi, gd, oi, od, "radii")
+
+
+@pytest.mark.parametrize("k", [32, 33, 64, 100])
+def test_knn_large_k(nabo, oracle, k):
+    cloud, query = datasets.uniform_cube(8_000, 1_500, seed=11)
+    nns = make(nabo, cloud, 8)
+    otree = oracle.kdtree(cloud, bucket_size=8)
+    print(f"case knn large k={k}", flush=True)
+    gi, gd = nns.knn(query, k, 0.0, 0)
+    oi, od = otree.knn(query, k, 0.0, 0)
+    assert_bit_exact(gi, gd, oi, od, f"k={k}")
+
+
+def test_knn_high_dim_generic(nabo, oracle, rng):
+    cloud = rng.random((6000, 7)).astype(np.float32)
+    query = rng.random((800, 7)).astype(np.float32)
+    nns = make(nabo, cloud, 8)
+    otree = oracle.kdtree(cloud, bucket_size=8)
+    print("case knn dim7", flush=True)
+    assert_same_tree(nns, otree)
+    gi, gd = nns.knn(query, 5, 0.0, 0)
+    oi, od = otree.knn(query, 5, 0.0, 0)
+    assert_bit_exact(gi, gd, oi, od, "dim 7")
+
+
+def test_knn_dim_subset(nabo, oracle, rng):
+    # cloud has 4 rows, search uses dim=3 (nabo.cpp:71 clamps dim to rows)
+    cloud = rng.random((7000, 4)).astype(np.float32)
+    query = rng.random((900, 4)).astype(np.float32)
+    nns = make(nabo, cloud, 8, dim=3)
+    otree = oracle.kdtree(cloud, dim=3, bucket_size=8)
+    print("case knn dim subset", flush=True)
+    assert_same_tree(nns, otree)
+    gi, gd = nns.knn(query, 6, 0.0, 0)
+    oi, od = otree.knn(query, 6, 0.0, 0)
+    assert_bit_exact(gi, gd, oi, od, "dim subset")
+
+
+def test_knn_no_query_sort(nabo, oracle):
+    cloud, query = datasets.uniform_cube(10_000, 4_000, seed=3)
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0,
+                                      {"bucketSize": 8, "querySort": 0})
+    otree = oracle.kdtree(cloud, bucket_size=8)
+    print("case knn unsorted queries", flush=True)
+    gi, gd = nns.knn(query, 10, 0.0, 0)
+    oi, od = otree.knn(query, 10, 0.0, 0)
+    assert_bit_exact(gi, gd, oi, od, "no sort")
