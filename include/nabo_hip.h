@@ -141,6 +141,13 @@ int nabo_hip_get_bounds(const nabo_hip_index* index, float* min_bound, float* ma
  * in leaf order (BucketEntry::index, nabo/nabo_private.h:158-168). */
 int nabo_hip_get_tree(const nabo_hip_index* index, uint32_t* nodes, int32_t* bucket_indices);
 
+/* Kernel timing for measurement tools. While enabled, each launch of the
+ * search kernels is bracketed by HIP events recorded on its own stream.
+ * collect() waits for the recorded events, returns the summed kernel time in
+ * milliseconds and the number of launches, and clears the record. */
+int nabo_hip_kernel_timing(int32_t enable);
+int nabo_hip_kernel_timing_collect(double* milliseconds, int64_t* launches);
+
 /* Number of HIP devices visible (0 when none). */
 int nabo_hip_device_count(int32_t* count);
 

@@ -97,6 +97,7 @@ EXPORTED = [
     "nabo_hip_last_error", "nabo_hip_params_default", "nabo_hip_create",
     "nabo_hip_create_device", "nabo_hip_destroy", "nabo_hip_knn", "nabo_hip_knn_device",
     "nabo_hip_get_info", "nabo_hip_get_bounds", "nabo_hip_get_tree", "nabo_hip_device_count",
+    "nabo_hip_kernel_timing", "nabo_hip_kernel_timing_collect",
 ]
 
 _LIB = None
@@ -126,6 +127,8 @@ def load_library(path: str = LIB_PATH):
     lib.nabo_hip_get_bounds.argtypes = [vp, vp, vp]
     lib.nabo_hip_get_tree.argtypes = [vp, vp, vp]
     lib.nabo_hip_device_count.argtypes = [C.POINTER(C.c_int32)]
+    lib.nabo_hip_kernel_timing.argtypes = [i32]
+    lib.nabo_hip_kernel_timing_collect.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     for name in EXPORTED:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.nabo_hip_last_error.restype = C.c_char_p
@@ -144,6 +147,21 @@ def device_count() -> int:
     c = C.c_int32(0)
     _check(lib, lib.nabo_hip_device_count(C.byref(c)))
     return c.value
+
+
+def kernel_timing(enable: bool) -> None:
+    """Brackets every search-kernel launch with HIP events on its stream."""
+    lib = load_library()
+    _check(lib, lib.nabo_hip_kernel_timing(1 if enable else 0))
+
+
+def kernel_timing_collect():
+    """(summed kernel milliseconds, launches) since the last collect."""
+    lib = load_library()
+    ms = C.c_double(0)
+    n = C.c_int64(0)
+    _check(lib, lib.nabo_hip_kernel_timing_collect(C.byref(ms), C.byref(n)))
+    return ms.value, n.value
 
 
 def _as_points(a, name: str) -> np.ndarray:

@@ -237,6 +237,7 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   DevBuf<int32_t> pi(size_t(p.chunks) * a.m * a.k, s);
   p.part_d = pd.p;
   p.part_i = pi.p;
+  KernelTimer timer(s);
   if (ix.dim <= 4)
     bf_dispatch_k<4>(p, s, a.out_idx, a.out_d2);
   else if (ix.dim <= 8)

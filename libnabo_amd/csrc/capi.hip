@@ -200,6 +200,31 @@ void throw_hip(hipError_t e, const char* what, const char* file, int line) {
                         std::to_string(line) + ")");
 }
 
+namespace {
+std::mutex g_timing_mu;
+bool g_timing = false;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_timing_events;
+}  // namespace
+
+bool kernel_timing_enabled() {
+  std::lock_guard<std::mutex> lock(g_timing_mu);
+  return g_timing;
+}
+
+KernelTimer::KernelTimer(hipStream_t stream) : s(stream) {
+  if (!kernel_timing_enabled()) return;
+  NH_CHECK(hipEventCreate(&start));
+  NH_CHECK(hipEventCreate(&stop));
+  NH_CHECK(hipEventRecord(start, s));
+}
+
+KernelTimer::~KernelTimer() {
+  if (!start) return;
+  if (hipEventRecord(stop, s) != hipSuccess) return;
+  std::lock_guard<std::mutex> lock(g_timing_mu);
+  g_timing_events.emplace_back(start, stop);
+}
+
 Index::~Index() {
   if (d_nodes) (void)hipFree(d_nodes);
   if (d_pts) (void)hipFree(d_pts);
@@ -224,6 +249,34 @@ void nabo_hip_params_default(nabo_hip_params* p) {
   p->bf_precision = NABO_HIP_BF_FP32;
   p->bf_extra = 32;
   p->device = -1;
+}
+
+int nabo_hip_kernel_timing(int32_t enable) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lock(g_timing_mu);
+    g_timing = enable != 0;
+  });
+}
+
+int nabo_hip_kernel_timing_collect(double* milliseconds, int64_t* launches) {
+  return guarded([&] {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    {
+      std::lock_guard<std::mutex> lock(g_timing_mu);
+      ev.swap(g_timing_events);
+    }
+    double total = 0.0;
+    for (auto& e : ev) {
+      NH_CHECK(hipEventSynchronize(e.second));
+      float ms = 0.f;
+      NH_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+      total += ms;
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    if (milliseconds) *milliseconds = total;
+    if (launches) *launches = int64_t(ev.size());
+  });
 }
 
 int nabo_hip_device_count(int32_t* count) {

@@ -114,6 +114,15 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s);
 void bf_prepare(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s);
 void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s);
 
+// Measurement hook: brackets a kernel launch with events when enabled.
+struct KernelTimer {
+  hipEvent_t start = nullptr, stop = nullptr;
+  hipStream_t s = nullptr;
+  explicit KernelTimer(hipStream_t stream);  // records start if timing is on
+  ~KernelTimer();                            // records stop and files the pair
+};
+bool kernel_timing_enabled();
+
 // Largest k the register-resident k-best arrays support.
 constexpr int kMaxK = 128;
 

@@ -378,13 +378,16 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   p.stack = stack.p;
   p.stack_lanes = lanes;
 
-  if (ix.dim == 3)
-    dispatch_k<3>(p, unsigned(blocks), s);
-  else if (ix.dim == 2)
-    dispatch_k<2>(p, unsigned(blocks), s);
-  else
-    dispatch_k<0>(p, unsigned(blocks), s);
-  NH_LAUNCH_CHECK();
+  {
+    KernelTimer timer(s);
+    if (ix.dim == 3)
+      dispatch_k<3>(p, unsigned(blocks), s);
+    else if (ix.dim == 2)
+      dispatch_k<2>(p, unsigned(blocks), s);
+    else
+      dispatch_k<0>(p, unsigned(blocks), s);
+    NH_LAUNCH_CHECK();
+  }
 }
 
 }  // namespace nabo_hip
