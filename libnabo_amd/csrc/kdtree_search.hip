@@ -41,6 +41,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint32_t kRestore = 0x80000000u;
 constexpr int kMaxGenericDim = 64;
+constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int kLeafBatch = 8;  // bucket entries loaded together
 
 // Coordinates / offsets of one query in VGPRs, indexed by a runtime cut
 // dimension through selects (DIM > 0), or a local array (DIM == 0).
@@ -132,17 +134,72 @@ __device__ __forceinline__ float point_dist(const SearchParams& a, const Vec<DIM
   }
 }
 
+// Scans one leaf bucket: loads up to kLeafBatch entries at once, then tests
+// them in bucket order (kdtree_cpu.cpp:380-398).
+template <int DIM, int K>
+__device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>& q, uint32_t count,
+                                          int64_t start, float r2, float (&hd)[K],
+                                          int32_t (&hi)[K]) {
+  if constexpr (DIM > 0 && DIM <= 3) {
+    const float4* pts4 = reinterpret_cast<const float4*>(a.pts) + start;
+    for (uint32_t j0 = 0; j0 < count; j0 += kLeafBatch) {
+      float4 pv[kLeafBatch];
+#pragma unroll
+      for (int j = 0; j < kLeafBatch; ++j)
+        if (j0 + j < count) pv[j] = pts4[j0 + j];
+#pragma unroll
+      for (int j = 0; j < kLeafBatch; ++j) {
+        if (j0 + j < count) {
+          const float c[4] = {pv[j].x, pv[j].y, pv[j].z, pv[j].w};
+          float dist = 0.f;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            const float diff = q.v[d] - c[d];
+            dist = dist + diff * diff;
+          }
+          if (dist <= r2 && dist < hd[K - 1] && (a.allow_self || dist > FLT_EPSILON))
+            heap_insert<K>(hd, hi, dist, __float_as_int(c[DIM]));
+        }
+      }
+    }
+  } else {
+    for (uint32_t j = 0; j < count; ++j) {
+      int32_t index;
+      const float dist = point_dist<DIM>(a, q, start + j, &index);
+      if (dist <= r2 && dist < hd[K - 1] && (a.allow_self || dist > FLT_EPSILON))
+        heap_insert<K>(hd, hi, dist, index);
+    }
+  }
+}
+
+// Work mapping: the queries (in leaf order) are cut into chunks of 64, one
+// per wave at a time. Workgroups are dispatched round-robin over the 8 XCDs,
+// so workgroup b runs on XCD b % 8; each XCD is given one contiguous eighth
+// of the chunks, which keeps the tree region it touches small enough for
+// its own L2.
 template <int DIM, int K>
 __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  constexpr int kWavesPerBlock = kBlock / kWave;
   const int64_t lane_id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t lanes = int64_t(gridDim.x) * blockDim.x;
+  const int xcd = blockIdx.x % kXcds;
+  const int64_t local = blockIdx.x / kXcds;
+  const int64_t xcd_blocks = gridDim.x / kXcds;
+  const int64_t chunks = (a.m + kWave - 1) / kWave;
+  const int64_t c_begin = chunks * xcd / kXcds;
+  const int64_t c_end = chunks * (xcd + 1) / kXcds;
+
   const int dim = DIM > 0 ? DIM : a.dim;
   const uint32_t leaf_tag = uint32_t(dim);
   uint2* stack = a.stack + lane_id;
   const int64_t sstride = a.stack_lanes;
   uint64_t touched = 0;
 
-  for (int64_t i = lane_id; i < a.m; i += lanes) {
+  for (int64_t c = c_begin + local * kWavesPerBlock + wave; c < c_end;
+       c += xcd_blocks * kWavesPerBlock) {
+    const int64_t i = c * kWave + lane;
+    if (i >= a.m) continue;
     const int64_t qi = a.order ? a.order[i] : i;
     Vec<DIM> q, off;
     const float* qp = a.query + qi * a.qstride;
@@ -173,19 +230,17 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
         const float new_off = q.get(cd) - __uint_as_float(nd.y);
         const float old_off = off.get(cd);
         const float rd_far = rd + (-old_off * old_off + new_off * new_off);
-        stack[sp * sstride] = make_uint2(n, __float_as_uint(rd_far));
-        ++sp;
+        // The far child is tested again after the near subtree, against a
+        // head that can only have shrunk: a test failed now fails then.
+        if (rd_far <= r2 && rd_far * a.max_err2 < hd[K - 1]) {
+          stack[sp * sstride] = make_uint2(n, __float_as_uint(rd_far));
+          ++sp;
+        }
         n = new_off > 0 ? (nd.x >> a.dim_bits) : n + 1;
         continue;
       }
       const uint32_t count = nd.x >> a.dim_bits;
-      const int64_t start = nd.y;
-      for (uint32_t j = 0; j < count; ++j) {
-        int32_t index;
-        const float dist = point_dist<DIM>(a, q, start + j, &index);
-        if (dist <= r2 && dist < hd[K - 1] && (a.allow_self || dist > FLT_EPSILON))
-          heap_insert<K>(hd, hi, dist, index);
-      }
+      scan_leaf<DIM, K>(a, q, count, int64_t(nd.y), r2, hd, hi);
       touched += count;
 
       bool descend = false;
@@ -227,8 +282,6 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
 
   if (a.touched) {
     uint32_t lo = uint32_t(touched), hi32 = uint32_t(touched >> 32);
-    uint64_t sum = 0;
-    // 64-bit wave sum in two halves with carry through a 64-bit accumulate.
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       const uint32_t olo = (uint32_t)__shfl_xor((int)lo, o, kWave);
@@ -237,8 +290,8 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
       lo = uint32_t(t);
       hi32 = uint32_t(t >> 32);
     }
-    sum = (uint64_t(hi32) << 32) | lo;
-    if (__lane_id() == 0 && sum) atomicAdd(a.touched, (unsigned long long)sum);
+    const uint64_t sum = (uint64_t(hi32) << 32) | lo;
+    if (lane == 0 && sum) atomicAdd(a.touched, (unsigned long long)sum);
   }
 }
 
@@ -371,7 +424,8 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   NH_CHECK(hipGetDevice(&dev));
   NH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int64_t max_blocks = int64_t(cus) * 8;
-  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(a.m, kBlock), max_blocks));
+  int64_t blocks = std::min<int64_t>(ceil_div(a.m, kBlock), max_blocks);
+  blocks = std::max<int64_t>(kXcds, ceil_div(blocks, kXcds) * kXcds);  // multiple of the XCDs
   const int64_t lanes = blocks * kBlock;
   const int depth = std::max(1, ix.depth);
   DevBuf<uint2> stack(size_t(lanes) * size_t(depth), s);
