@@ -92,19 +92,21 @@ struct SearchParams {
 
 // Sorted insertion of (value, index): entries greater than value move up by
 // one, value lands after the entries equal to it (index_heap.h:322-334).
+// With hd ascending, the new entry j is hd[j-1] if that is greater than
+// value, else min(hd[j], value): the median of (hd[j-1], hd[j], value).
 template <int K>
 __device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], float value,
                                             int32_t index) {
+  bool gt[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) gt[j] = hd[j] > value;
 #pragma unroll
   for (int j = K - 1; j > 0; --j) {
-    const bool shift = hd[j - 1] > value;
-    const bool place = !shift && hd[j] > value;
-    hd[j] = shift ? hd[j - 1] : (place ? value : hd[j]);
-    hi[j] = shift ? hi[j - 1] : (place ? index : hi[j]);
+    hi[j] = gt[j - 1] ? hi[j - 1] : (gt[j] ? index : hi[j]);
+    hd[j] = __builtin_amdgcn_fmed3f(hd[j - 1], hd[j], value);
   }
-  const bool place0 = hd[0] > value;
-  hd[0] = place0 ? value : hd[0];
-  hi[0] = place0 ? index : hi[0];
+  hi[0] = gt[0] ? index : hi[0];
+  hd[0] = fminf(hd[0], value);
 }
 
 template <int DIM>
@@ -224,9 +226,12 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
     float rd = 0.f;
     int sp = 0;
     for (;;) {
-      const uint2 nd = a.nodes[n];
-      const uint32_t cd = nd.x & a.dim_mask;
-      if (cd != leaf_tag) {
+      // Descend to a leaf. The wave leaves this loop only when every lane
+      // stands on a leaf, so the leaf scan below runs once for all lanes
+      // instead of once per node step of the slowest lane.
+      uint2 nd = a.nodes[n];
+      while ((nd.x & a.dim_mask) != leaf_tag) {
+        const uint32_t cd = nd.x & a.dim_mask;
         const float new_off = q.get(cd) - __uint_as_float(nd.y);
         const float old_off = off.get(cd);
         const float rd_far = rd + (-old_off * old_off + new_off * new_off);
@@ -237,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
           ++sp;
         }
         n = new_off > 0 ? (nd.x >> a.dim_bits) : n + 1;
-        continue;
+        nd = a.nodes[n];
       }
       const uint32_t count = nd.x >> a.dim_bits;
       scan_leaf<DIM, K>(a, q, count, int64_t(nd.y), r2, hd, hi);
