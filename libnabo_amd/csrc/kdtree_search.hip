@@ -46,18 +46,23 @@ constexpr int kLeafBatch = 8;  // bucket entries loaded together
 
 // Coordinates / offsets of one query in VGPRs, indexed by a runtime cut
 // dimension through selects (DIM > 0), or a local array (DIM == 0).
+// The selects are written as bit blends: a select chain over an array is
+// turned back into an indexed private array (and then into LDS) otherwise.
 template <int DIM>
 struct Vec {
   float v[DIM];
   __device__ __forceinline__ float get(uint32_t i) const {
-    float r = v[0];
+    uint32_t r = 0;
 #pragma unroll
-    for (int d = 1; d < DIM; ++d) r = (i == uint32_t(d)) ? v[d] : r;
-    return r;
+    for (int d = 0; d < DIM; ++d) r |= __float_as_uint(v[d]) & (0u - uint32_t(i == uint32_t(d)));
+    return __uint_as_float(r);
   }
   __device__ __forceinline__ void set(uint32_t i, float x) {
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) v[d] = (i == uint32_t(d)) ? x : v[d];
+    for (int d = 0; d < DIM; ++d) {
+      const uint32_t m = 0u - uint32_t(i == uint32_t(d));
+      v[d] = __uint_as_float((__float_as_uint(x) & m) | (__float_as_uint(v[d]) & ~m));
+    }
   }
 };
 template <>
@@ -94,18 +99,20 @@ struct SearchParams {
 // one, value lands after the entries equal to it (index_heap.h:322-334).
 // With hd ascending, the new entry j is hd[j-1] if that is greater than
 // value, else min(hd[j], value): the median of (hd[j-1], hd[j], value).
+// Indices follow with two selects per entry; both compares are taken on
+// the old distances, so the update has no branch.
 template <int K>
 __device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], float value,
                                             int32_t index) {
-  bool gt[K];
+  int32_t moved[K];
 #pragma unroll
-  for (int j = 0; j < K; ++j) gt[j] = hd[j] > value;
+  for (int j = 0; j < K; ++j) moved[j] = hd[j] > value ? index : hi[j];
 #pragma unroll
   for (int j = K - 1; j > 0; --j) {
-    hi[j] = gt[j - 1] ? hi[j - 1] : (gt[j] ? index : hi[j]);
+    hi[j] = hd[j - 1] > value ? hi[j - 1] : moved[j];
     hd[j] = __builtin_amdgcn_fmed3f(hd[j - 1], hd[j], value);
   }
-  hi[0] = gt[0] ? index : hi[0];
+  hi[0] = moved[0];
   hd[0] = fminf(hd[0], value);
 }
 
@@ -194,8 +201,11 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
 
   const int dim = DIM > 0 ? DIM : a.dim;
   const uint32_t leaf_tag = uint32_t(dim);
-  uint2* stack = a.stack + lane_id;
-  const int64_t sstride = a.stack_lanes;
+  // Slot s of this lane's stack is a.stack[s * lanes + lane_id]; the
+  // running offset stays in 32 bits (lanes * depth < 2^32 is checked on the
+  // host).
+  const uint32_t base_off = uint32_t(lane_id);
+  const uint32_t sstride = uint32_t(a.stack_lanes);
   uint64_t touched = 0;
 
   for (int64_t c = c_begin + local * kWavesPerBlock + wave; c < c_end;
@@ -224,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
 
     uint32_t n = 0;
     float rd = 0.f;
-    int sp = 0;
+    uint32_t top = base_off;  // offset of the next free slot
     for (;;) {
       // Descend to a leaf. The wave leaves this loop only when every lane
       // stands on a leaf, so the leaf scan below runs once for all lanes
@@ -238,8 +248,8 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
         // The far child is tested again after the near subtree, against a
         // head that can only have shrunk: a test failed now fails then.
         if (rd_far <= r2 && rd_far * a.max_err2 < hd[K - 1]) {
-          stack[sp * sstride] = make_uint2(n, __float_as_uint(rd_far));
-          ++sp;
+          a.stack[top] = make_uint2(n, __float_as_uint(rd_far));
+          top += sstride;
         }
         n = new_off > 0 ? (nd.x >> a.dim_bits) : n + 1;
         nd = a.nodes[n];
@@ -249,9 +259,9 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
       touched += count;
 
       bool descend = false;
-      while (sp > 0) {
-        --sp;
-        const uint2 e = stack[sp * sstride];
+      while (top != base_off) {
+        top -= sstride;
+        const uint2 e = a.stack[top];
         if (e.x & kRestore) {
           off.set(e.x & ~kRestore, __uint_as_float(e.y));
           continue;
@@ -262,8 +272,8 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
           const uint32_t pcd = pn.x & a.dim_mask;
           const float new_off = q.get(pcd) - __uint_as_float(pn.y);
           const uint32_t far = new_off > 0 ? e.x + 1 : (pn.x >> a.dim_bits);
-          stack[sp * sstride] = make_uint2(kRestore | pcd, __float_as_uint(off.get(pcd)));
-          ++sp;
+          a.stack[top] = make_uint2(kRestore | pcd, __float_as_uint(off.get(pcd)));
+          top += sstride;
           off.set(pcd, new_off);
           n = far;
           rd = rdf;
@@ -433,6 +443,8 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   blocks = std::max<int64_t>(kXcds, ceil_div(blocks, kXcds) * kXcds);  // multiple of the XCDs
   const int64_t lanes = blocks * kBlock;
   const int depth = std::max(1, ix.depth);
+  if (uint64_t(lanes) * uint64_t(depth + 1) >= (uint64_t(1) << 32))
+    throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
   DevBuf<uint2> stack(size_t(lanes) * size_t(depth), s);
   p.stack = stack.p;
   p.stack_lanes = lanes;
