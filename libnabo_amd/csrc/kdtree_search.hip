@@ -43,6 +43,7 @@ constexpr uint32_t kRestore = 0x80000000u;
 constexpr int kMaxGenericDim = 64;
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kLeafBatch = 8;  // bucket entries loaded together
+constexpr int kLdsStackSlots = 32;  // deepest tree whose stacks fit in LDS
 
 // Coordinates / offsets of one query in VGPRs, indexed by a runtime cut
 // dimension through selects (DIM > 0), or a local array (DIM == 0).
@@ -186,8 +187,12 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
 // so workgroup b runs on XCD b % 8; each XCD is given one contiguous eighth
 // of the chunks, which keeps the tree region it touches small enough for
 // its own L2.
-template <int DIM, int K>
+template <int DIM, int K, bool LDS_STACK>
 __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
+  // Per-lane stack: in LDS when the tree is shallow enough ([slot][lane] of
+  // this workgroup), else in HBM ([slot][lane] over the grid).
+  extern __shared__ uint2 lds_stack[];
+  uint2* const stack_base = LDS_STACK ? lds_stack : a.stack;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   constexpr int kWavesPerBlock = kBlock / kWave;
@@ -204,8 +209,8 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
   // Slot s of this lane's stack is a.stack[s * lanes + lane_id]; the
   // running offset stays in 32 bits (lanes * depth < 2^32 is checked on the
   // host).
-  const uint32_t base_off = uint32_t(lane_id);
-  const uint32_t sstride = uint32_t(a.stack_lanes);
+  const uint32_t base_off = LDS_STACK ? uint32_t(threadIdx.x) : uint32_t(lane_id);
+  const uint32_t sstride = LDS_STACK ? uint32_t(kBlock) : uint32_t(a.stack_lanes);
   uint64_t touched = 0;
 
   for (int64_t c = c_begin + local * kWavesPerBlock + wave; c < c_end;
@@ -248,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
         // The far child is tested again after the near subtree, against a
         // head that can only have shrunk: a test failed now fails then.
         if (rd_far <= r2 && rd_far * a.max_err2 < hd[K - 1]) {
-          a.stack[top] = make_uint2(n, __float_as_uint(rd_far));
+          stack_base[top] = make_uint2(n, __float_as_uint(rd_far));
           top += sstride;
         }
         n = new_off > 0 ? (nd.x >> a.dim_bits) : n + 1;
@@ -261,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
       bool descend = false;
       while (top != base_off) {
         top -= sstride;
-        const uint2 e = a.stack[top];
+        const uint2 e = stack_base[top];
         if (e.x & kRestore) {
           off.set(e.x & ~kRestore, __uint_as_float(e.y));
           continue;
@@ -272,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
           const uint32_t pcd = pn.x & a.dim_mask;
           const float new_off = q.get(pcd) - __uint_as_float(pn.y);
           const uint32_t far = new_off > 0 ? e.x + 1 : (pn.x >> a.dim_bits);
-          a.stack[top] = make_uint2(kRestore | pcd, __float_as_uint(off.get(pcd)));
+          stack_base[top] = make_uint2(kRestore | pcd, __float_as_uint(off.get(pcd)));
           top += sstride;
           off.set(pcd, new_off);
           n = far;
@@ -316,7 +321,7 @@ template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nodes,
                                                    uint32_t dim_bits, uint32_t dim_mask, int dim_rt,
                                                    const float* __restrict__ query,
-                                                   int64_t qstride, int64_t m,
+                                                   int64_t qstride, int64_t m, uint32_t shift,
                                                    uint32_t* __restrict__ keys,
                                                    int32_t* __restrict__ vals) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -328,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nod
     const uint2 nd = nodes[n];
     const uint32_t cd = nd.x & dim_mask;
     if (cd == uint32_t(dim)) {
-      keys[i] = nd.y;
+      keys[i] = nd.y >> shift;
       break;
     }
     const float new_off = qp[cd] - __uint_as_float(nd.y);
@@ -338,29 +343,34 @@ __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nod
 }
 
 template <int DIM, int K>
-void launch_search(const SearchParams& p, unsigned grid, hipStream_t s) {
-  k_kd_search<DIM, K><<<grid, kBlock, 0, s>>>(p);
+void launch_search(const SearchParams& p, unsigned grid, int depth, hipStream_t s) {
+  if (depth <= kLdsStackSlots) {
+    const size_t lds = size_t(depth) * kBlock * sizeof(uint2);
+    k_kd_search<DIM, K, true><<<grid, kBlock, lds, s>>>(p);
+  } else {
+    k_kd_search<DIM, K, false><<<grid, kBlock, 0, s>>>(p);
+  }
 }
 
 template <int DIM>
-void dispatch_k(const SearchParams& p, unsigned grid, hipStream_t s) {
+void dispatch_k(const SearchParams& p, unsigned grid, int depth, hipStream_t s) {
   const int k = p.k;
   if (k <= 1)
-    launch_search<DIM, 1>(p, grid, s);
+    launch_search<DIM, 1>(p, grid, depth, s);
   else if (k <= 2)
-    launch_search<DIM, 2>(p, grid, s);
+    launch_search<DIM, 2>(p, grid, depth, s);
   else if (k <= 4)
-    launch_search<DIM, 4>(p, grid, s);
+    launch_search<DIM, 4>(p, grid, depth, s);
   else if (k <= 8)
-    launch_search<DIM, 8>(p, grid, s);
+    launch_search<DIM, 8>(p, grid, depth, s);
   else if (k <= 16)
-    launch_search<DIM, 16>(p, grid, s);
+    launch_search<DIM, 16>(p, grid, depth, s);
   else if (k <= 32)
-    launch_search<DIM, 32>(p, grid, s);
+    launch_search<DIM, 32>(p, grid, depth, s);
   else if (k <= 64)
-    launch_search<DIM, 64>(p, grid, s);
+    launch_search<DIM, 64>(p, grid, depth, s);
   else
-    launch_search<DIM, 128>(p, grid, s);
+    launch_search<DIM, 128>(p, grid, depth, s);
 }
 
 uint32_t bits_needed(uint64_t v) {
@@ -413,18 +423,22 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     keys_sorted.alloc(size_t(a.m), s);
     vals.alloc(size_t(a.m), s);
     order.alloc(size_t(a.m), s);
+    // Key: the leaf's bucket start, cut to 16 bits (two radix passes); the
+    // dropped low bits only reorder queries of neighbouring leaves.
+    const uint32_t full_bits = std::max(1u, bits_needed(uint64_t(ix.n)));
+    const uint32_t shift = full_bits > 16 ? full_bits - 16 : 0;
     const unsigned g = unsigned(ceil_div(a.m, kBlock));
     if (ix.dim == 3)
       k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, keys.p, vals.p);
+                                       a.qstride, a.m, shift, keys.p, vals.p);
     else if (ix.dim == 2)
       k_locate<2><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, keys.p, vals.p);
+                                       a.qstride, a.m, shift, keys.p, vals.p);
     else
       k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, keys.p, vals.p);
+                                       a.qstride, a.m, shift, keys.p, vals.p);
     NH_LAUNCH_CHECK();
-    const unsigned end_bit = std::max(1u, bits_needed(uint64_t(ix.n)));
+    const unsigned end_bit = full_bits - shift;
     size_t bytes = 0;
     NH_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, keys.p, keys_sorted.p, vals.p, order.p,
                                        size_t(a.m), 0, end_bit, s));
@@ -445,18 +459,19 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   const int depth = std::max(1, ix.depth);
   if (uint64_t(lanes) * uint64_t(depth + 1) >= (uint64_t(1) << 32))
     throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
-  DevBuf<uint2> stack(size_t(lanes) * size_t(depth), s);
+  DevBuf<uint2> stack;
+  if (depth > kLdsStackSlots) stack.alloc(size_t(lanes) * size_t(depth), s);
   p.stack = stack.p;
   p.stack_lanes = lanes;
 
   {
     KernelTimer timer(s);
     if (ix.dim == 3)
-      dispatch_k<3>(p, unsigned(blocks), s);
+      dispatch_k<3>(p, unsigned(blocks), depth, s);
     else if (ix.dim == 2)
-      dispatch_k<2>(p, unsigned(blocks), s);
+      dispatch_k<2>(p, unsigned(blocks), depth, s);
     else
-      dispatch_k<0>(p, unsigned(blocks), s);
+      dispatch_k<0>(p, unsigned(blocks), depth, s);
     NH_LAUNCH_CHECK();
   }
 }
