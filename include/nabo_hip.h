@@ -114,6 +114,31 @@ int nabo_hip_knn_device(const nabo_hip_index* index, const float* d_query, int32
                         int32_t* d_indices, float* d_dists2, uint64_t* d_touched,
                         void* hip_stream);
 
+/* Batched kd-trees: `clouds` independent clouds concatenated in one device
+ * buffer (cloud c = columns [offsets[c], offsets[c+1]), host offsets array
+ * of clouds + 1 entries), all built in one pass on the GPU. Tree c is node
+ * for node the tree nabo_hip_create builds for cloud c alone. No counterpart
+ * in the reference, which builds one tree per NearestNeighbourSearch object
+ * (nabo/kdtree_cpu.cpp:227-272); for callers with many small clouds. */
+int nabo_hip_batch_create(const float* d_points, const int64_t* offsets, int32_t clouds,
+                          int32_t rows, int64_t stride, int32_t dim, uint32_t creation_flags,
+                          const nabo_hip_params* params, void* hip_stream,
+                          nabo_hip_index** out);
+
+/* kNN on batched trees, one search launch for every cloud: d_query holds m
+ * queries, query set c = columns [d_query_offsets[c], d_query_offsets[c+1])
+ * (device array of clouds + 1 entries, last = m) searched in tree c.
+ * Outputs are k x m; indices are local to each cloud. Semantics per set are
+ * those of nabo_hip_knn_device on cloud c's own index. */
+int nabo_hip_batch_knn(const nabo_hip_index* index, const float* d_query, int32_t qrows,
+                       int64_t qstride, int64_t m, const int64_t* d_query_offsets, int32_t k,
+                       float epsilon, uint32_t option_flags, float max_radius,
+                       int32_t* d_indices, float* d_dists2, uint64_t* d_touched,
+                       void* hip_stream);
+
+/* Root node of each batched tree (clouds entries; {0} for a single tree). */
+int nabo_hip_batch_roots(const nabo_hip_index* index, int32_t* roots);
+
 /* Read-only properties of an index. */
 typedef struct nabo_hip_info {
   int64_t n;            /* points */
@@ -127,12 +152,14 @@ typedef struct nabo_hip_info {
   int64_t device_bytes; /* HBM held by the index */
   uint32_t creation_flags;
   int32_t device;
+  int32_t clouds;       /* trees held (1 unless batched) */
 } nabo_hip_info;
 
 int nabo_hip_get_info(const nabo_hip_index* index, nabo_hip_info* info);
 
 /* Copies the bounding box (minBound/maxBound of nabo/nabo.h:276-279, with the
- * reference's initial values when the tree is a single bucket) to host. */
+ * reference's initial values when the tree is a single bucket) to host;
+ * clouds x dim values each for batched trees. */
 int nabo_hip_get_bounds(const nabo_hip_index* index, float* min_bound, float* max_bound);
 
 /* Copies the kd-tree to host: nodes as 2 x node_count uint32 (the node word

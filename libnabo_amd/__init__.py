@@ -90,6 +90,7 @@ class Info(C.Structure):
         ("device_bytes", C.c_int64),
         ("creation_flags", C.c_uint32),
         ("device", C.c_int32),
+        ("clouds", C.c_int32),
     ]
 
 
@@ -97,7 +98,8 @@ EXPORTED = [
     "nabo_hip_last_error", "nabo_hip_params_default", "nabo_hip_create",
     "nabo_hip_create_device", "nabo_hip_destroy", "nabo_hip_knn", "nabo_hip_knn_device",
     "nabo_hip_get_info", "nabo_hip_get_bounds", "nabo_hip_get_tree", "nabo_hip_device_count",
-    "nabo_hip_kernel_timing", "nabo_hip_kernel_timing_collect",
+    "nabo_hip_kernel_timing", "nabo_hip_kernel_timing_collect", "nabo_hip_batch_create",
+    "nabo_hip_batch_knn", "nabo_hip_batch_roots",
 ]
 
 _LIB = None
@@ -110,6 +112,14 @@ def load_library(path: str = LIB_PATH):
         return _LIB
     if not os.path.exists(path):
         raise NaboError(2, f"{path} is missing: build it with `python -m libnabo_amd.build`")
+    # torch bundles its own HIP runtime under the same soname
+    # (libamdhip64.so.7); whichever is loaded first serves the whole process.
+    # Load torch's first when torch is present, so that torch tensors and
+    # streams and this library share one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     vp, i32, i64, u32, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_float
     lib.nabo_hip_last_error.restype = C.c_char_p
@@ -127,6 +137,11 @@ def load_library(path: str = LIB_PATH):
     lib.nabo_hip_get_bounds.argtypes = [vp, vp, vp]
     lib.nabo_hip_get_tree.argtypes = [vp, vp, vp]
     lib.nabo_hip_device_count.argtypes = [C.POINTER(C.c_int32)]
+    lib.nabo_hip_batch_create.argtypes = [vp, vp, i32, i32, i64, i32, u32, C.POINTER(Params), vp,
+                                          C.POINTER(vp)]
+    lib.nabo_hip_batch_knn.argtypes = [vp, vp, i32, i64, i64, vp, i32, f32, u32, f32, vp, vp, vp,
+                                       vp]
+    lib.nabo_hip_batch_roots.argtypes = [vp, vp]
     lib.nabo_hip_kernel_timing.argtypes = [i32]
     lib.nabo_hip_kernel_timing_collect.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     for name in EXPORTED:
@@ -307,3 +322,103 @@ class NearestNeighbourSearch:
             C.c_void_p(indices.data_ptr()), C.c_void_p(dists2.data_ptr()),
             None if touched is None else C.c_void_p(touched.data_ptr()),
             C.c_void_p(stream.cuda_stream)))
+
+
+def _stream_handle(stream, device):
+    import torch
+
+    if stream is None:
+        stream = torch.cuda.current_stream(device)
+    return C.c_void_p(stream.cuda_stream)
+
+
+class BatchSearch:
+    """Many independent clouds: one GPU build pass, one search launch.
+
+    ``points``: torch float32 CUDA tensor (total, rows), contiguous, the
+    clouds one after the other; ``offsets``: clouds + 1 point offsets. Tree c
+    is the KDTREE_HIP tree of cloud c alone; results of a query set are those
+    of that cloud's own search, with indices local to the cloud.
+    """
+
+    def __init__(self, points, offsets, dim: int = 2**31 - 1, creationOptionFlags: int = 0,
+                 params: dict | None = None, stream=None):
+        self._lib = load_library()
+        self._h = None
+        kw = {}
+        for key, val in (params or {}).items():
+            if key not in _PARAM_KEYS:
+                raise NaboError(1, f"unknown parameter {key!r}")
+            kw[_PARAM_KEYS[key]] = val
+        p = make_params(**kw)
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        total, rows = points.shape
+        if int(off[-1]) != total:
+            raise NaboError(1, "the last offset must equal the number of points")
+        h = C.c_void_p()
+        _check(self._lib, self._lib.nabo_hip_batch_create(
+            C.c_void_p(points.data_ptr()), _ptr(off), len(off) - 1, rows, points.stride(0),
+            int(dim), int(creationOptionFlags), C.byref(p),
+            _stream_handle(stream, points.device), C.byref(h)))
+        self._h = h
+        self.offsets = off
+        self.device = points.device
+
+    def close(self):
+        if self._h:
+            self._lib.nabo_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def clouds(self) -> int:
+        return len(self.offsets) - 1
+
+    def info(self) -> Info:
+        inf = Info()
+        _check(self._lib, self._lib.nabo_hip_get_info(self._h, C.byref(inf)))
+        return inf
+
+    def roots(self) -> np.ndarray:
+        r = np.empty(self.clouds, np.int32)
+        _check(self._lib, self._lib.nabo_hip_batch_roots(self._h, _ptr(r)))
+        return r
+
+    def tree(self):
+        """(nodes[count, 2] uint32 with global child ids, cloud-local bucket indices[n])."""
+        inf = self.info()
+        nodes = np.empty((inf.node_count, 2), np.uint32)
+        bucket = np.empty(inf.n, np.int32)
+        _check(self._lib, self._lib.nabo_hip_get_tree(self._h, _ptr(nodes), _ptr(bucket)))
+        return nodes, bucket
+
+    def bounds(self):
+        inf = self.info()
+        lo = np.empty((self.clouds, inf.dim), np.float32)
+        hi = np.empty((self.clouds, inf.dim), np.float32)
+        _check(self._lib, self._lib.nabo_hip_get_bounds(self._h, _ptr(lo), _ptr(hi)))
+        return lo, hi
+
+    def knn(self, queries, query_offsets, k: int, epsilon: float = 0.0, optionFlags: int = 0,
+            maxRadius: float = np.inf, indices=None, dists2=None, touched=None, stream=None):
+        """queries: (m, rows) float32 CUDA tensor; query_offsets: clouds + 1
+        int64 CUDA tensor. Returns (indices, dists2) CUDA tensors (m, k)."""
+        import torch
+
+        m, qrows = queries.shape
+        if indices is None:
+            indices = torch.empty((m, k), dtype=torch.int32, device=queries.device)
+        if dists2 is None:
+            dists2 = torch.empty((m, k), dtype=torch.float32, device=queries.device)
+        _check(self._lib, self._lib.nabo_hip_batch_knn(
+            self._h, C.c_void_p(queries.data_ptr()), qrows, queries.stride(0), m,
+            C.c_void_p(query_offsets.data_ptr()), int(k), float(epsilon), int(optionFlags),
+            float(maxRadius), C.c_void_p(indices.data_ptr()), C.c_void_p(dists2.data_ptr()),
+            None if touched is None else C.c_void_p(touched.data_ptr()),
+            _stream_handle(stream, queries.device)))
+        return indices, dists2

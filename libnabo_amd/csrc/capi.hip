@@ -128,7 +128,7 @@ void check_create(int64_t n, int32_t rows, int64_t stride, int32_t dim, int32_t 
 
 void build_index(nabo_hip_index* h, const float* d_cloud, int64_t n, int32_t rows, int64_t stride,
                  int32_t dim, int32_t type, uint32_t cflags, const nabo_hip_params& p,
-                 hipStream_t s) {
+                 hipStream_t s, const std::vector<int64_t>* offsets = nullptr) {
   Index& ix = h->ix;
   ix.search_type = type;
   ix.rows = rows;
@@ -140,7 +140,7 @@ void build_index(nabo_hip_index* h, const float* d_cloud, int64_t n, int32_t row
   ix.dim_bits = storage_bits(uint32_t(ix.dim));
   ix.dim_mask = (1u << ix.dim_bits) - 1u;
   if (type == NABO_HIP_SEARCH_KDTREE) {
-    kd_build(ix, d_cloud, stride, s);
+    kd_build(ix, d_cloud, stride, offsets ? *offsets : std::vector<int64_t>{0, n}, s);
     ix.device_bytes = int64_t(sizeof(uint2)) * ix.node_count +
                       int64_t(sizeof(float)) * ix.pts_stride * ix.n;
   } else {
@@ -153,13 +153,17 @@ void build_index(nabo_hip_index* h, const float* d_cloud, int64_t n, int32_t row
 void check_knn(const Index& ix, int32_t qrows, int64_t qstride, int64_t m, int32_t k,
                uint32_t flags) {
   const bool allow_self = flags & NABO_HIP_ALLOW_SELF_MATCH;
+  // Batched trees: every cloud must satisfy the reference's k rule.
+  int64_t npts = ix.n;
+  for (int32_t c = 0; c + 1 < int32_t(ix.cloud_offsets.size()); ++c)
+    npts = std::min(npts, ix.cloud_offsets[c + 1] - ix.cloud_offsets[c]);
   if (allow_self) {
-    if (k > ix.n)
+    if (k > npts)
       arg_error("Requesting more points (" + std::to_string(k) + ") than available in cloud (" +
-                std::to_string(ix.n) + ")");
-  } else if (k > ix.n - 1) {
+                std::to_string(npts) + ")");
+  } else if (k > npts - 1) {
     arg_error("Requesting more points (" + std::to_string(k) +
-              ") than available in cloud minus 1 (" + std::to_string(ix.n - 1) +
+              ") than available in cloud minus 1 (" + std::to_string(npts - 1) +
               ") (as self match is forbidden)");
   }
   if (qrows < ix.dim)
@@ -174,6 +178,22 @@ void check_knn(const Index& ix, int32_t qrows, int64_t qstride, int64_t m, int32
 }
 
 __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
+
+// Cloud id of each query of a batched search (binary search in the offsets).
+__global__ void k_query_clouds(int64_t m, int32_t clouds, const int64_t* __restrict__ offsets,
+                               int32_t* __restrict__ qcloud) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int lo = 0, hi = clouds - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (offsets[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  qcloud[i] = lo;
+}
 
 // The touched counter is cleared by a kernel rather than a memset node so
 // that callers may capture the call in a HIP graph.
@@ -231,6 +251,7 @@ Index::~Index() {
   if (d_cloud) (void)hipFree(d_cloud);
   if (d_cloud_lp) (void)hipFree(d_cloud_lp);
   if (d_norms) (void)hipFree(d_norms);
+  if (d_roots) (void)hipFree(d_roots);
 }
 
 }  // namespace nabo_hip
@@ -440,6 +461,98 @@ int nabo_hip_knn(const nabo_hip_index* index, const float* query, int32_t qrows,
   });
 }
 
+int nabo_hip_batch_create(const float* d_points, const int64_t* offsets, int32_t clouds,
+                          int32_t rows, int64_t stride, int32_t dim, uint32_t creation_flags,
+                          const nabo_hip_params* params, void* hip_stream,
+                          nabo_hip_index** out) {
+  return guarded([&] {
+    if (!out) arg_error("output handle pointer is null");
+    *out = nullptr;
+    if (clouds < 1 || !offsets) arg_error("a batch needs at least one cloud");
+    nabo_hip_params p;
+    nabo_hip_params_default(&p);
+    if (params) p = *params;
+    std::vector<int64_t> off(offsets, offsets + clouds + 1);
+    if (off[0] != 0) arg_error("cloud offsets must start at 0");
+    for (int32_t c = 0; c < clouds; ++c) {
+      if (off[c + 1] < off[c]) arg_error("cloud offsets must not decrease");
+      check_create(off[c + 1] - off[c], rows, stride, dim, NABO_HIP_SEARCH_KDTREE, p);
+    }
+    const int64_t n = off[clouds];
+    if (n >= (int64_t(1) << 30))
+      throw Error(NABO_HIP_ERR_UNSUPPORTED, "batches of 2^30 points or more are not supported");
+    if (!d_points) arg_error("points pointer is null");
+    int dev = p.device;
+    if (dev < 0) NH_CHECK(hipGetDevice(&dev));
+    DeviceScope scope(dev);
+    prepare_device(dev);
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    auto* h = new nabo_hip_index();
+    h->ix.device = dev;
+    try {
+      build_index(h, d_points, n, rows, stride, dim, NABO_HIP_SEARCH_KDTREE, creation_flags, p, s,
+                  &off);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int nabo_hip_batch_knn(const nabo_hip_index* index, const float* d_query, int32_t qrows,
+                       int64_t qstride, int64_t m, const int64_t* d_query_offsets, int32_t k,
+                       float epsilon, uint32_t option_flags, float max_radius,
+                       int32_t* d_indices, float* d_dists2, uint64_t* d_touched,
+                       void* hip_stream) {
+  return guarded([&] {
+    if (!index) arg_error("index is null");
+    const Index& ix = index->ix;
+    check_knn(ix, qrows, qstride, m, k, option_flags);
+    if (m > 0 && k > 0 && (!d_query || !d_indices || !d_dists2 || !d_query_offsets))
+      arg_error("query, offsets or output pointer is null");
+    DeviceScope scope(ix.device);
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    DevBuf<int32_t> qcloud;
+    if (ix.clouds > 1 && m > 0) {
+      qcloud.alloc(size_t(m), s);
+      k_query_clouds<<<unsigned(ceil_div(m, 256)), 256, 0, s>>>(m, ix.clouds, d_query_offsets,
+                                                                 qcloud.p);
+      NH_LAUNCH_CHECK();
+    }
+    KnnArgs a;
+    a.query = d_query;
+    a.qrows = qrows;
+    a.qstride = qstride;
+    a.m = m;
+    a.k = k;
+    a.epsilon = epsilon;
+    a.flags = option_flags;
+    a.max_radius = max_radius;
+    a.out_idx = d_indices;
+    a.out_d2 = d_dists2;
+    a.touched = d_touched;
+    a.qcloud = qcloud.p;
+    run_knn(ix, a, s);
+  });
+}
+
+int nabo_hip_batch_roots(const nabo_hip_index* index, int32_t* roots) {
+  return guarded([&] {
+    if (!index || !roots) arg_error("null argument");
+    const Index& ix = index->ix;
+    if (!ix.d_roots) {
+      roots[0] = 0;
+      return;
+    }
+    DeviceScope scope(ix.device);
+    hipStream_t s = thread_stream(ix.device);
+    NH_CHECK(hipMemcpyAsync(roots, ix.d_roots, sizeof(int32_t) * ix.clouds, hipMemcpyDeviceToHost,
+                            s));
+    NH_CHECK(hipStreamSynchronize(s));
+  });
+}
+
 int nabo_hip_get_info(const nabo_hip_index* index, nabo_hip_info* info) {
   return guarded([&] {
     if (!index || !info) arg_error("null argument");
@@ -456,6 +569,7 @@ int nabo_hip_get_info(const nabo_hip_index* index, nabo_hip_info* info) {
     info->device_bytes = ix.device_bytes;
     info->creation_flags = ix.creation_flags;
     info->device = ix.device;
+    info->clouds = ix.clouds;
   });
 }
 

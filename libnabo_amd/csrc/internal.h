@@ -85,7 +85,15 @@ struct Index {
   uint16_t* d_cloud_lp = nullptr;  // low-precision copy for the MFMA shortlist
   float* d_norms = nullptr;        // |p|^2 per point (low-precision path)
 
-  std::vector<float> min_bound, max_bound;
+  // Batched trees: clouds > 1 trees stored one after the other in d_nodes /
+  // d_pts; cloud c holds points [cloud_offsets[c], cloud_offsets[c+1]) of
+  // the concatenated input and its root is node d_roots[c] (node 0 when
+  // clouds == 1, d_roots null).
+  int32_t clouds = 1;
+  std::vector<int64_t> cloud_offsets;
+  int32_t* d_roots = nullptr;
+
+  std::vector<float> min_bound, max_bound;  // clouds x dim
   int64_t device_bytes = 0;
 
   ~Index();
@@ -104,10 +112,13 @@ struct KnnArgs {
   int32_t* out_idx = nullptr;    // device
   float* out_d2 = nullptr;       // device
   uint64_t* touched = nullptr;   // device, nullable
+  const int32_t* qcloud = nullptr;  // device, per-query cloud id (batched trees)
 };
 
 // Builds the kd-tree of d_cloud on the GPU (kdtree_build.hip).
-void kd_build(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s);
+// offsets: clouds + 1 point offsets into d_cloud ({0, n} for one cloud).
+void kd_build(Index& ix, const float* d_cloud, int64_t stride,
+              const std::vector<int64_t>& offsets, hipStream_t s);
 // Searches a built kd-tree (kdtree_search.hip).
 void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s);
 // Prepares and searches a brute-force index (brute_force.hip).

@@ -65,50 +65,108 @@ struct Bfs {  // all nodes in BFS order
   uint32_t* pre;    // pre-order id
 };
 
-__global__ void k_bounds(int64_t n, const float* __restrict__ cloud, int64_t stride, int dim,
-                         uint32_t* __restrict__ keys /* 2*dim: min keys, max keys */) {
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const bool act = p < n;
-  for (int d = 0; d < dim; ++d) {
-    const float v = act ? cloud[p * stride + d] : 0.f;
-    const uint32_t kv = float_key(v);
-    uint32_t mn = wave_min_u32(act ? kv : 0xffffffffu);
-    uint32_t mx = wave_max_u32(act ? kv : 0u);
-    if (__lane_id() == 0) {
-      atomicMin(&keys[d], mn);
-      atomicMax(&keys[dim + d], mx);
+// Tiled segmented min/max/add: each thread folds the points of one segment
+// at a time into a register and emits an atomic when its segment changes;
+// the last partial results are combined per wave. Segments are contiguous
+// ranges of positions, so a thread rarely changes segment inside a tile.
+template <int OP>
+struct SegAcc {
+  int seg = -1;
+  uint32_t v = 0;
+  __device__ __forceinline__ static uint32_t op(uint32_t a, uint32_t b) {
+    return OP == 0 ? min(a, b) : (OP == 1 ? max(a, b) : a + b);
+  }
+  __device__ __forceinline__ void add(int s, uint32_t x, uint32_t* target) {
+    if (s != seg) {
+      if (seg >= 0) atomic(target);
+      seg = s;
+      v = x;
+    } else {
+      v = op(v, x);
     }
   }
-}
+  __device__ __forceinline__ void atomic(uint32_t* target) {
+    if (OP == 0)
+      atomicMin(&target[seg], v);
+    else if (OP == 1)
+      atomicMax(&target[seg], v);
+    else
+      atomicAdd(&target[seg], v);
+  }
+  // All lanes of the wave call this once, at the end.
+  __device__ __forceinline__ void finish(uint32_t* target) {
+    wave_segmented_atomic<OP>(seg >= 0, seg, v, target);
+  }
+};
 
-__global__ void k_init_keys(int dim, uint32_t* keys) {
-  for (int d = threadIdx.x; d < dim; d += blockDim.x) {
-    keys[d] = float_key(FLT_MAX);        // minBound starts at max() (nabo.cpp:73)
-    keys[dim + d] = float_key(FLT_MIN);  // maxBound starts at min() > 0 (nabo.cpp:74)
+constexpr int kTile = 2048;  // points per workgroup in the tiled passes
+
+// Per-cloud bounds (kdtree_cpu.cpp:256-267): keys[c*2*dim + d] holds the min
+// key of dimension d, keys[c*2*dim + dim + d] the max key.
+__global__ void k_bounds(int64_t n, const float* __restrict__ cloud, int64_t stride, int dim,
+                         const int32_t* __restrict__ cid, uint32_t* __restrict__ keys) {
+  const int64_t t0 = int64_t(blockIdx.x) * kTile;
+  const int64_t t1 = min(n, t0 + kTile);
+  for (int d = 0; d < dim; ++d) {
+    SegAcc<0> mn;
+    SegAcc<1> mx;
+    for (int64_t p = t0 + threadIdx.x; p < t1; p += blockDim.x) {
+      const uint32_t kv = float_key(cloud[p * stride + d]);
+      const int c = cid[p];
+      mn.add(c * 2 * dim + d, kv, keys);
+      mx.add(c * 2 * dim + dim + d, kv, keys);
+    }
+    mn.finish(keys);
+    mx.finish(keys);
   }
 }
 
-__global__ void k_root(int dim, int32_t n, const uint32_t* keys, Seg seg, float* bounds_out) {
-  for (int d = threadIdx.x; d < dim; d += blockDim.x) {
-    const float lo = key_float(keys[d]);
-    const float hi = key_float(keys[dim + d]);
-    seg.lo[d] = lo;
-    seg.hi[d] = hi;
-    bounds_out[d] = lo;
-    bounds_out[dim + d] = hi;
-  }
-  if (threadIdx.x == 0) {
-    seg.start[0] = 0;
-    seg.count[0] = n;
+__global__ void k_init_keys(int32_t clouds, int dim, uint32_t* keys) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(clouds) * dim) return;
+  const int64_t c = e / dim;
+  const int d = int(e % dim);
+  keys[c * 2 * dim + d] = float_key(FLT_MAX);        // minBound starts at max() (nabo.cpp:73)
+  keys[c * 2 * dim + dim + d] = float_key(FLT_MIN);  // maxBound starts at min() > 0 (nabo.cpp:74)
+}
+
+// Level 0: one root segment per cloud, its cell = the cloud's bounds.
+__global__ void k_roots(int32_t clouds, int dim, const int64_t* __restrict__ offsets,
+                        const uint32_t* __restrict__ keys, Seg seg, float* __restrict__ bounds) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(clouds) * dim) return;
+  const int64_t c = e / dim;
+  const int d = int(e % dim);
+  const float lo = key_float(keys[c * 2 * dim + d]);
+  const float hi = key_float(keys[c * 2 * dim + dim + d]);
+  seg.lo[c * dim + d] = lo;
+  seg.hi[c * dim + d] = hi;
+  bounds[c * 2 * dim + d] = lo;
+  bounds[c * 2 * dim + dim + d] = hi;
+  if (d == 0) {
+    seg.start[c] = int32_t(offsets[c]);
+    seg.count[c] = int32_t(offsets[c + 1] - offsets[c]);
   }
 }
 
-__global__ void k_iota(int64_t n, int32_t* perm, int32_t* pseg) {
+// Cloud id of every point (binary search in the offsets), identity order,
+// every point in its cloud's root segment.
+__global__ void k_init_points(int64_t n, int32_t clouds, const int64_t* __restrict__ offsets,
+                              int32_t* __restrict__ cid, int32_t* __restrict__ perm,
+                              int32_t* __restrict__ pseg) {
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p < n) {
-    perm[p] = int32_t(p);
-    pseg[p] = 0;
+  if (p >= n) return;
+  int lo = 0, hi = clouds - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (offsets[mid] <= p)
+      lo = mid;
+    else
+      hi = mid - 1;
   }
+  cid[p] = lo;
+  perm[p] = int32_t(p);
+  pseg[p] = lo;
 }
 
 __global__ void k_seg_classify(int32_t S, Seg seg, SegStat st, Bfs bfs, int64_t node_base,
@@ -154,21 +212,25 @@ __global__ void k_split_total(int32_t S, const uint32_t* split, const uint32_t* 
 __global__ void k_pt_gather(int64_t n, int32_t* __restrict__ pseg, const int32_t* __restrict__ perm,
                             const float* __restrict__ cloud, int64_t stride, SegStat st,
                             float* __restrict__ xs) {
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int s = p < n ? pseg[p] : -1;
-  if (s >= 0 && st.split[s] == 0) {
-    pseg[p] = -1;  // its segment became a leaf
-    s = -1;
-  }
-  const bool act = s >= 0;
-  uint32_t key = 0;
-  if (act) {
+  const int64_t t0 = int64_t(blockIdx.x) * kTile;
+  const int64_t t1 = min(n, t0 + kTile);
+  SegAcc<0> mn;
+  SegAcc<1> mx;
+  for (int64_t p = t0 + threadIdx.x; p < t1; p += blockDim.x) {
+    int s = pseg[p];
+    if (s < 0) continue;
+    if (st.split[s] == 0) {
+      pseg[p] = -1;  // its segment became a leaf
+      continue;
+    }
     const float x = cloud[int64_t(perm[p]) * stride + st.cd[s]];
     xs[p] = x;
-    key = float_key(x);
+    const uint32_t key = float_key(x);
+    mn.add(s, key, st.minkey);
+    mx.add(s, key, st.maxkey);
   }
-  wave_segmented_atomic<0>(act, s, key, st.minkey);
-  wave_segmented_atomic<1>(act, s, key, st.maxkey);
+  mn.finish(st.minkey);
+  mx.finish(st.maxkey);
 }
 
 __global__ void k_seg_cut(int32_t S, SegStat st) {
@@ -189,18 +251,19 @@ __global__ void k_seg_cut(int32_t S, SegStat st) {
 
 __global__ void k_pt_count(int64_t n, const int32_t* __restrict__ pseg,
                            const float* __restrict__ xs, SegStat st) {
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int s = p < n ? pseg[p] : -1;
-  const bool act = s >= 0;
-  uint32_t lt = 0, eq = 0;
-  if (act) {
+  const int64_t t0 = int64_t(blockIdx.x) * kTile;
+  const int64_t t1 = min(n, t0 + kTile);
+  SegAcc<2> lt, eq;
+  for (int64_t p = t0 + threadIdx.x; p < t1; p += blockDim.x) {
+    const int s = pseg[p];
+    if (s < 0) continue;
     const float x = xs[p];
     const float cut = st.cut[s];
-    lt = x < cut;
-    eq = x == cut;
+    lt.add(s, x < cut ? 1u : 0u, st.lt);
+    eq.add(s, x == cut ? 1u : 0u, st.eq);
   }
-  wave_segmented_atomic<2>(act, s, lt, st.lt);
-  wave_segmented_atomic<2>(act, s, eq, st.eq);
+  lt.finish(st.lt);
+  eq.finish(st.eq);
 }
 
 // Out-of-place flags of one Hoare pass: low word = belongs right but sits
@@ -357,11 +420,14 @@ __global__ void k_subtree_size(int64_t base, int32_t S, Bfs bfs, int dim) {
     bfs.size[node] = 1 + bfs.size[bfs.left[node]] + bfs.size[bfs.right[node]];
 }
 
-__global__ void k_preorder(int64_t base, int32_t S, Bfs bfs, int dim) {
+// Pre-order ids: a root takes its tree's base (trees are stored one after
+// the other), then left = parent + 1 and right = parent + 1 + |left subtree|.
+__global__ void k_preorder(int64_t base, int32_t S, Bfs bfs, int dim,
+                           const uint32_t* __restrict__ root_base) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= S) return;
   const int64_t node = base + i;
-  if (node == 0) bfs.pre[0] = 0;
+  if (root_base) bfs.pre[node] = root_base[node];
   if (bfs.cd[node] == uint32_t(dim)) return;
   const uint32_t pre = bfs.pre[node];
   const int32_t l = bfs.left[node];
@@ -385,8 +451,9 @@ __global__ void k_encode(int64_t count, Bfs bfs, uint32_t dim_bits, int dim, uin
 }
 
 // Leaf-ordered copy of the points: pts_stride floats per entry, coordinates
-// then the point index (as bits) in slot dim.
+// then the point's index within its own cloud (as bits) in slot dim.
 __global__ void k_pack_points(int64_t n, const int32_t* __restrict__ perm,
+                              const int32_t* __restrict__ cid, const int64_t* __restrict__ offsets,
                               const float* __restrict__ cloud, int64_t stride, int dim,
                               int pts_stride, float* __restrict__ pts) {
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -395,74 +462,68 @@ __global__ void k_pack_points(int64_t n, const int32_t* __restrict__ perm,
   float* dst = pts + p * pts_stride;
   const float* src = cloud + int64_t(idx) * stride;
   for (int d = 0; d < dim; ++d) dst[d] = src[d];
-  dst[dim] = __int_as_float(idx);
+  dst[dim] = __int_as_float(int32_t(idx - offsets[cid[idx]]));
   for (int d = dim + 1; d < pts_stride; ++d) dst[d] = 0.f;
 }
 
-inline unsigned grid_for(int64_t n) { return unsigned(ceil_div(n, kBlock)); }
+__global__ void k_roots_out(int32_t clouds, const uint32_t* __restrict__ root_base,
+                            int32_t* __restrict__ roots) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < clouds) roots[c] = int32_t(root_base[c]);
+}
+
+inline unsigned grid_for(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, kBlock))); }
+inline unsigned tiles_for(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, kTile))); }
 
 }  // namespace
 
-void kd_build(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s) {
+void kd_build(Index& ix, const float* d_cloud, int64_t stride,
+              const std::vector<int64_t>& offsets, hipStream_t s) {
   const int64_t n = ix.n;
   const int dim = ix.dim;
+  const int32_t clouds = int32_t(offsets.size() - 1);
+  ix.clouds = clouds;
+  ix.cloud_offsets = offsets;
   ix.pts_stride = int((dim + 1 + 3) / 4 * 4);
 
   NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_pts), sizeof(float) * size_t(n) * ix.pts_stride));
   float* pts = ix.d_pts;
-  ix.min_bound.assign(size_t(dim), FLT_MAX);
-  ix.max_bound.assign(size_t(dim), FLT_MIN);
-
-  if (n <= int64_t(ix.bucket_size)) {
-    // Single bucket holding every point in index order (kdtree_cpu.cpp:236-243);
-    // the bounds keep their initial values there.
-    DevBuf<int32_t> perm(size_t(n), s), pseg(size_t(n), s);
-    k_iota<<<grid_for(n), kBlock, 0, s>>>(n, perm.p, pseg.p);
-    k_pack_points<<<grid_for(n), kBlock, 0, s>>>(n, perm.p, d_cloud, stride, dim, ix.pts_stride,
-                                                  pts);
-    NH_LAUNCH_CHECK();
-    uint2 root;
-    root.x = uint32_t(dim) | (uint32_t(n) << ix.dim_bits);
-    root.y = 0;
-    NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_nodes), sizeof(uint2)));
-    NH_CHECK(hipMemcpyAsync(ix.d_nodes, &root, sizeof(uint2), hipMemcpyHostToDevice, s));
-    NH_CHECK(hipStreamSynchronize(s));
-    ix.node_count = 1;
-    ix.depth = 0;
-    return;
-  }
 
   const int64_t max_nodes = 2 * n;
   // Level buffers (double-buffered), per-segment stats, BFS node records.
   DevBuf<int32_t> seg_start[2], seg_count[2];
   DevBuf<float> seg_lo[2], seg_hi[2];
+  const int64_t seg_cap = std::max<int64_t>(n, clouds);
   for (int b = 0; b < 2; ++b) {
-    seg_start[b].alloc(size_t(n), s);
-    seg_count[b].alloc(size_t(n), s);
-    seg_lo[b].alloc(size_t(n) * dim, s);
-    seg_hi[b].alloc(size_t(n) * dim, s);
+    seg_start[b].alloc(size_t(seg_cap), s);
+    seg_count[b].alloc(size_t(seg_cap), s);
+    seg_lo[b].alloc(size_t(seg_cap) * dim, s);
+    seg_hi[b].alloc(size_t(seg_cap) * dim, s);
   }
-  DevBuf<uint32_t> st_cd(n, s), st_minkey(n, s), st_maxkey(n, s), st_lt(n, s), st_eq(n, s),
-      st_split(n, s), st_childbase(n, s);
-  DevBuf<float> st_ideal(n, s), st_cut(n, s);
-  DevBuf<int32_t> st_left(n, s);
+  DevBuf<uint32_t> st_cd(seg_cap, s), st_minkey(seg_cap, s), st_maxkey(seg_cap, s),
+      st_lt(seg_cap, s), st_eq(seg_cap, s), st_split(seg_cap, s), st_childbase(seg_cap, s);
+  DevBuf<float> st_ideal(seg_cap, s), st_cut(seg_cap, s);
+  DevBuf<int32_t> st_left(seg_cap, s);
   DevBuf<uint32_t> b_cd(max_nodes, s), b_aux(max_nodes, s), b_count(max_nodes, s),
       b_size(max_nodes, s), b_pre(max_nodes, s);
   DevBuf<int32_t> b_left(max_nodes, s), b_right(max_nodes, s);
 
-  DevBuf<int32_t> perm(n, s), perm2(n, s), pseg(n, s), slot_l(n, s), slot_r(n, s);
+  DevBuf<int32_t> perm(n, s), perm2(n, s), pseg(n, s), slot_l(n, s), slot_r(n, s), cid(n, s);
   DevBuf<float> xs(n, s), xs2(n, s);
   DevBuf<uint64_t> flags(n, s), excl(n, s);
-  DevBuf<uint32_t> keys(size_t(2 * dim), s);
-  DevBuf<float> bounds(size_t(2 * dim), s);
+  DevBuf<uint32_t> keys(size_t(2) * clouds * dim, s), root_base(size_t(clouds), s);
+  DevBuf<float> bounds(size_t(2) * clouds * dim, s);
   DevBuf<int32_t> d_total(1, s);
+  DevBuf<int64_t> d_offsets(offsets.size(), s);
+  NH_CHECK(hipMemcpyAsync(d_offsets.p, offsets.data(), sizeof(int64_t) * offsets.size(),
+                          hipMemcpyHostToDevice, s));
 
   // Scan scratch, sized for the largest scan (n uint64).
   size_t scan_bytes64 = 0, scan_bytes32 = 0;
   NH_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes64, flags.p, excl.p, uint64_t(0), size_t(n),
                                    rocprim::plus<uint64_t>(), s));
   NH_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes32, st_split.p, st_childbase.p, uint32_t(0),
-                                   size_t(n), rocprim::plus<uint32_t>(), s));
+                                   size_t(seg_cap), rocprim::plus<uint32_t>(), s));
   DevBuf<uint8_t> scan_tmp(std::max(scan_bytes64, scan_bytes32) + 16, s);
 
   SegStat st{st_cd.p, st_ideal.p, st_cut.p, st_minkey.p, st_maxkey.p, st_lt.p, st_eq.p,
@@ -472,18 +533,20 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s) {
     return Seg{seg_start[b].p, seg_count[b].p, seg_lo[b].p, seg_hi[b].p};
   };
 
-  // Bounds pass (kdtree_cpu.cpp:256-267) and the root segment.
-  k_init_keys<<<1, 64, 0, s>>>(dim, keys.p);
-  k_bounds<<<grid_for(n), kBlock, 0, s>>>(n, d_cloud, stride, dim, keys.p);
-  k_root<<<1, 64, 0, s>>>(dim, int32_t(n), keys.p, level(0), bounds.p);
-  k_iota<<<grid_for(n), kBlock, 0, s>>>(n, perm.p, pseg.p);
+  // Bounds pass (kdtree_cpu.cpp:256-267) and the root segments.
+  const int64_t cd_elems = int64_t(clouds) * dim;
+  k_init_points<<<grid_for(n), kBlock, 0, s>>>(n, clouds, d_offsets.p, cid.p, perm.p, pseg.p);
+  k_init_keys<<<grid_for(cd_elems), kBlock, 0, s>>>(clouds, dim, keys.p);
+  k_bounds<<<tiles_for(n), kBlock, 0, s>>>(n, d_cloud, stride, dim, cid.p, keys.p);
+  k_roots<<<grid_for(cd_elems), kBlock, 0, s>>>(clouds, dim, d_offsets.p, keys.p, level(0),
+                                                 bounds.p);
   NH_LAUNCH_CHECK();
-  std::vector<float> hb(size_t(2 * dim));
-  NH_CHECK(hipMemcpyAsync(hb.data(), bounds.p, sizeof(float) * 2 * dim, hipMemcpyDeviceToHost, s));
+  std::vector<float> hb(size_t(2) * clouds * dim);
+  NH_CHECK(hipMemcpyAsync(hb.data(), bounds.p, sizeof(float) * hb.size(), hipMemcpyDeviceToHost, s));
 
   std::vector<int64_t> level_base;
   std::vector<int32_t> level_size;
-  int32_t S = 1;
+  int32_t S = clouds;
   int64_t node_base = 0;
   int cur = 0;
   int32_t host_total = 0;
@@ -506,9 +569,10 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s) {
     const int32_t T = host_total;
     if (T > 0) {
       const unsigned gp = grid_for(n);
-      k_pt_gather<<<gp, kBlock, 0, s>>>(n, pseg.p, perm.p, d_cloud, stride, st, xs.p);
+      const unsigned gt = tiles_for(n);
+      k_pt_gather<<<gt, kBlock, 0, s>>>(n, pseg.p, perm.p, d_cloud, stride, st, xs.p);
       k_seg_cut<<<gs, kBlock, 0, s>>>(S, st);
-      k_pt_count<<<gp, kBlock, 0, s>>>(n, pseg.p, xs.p, st);
+      k_pt_count<<<gt, kBlock, 0, s>>>(n, pseg.p, xs.p, st);
       for (int pass = 1; pass <= 2; ++pass) {
         k_pt_flags<<<gp, kBlock, 0, s>>>(n, pass, pseg.p, xs.p, sg, st, flags.p);
         bytes = scan_tmp.count;
@@ -534,17 +598,34 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s) {
   for (int L = levels - 1; L >= 0; --L)
     k_subtree_size<<<grid_for(level_size[L]), kBlock, 0, s>>>(level_base[L], level_size[L], bfs,
                                                               dim);
+  // Trees one after the other: the root of cloud c (BFS id c) starts at the
+  // sum of the sizes of the trees before it.
+  size_t bytes = scan_tmp.count;
+  NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, bfs.size, root_base.p, uint32_t(0),
+                                   size_t(clouds), rocprim::plus<uint32_t>(), s));
   for (int L = 0; L < levels; ++L)
-    k_preorder<<<grid_for(level_size[L]), kBlock, 0, s>>>(level_base[L], level_size[L], bfs, dim);
+    k_preorder<<<grid_for(level_size[L]), kBlock, 0, s>>>(level_base[L], level_size[L], bfs, dim,
+                                                          L == 0 ? root_base.p : nullptr);
   NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_nodes), sizeof(uint2) * count));
   k_encode<<<grid_for(count), kBlock, 0, s>>>(count, bfs, ix.dim_bits, dim, ix.d_nodes);
-  k_pack_points<<<grid_for(n), kBlock, 0, s>>>(n, perm.p, d_cloud, stride, dim, ix.pts_stride,
-                                                pts);
+  k_pack_points<<<grid_for(n), kBlock, 0, s>>>(n, perm.p, cid.p, d_offsets.p, d_cloud, stride,
+                                                dim, ix.pts_stride, pts);
+  if (clouds > 1) {
+    NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_roots), sizeof(int32_t) * clouds));
+    k_roots_out<<<grid_for(clouds), kBlock, 0, s>>>(clouds, root_base.p, ix.d_roots);
+  }
   NH_LAUNCH_CHECK();
   NH_CHECK(hipStreamSynchronize(s));
-  for (int d = 0; d < dim; ++d) {
-    ix.min_bound[d] = hb[d];
-    ix.max_bound[d] = hb[dim + d];
+  // A cloud of at most bucketSize points is one bucket and keeps the initial
+  // bounds (kdtree_cpu.cpp:236-243).
+  ix.min_bound.assign(size_t(clouds) * dim, FLT_MAX);
+  ix.max_bound.assign(size_t(clouds) * dim, FLT_MIN);
+  for (int32_t c = 0; c < clouds; ++c) {
+    if (offsets[c + 1] - offsets[c] <= int64_t(ix.bucket_size)) continue;
+    for (int d = 0; d < dim; ++d) {
+      ix.min_bound[size_t(c) * dim + d] = hb[size_t(c) * 2 * dim + d];
+      ix.max_bound[size_t(c) * dim + d] = hb[size_t(c) * 2 * dim + dim + d];
+    }
   }
   ix.node_count = count;
   ix.depth = levels - 1;

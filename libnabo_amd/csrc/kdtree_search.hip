@@ -84,6 +84,8 @@ struct SearchParams {
   int64_t qstride;
   int64_t m;
   const int32_t* order;
+  const int32_t* roots;   // per-cloud root node (batched trees), or null
+  const int32_t* qcloud;  // per-query cloud id when roots is set
   int k;
   float max_err2;
   float max_r2;
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
       r2 = r * r;
     }
 
-    uint32_t n = 0;
+    uint32_t n = a.roots ? uint32_t(a.roots[a.qcloud[qi]]) : 0u;
     float rd = 0.f;
     uint32_t top = base_off;  // offset of the next free slot
     for (;;) {
@@ -322,13 +324,15 @@ __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nod
                                                    uint32_t dim_bits, uint32_t dim_mask, int dim_rt,
                                                    const float* __restrict__ query,
                                                    int64_t qstride, int64_t m, uint32_t shift,
+                                                   const int32_t* __restrict__ roots,
+                                                   const int32_t* __restrict__ qcloud,
                                                    uint32_t* __restrict__ keys,
                                                    int32_t* __restrict__ vals) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const int dim = DIM > 0 ? DIM : dim_rt;
   const float* qp = query + i * qstride;
-  uint32_t n = 0;
+  uint32_t n = roots ? uint32_t(roots[qcloud[i]]) : 0u;
   for (;;) {
     const uint2 nd = nodes[n];
     const uint32_t cd = nd.x & dim_mask;
@@ -413,8 +417,14 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   p.out_d2 = a.out_d2;
   p.touched = reinterpret_cast<unsigned long long*>(a.touched);
   p.allow_self = (a.flags & NABO_HIP_ALLOW_SELF_MATCH) ? 1 : 0;
+  if (ix.clouds > 1) {
+    if (!a.qcloud) throw Error(NABO_HIP_ERR_ARGUMENT, "batched trees need per-query cloud ids");
+    p.roots = ix.d_roots;
+    p.qcloud = a.qcloud;
+  }
 
-  // Query order by home leaf.
+  // Query order by home leaf (bucket starts are global, so this also groups
+  // the queries of batched trees by cloud).
   DevBuf<uint32_t> keys, keys_sorted;
   DevBuf<int32_t> vals, order;
   DevBuf<uint8_t> sort_tmp;
@@ -430,13 +440,13 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     const unsigned g = unsigned(ceil_div(a.m, kBlock));
     if (ix.dim == 3)
       k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, shift, keys.p, vals.p);
+                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, vals.p);
     else if (ix.dim == 2)
       k_locate<2><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, shift, keys.p, vals.p);
+                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, vals.p);
     else
       k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, shift, keys.p, vals.p);
+                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, vals.p);
     NH_LAUNCH_CHECK();
     const unsigned end_bit = full_bits - shift;
     size_t bytes = 0;
