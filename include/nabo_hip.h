@@ -175,6 +175,52 @@ int nabo_hip_get_tree(const nabo_hip_index* index, uint32_t* nodes, int32_t* buc
 int nabo_hip_kernel_timing(int32_t enable);
 int nabo_hip_kernel_timing_collect(double* milliseconds, int64_t* launches);
 
+/* ---- Served kNN -----------------------------------------------------------
+ * A request server that keeps kd-trees resident in HBM. No counterpart in
+ * the reference (a library without a serving layer); for callers that issue
+ * many kNN requests against a pool of clouds. Clouds are registered once;
+ * a tree is built on the first request that needs it and stays resident
+ * until the HBM budget forces it out (least recently used first, never one
+ * in use). A worker thread admits requests in arrival order as stream slots
+ * and memory free up, so requests that arrive while others run overlap on
+ * the device; pending requests on the same cloud with the same k, epsilon,
+ * flags and radius are merged into one search launch. Each result equals
+ * nabo_hip_knn on that cloud alone. */
+typedef struct nabo_hip_server nabo_hip_server;
+
+typedef struct nabo_hip_server_config {
+  int64_t hbm_budget_bytes;   /* resident trees + request buffers (0: no limit) */
+  int32_t slots;              /* concurrent streams (default 4) */
+  int64_t max_merge_queries;  /* cap on queries merged into one launch */
+  int32_t device;             /* -1: current */
+  int32_t reserved[9];
+} nabo_hip_server_config;
+
+typedef struct nabo_hip_server_stats {
+  int64_t submitted, completed, launches, merged_requests;
+  int64_t builds, evictions, resident_trees, resident_bytes, peak_bytes;
+} nabo_hip_server_stats;
+
+void nabo_hip_server_config_default(nabo_hip_server_config* c);
+int nabo_hip_server_create(const nabo_hip_server_config* config, nabo_hip_server** out);
+/* Stops the worker after the requests already submitted and frees all. */
+int nabo_hip_server_destroy(nabo_hip_server* server);
+/* Registers a host cloud (copied); the tree is built when first needed. */
+int nabo_hip_server_add_cloud(nabo_hip_server* server, const float* cloud, int64_t n,
+                              int32_t rows, int64_t stride, int32_t dim,
+                              const nabo_hip_params* params, int32_t* cloud_id);
+/* Queues a request; query/indices/dists2 are host buffers that must stay
+ * valid until nabo_hip_server_wait returns for it. Arguments are checked
+ * here as nabo_hip_knn checks them. */
+int nabo_hip_server_submit(nabo_hip_server* server, int32_t cloud_id, const float* query,
+                           int32_t qrows, int64_t qstride, int64_t m, int32_t k, float epsilon,
+                           uint32_t option_flags, float max_radius, int32_t* indices,
+                           float* dists2, int64_t* request_id);
+/* Blocks until the request is done; returns its status and, if latency_ms
+ * is not null, the time from submit to completion. */
+int nabo_hip_server_wait(nabo_hip_server* server, int64_t request_id, double* latency_ms);
+int nabo_hip_server_get_stats(const nabo_hip_server* server, nabo_hip_server_stats* stats);
+
 /* Number of HIP devices visible (0 when none). */
 int nabo_hip_device_count(int32_t* count);
 

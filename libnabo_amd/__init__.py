@@ -94,12 +94,30 @@ class Info(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_int64),
+        ("slots", C.c_int32),
+        ("max_merge_queries", C.c_int64),
+        ("device", C.c_int32),
+        ("reserved", C.c_int32 * 9),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(name, C.c_int64) for name in (
+        "submitted", "completed", "launches", "merged_requests", "builds", "evictions",
+        "resident_trees", "resident_bytes", "peak_bytes")]
+
+
 EXPORTED = [
     "nabo_hip_last_error", "nabo_hip_params_default", "nabo_hip_create",
     "nabo_hip_create_device", "nabo_hip_destroy", "nabo_hip_knn", "nabo_hip_knn_device",
     "nabo_hip_get_info", "nabo_hip_get_bounds", "nabo_hip_get_tree", "nabo_hip_device_count",
     "nabo_hip_kernel_timing", "nabo_hip_kernel_timing_collect", "nabo_hip_batch_create",
-    "nabo_hip_batch_knn", "nabo_hip_batch_roots",
+    "nabo_hip_batch_knn", "nabo_hip_batch_roots", "nabo_hip_server_config_default",
+    "nabo_hip_server_create", "nabo_hip_server_destroy", "nabo_hip_server_add_cloud",
+    "nabo_hip_server_submit", "nabo_hip_server_wait", "nabo_hip_server_get_stats",
 ]
 
 _LIB = None
@@ -142,10 +160,21 @@ def load_library(path: str = LIB_PATH):
     lib.nabo_hip_batch_knn.argtypes = [vp, vp, i32, i64, i64, vp, i32, f32, u32, f32, vp, vp, vp,
                                        vp]
     lib.nabo_hip_batch_roots.argtypes = [vp, vp]
+    lib.nabo_hip_server_config_default.argtypes = [C.POINTER(ServerConfig)]
+    lib.nabo_hip_server_config_default.restype = None
+    lib.nabo_hip_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(vp)]
+    lib.nabo_hip_server_destroy.argtypes = [vp]
+    lib.nabo_hip_server_add_cloud.argtypes = [vp, vp, i64, i32, i64, i32, C.POINTER(Params),
+                                              C.POINTER(i32)]
+    lib.nabo_hip_server_submit.argtypes = [vp, i32, vp, i32, i64, i64, i32, f32, u32, f32, vp,
+                                           vp, C.POINTER(i64)]
+    lib.nabo_hip_server_wait.argtypes = [vp, i64, C.POINTER(C.c_double)]
+    lib.nabo_hip_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
     lib.nabo_hip_kernel_timing.argtypes = [i32]
     lib.nabo_hip_kernel_timing_collect.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     for name in EXPORTED:
-        getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
+        if name != "nabo_hip_server_config_default":
+            getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.nabo_hip_last_error.restype = C.c_char_p
     if path == LIB_PATH:
         _LIB = lib
@@ -422,3 +451,83 @@ class BatchSearch:
             None if touched is None else C.c_void_p(touched.data_ptr()),
             _stream_handle(stream, queries.device)))
         return indices, dists2
+
+
+class ServedRequest:
+    """A submitted request: holds its query and output arrays until waited."""
+
+    def __init__(self, server, rid, query, indices, dists2):
+        self.server, self.id = server, rid
+        self.query, self.indices, self.dists2 = query, indices, dists2
+        self.latency_ms = None
+
+    def wait(self):
+        """Blocks until done; returns (indices, dists2) of shape (m, k)."""
+        if self.latency_ms is None:
+            lat = C.c_double(0)
+            _check(self.server._lib, self.server._lib.nabo_hip_server_wait(
+                self.server._h, self.id, C.byref(lat)))
+            self.latency_ms = lat.value
+        return self.indices, self.dists2
+
+
+class Server:
+    """Served kNN over a pool of resident kd-trees (nabo_hip_server_*).
+
+    ``budget_bytes``: HBM target for resident trees and request buffers (0:
+    unlimited); ``slots``: concurrent streams. Register clouds with
+    add_cloud, then submit requests from any thread and wait on them.
+    """
+
+    def __init__(self, budget_bytes: int = 0, slots: int = 4, max_merge_queries: int = 1 << 22,
+                 device: int = -1):
+        self._lib = load_library()
+        self._h = None
+        cfg = ServerConfig()
+        self._lib.nabo_hip_server_config_default(C.byref(cfg))
+        cfg.hbm_budget_bytes = int(budget_bytes)
+        cfg.slots = int(slots)
+        cfg.max_merge_queries = int(max_merge_queries)
+        cfg.device = int(device)
+        h = C.c_void_p()
+        _check(self._lib, self._lib.nabo_hip_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            _check(self._lib, self._lib.nabo_hip_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_cloud(self, cloud, dim: int = 2**31 - 1, params: dict | None = None) -> int:
+        kw = {_PARAM_KEYS[key]: val for key, val in (params or {}).items()}
+        p = make_params(**kw)
+        pts = _as_points(cloud, "cloud")
+        n, rows = pts.shape
+        cid = C.c_int32(-1)
+        _check(self._lib, self._lib.nabo_hip_server_add_cloud(self._h, _ptr(pts), n, rows, rows,
+                                                               int(dim), C.byref(p),
+                                                               C.byref(cid)))
+        return cid.value
+
+    def submit(self, cloud_id: int, query, k: int = 1, epsilon: float = 0.0,
+               optionFlags: int = 0, maxRadius: float = np.inf) -> ServedRequest:
+        q = _as_points(query, "query")
+        m, qrows = q.shape
+        idx = np.empty((m, k), np.int32)
+        d2 = np.empty((m, k), np.float32)
+        rid = C.c_int64(0)
+        _check(self._lib, self._lib.nabo_hip_server_submit(
+            self._h, int(cloud_id), _ptr(q), qrows, qrows, m, int(k), float(epsilon),
+            int(optionFlags), float(maxRadius), _ptr(idx), _ptr(d2), C.byref(rid)))
+        return ServedRequest(self, rid.value, q, idx, d2)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(self._lib, self._lib.nabo_hip_server_get_stats(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}

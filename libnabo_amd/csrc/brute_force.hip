@@ -179,16 +179,335 @@ __global__ void k_copy_cloud(int64_t n, int dim, const float* __restrict__ src, 
   dst[e] = src[p * stride + d];
 }
 
+
+// ---------------------------------------------------------------------------
+// Mixed-precision shortlist (bf16 or fp16 MFMA, fp32 accumulation).
+//
+// Ranking value of (query q, point p): |p|^2 - 2 q.p, which orders points as
+// the squared distance |q|^2 + |p|^2 - 2 q.p does for a fixed query. It is
+// one GEMM: point row [p~_0..p~_{d-1}, hi(|p|^2), lo(|p|^2), 0...] times query
+// column [-2q~_0..-2q~_{d-1}, 1, 1, 0...] (x~ = x rounded to 16 bits; the
+// norm is split in two 16-bit parts so that it keeps ~16 significant bits;
+// -2q is exact). K is padded to a multiple of 16 for
+// v_mfma_f32_32x32x16_{bf16,f16}.
+//
+// MFMA operand/result layout (gfx950, checked by a probe): A[i][k] in lane
+// i + 32*(k/8), element k%8; B[k][j] in lane j + 32*(k/8); D[i][j] in lane
+// j + 32*((i/4)%2), register (i%4) + 4*(i/8). With points as rows and
+// queries as columns, lane l holds 16 ranking values of query l%32, for the
+// rows of half l/32 of each 32-point tile.
+//
+// Each lane keeps the L = k + m smallest ranking values of its half of a
+// chunk of points; a second kernel takes the L smallest of a query's lists
+// (the shortlist), recomputes their squared distances exactly in fp32 with
+// the sequential sum, applies the reference's admission rule and keeps the
+// k best by (distance, index).
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kMfmaTile = 32;  // points (rows) and queries (columns) per MFMA
+
+template <bool F16>
+struct Lp;
+template <>
+struct Lp<false> {
+  typedef __bf16 T;
+  typedef bf16x8 V;
+  static __device__ __forceinline__ f32x16 mfma(V a, V b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Lp<true> {
+  typedef _Float16 T;
+  typedef f16x8 V;
+  static __device__ __forceinline__ f32x16 mfma(V a, V b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  }
+};
+
+// Point rows: coordinates, then the two halves of |p|^2 (fp32 sequential
+// sum); padding rows get an infinite norm so that they rank last.
+template <bool F16>
+__global__ void k_lp_points(int64_t n, int64_t n_pad, int dim, int kp,
+                            const float* __restrict__ cloud, int64_t stride,
+                            typename Lp<F16>::T* __restrict__ out) {
+  typedef typename Lp<F16>::T T;
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n_pad) return;
+  T* row = out + p * kp;
+  if (p >= n) {
+    for (int c = 0; c < kp; ++c) row[c] = T(c == dim ? INFINITY : 0.f);
+    return;
+  }
+  const float* src = cloud + p * stride;
+  float norm = 0.f;
+  for (int d = 0; d < dim; ++d) {
+    const float x = src[d];
+    row[d] = T(x);
+    norm = norm + x * x;
+  }
+  const T hi = T(norm);
+  row[dim] = hi;
+  row[dim + 1] = T(norm - float(hi));
+  for (int c = dim + 2; c < kp; ++c) row[c] = T(0.f);
+}
+
+// Query columns: -2 q, then 1, 1 for the norm halves.
+template <bool F16>
+__global__ void k_lp_queries(int64_t m, int64_t m_pad, int dim, int kp,
+                             const float* __restrict__ query, int64_t qstride,
+                             typename Lp<F16>::T* __restrict__ out) {
+  typedef typename Lp<F16>::T T;
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= m_pad) return;
+  T* row = out + q * kp;
+  for (int c = 0; c < kp; ++c) {
+    float v = 0.f;
+    if (q < m) v = c < dim ? -2.f * query[q * qstride + c] : (c < dim + 2 ? 1.f : 0.f);
+    row[c] = T(v);
+  }
+}
+
+// Sorted insertion of a ranking value (ties keep the earlier entry first).
+template <int L>
+__device__ __forceinline__ void list_insert(float (&ld)[L], int32_t (&li)[L], float value,
+                                            int32_t index) {
+  int32_t moved[L];
+#pragma unroll
+  for (int j = 0; j < L; ++j) moved[j] = ld[j] > value ? index : li[j];
+#pragma unroll
+  for (int j = L - 1; j > 0; --j) {
+    li[j] = ld[j - 1] > value ? li[j - 1] : moved[j];
+    ld[j] = __builtin_amdgcn_fmed3f(ld[j - 1], ld[j], value);
+  }
+  li[0] = moved[0];
+  ld[0] = fminf(ld[0], value);
+}
+
+__device__ __forceinline__ float pick16(const f32x16& v, uint32_t r) {
+  float out = v[0];
+#pragma unroll
+  for (int e = 1; e < 16; ++e) out = (r == uint32_t(e)) ? v[e] : out;
+  return out;
+}
+
+struct ShortlistParams {
+  const void* points;   // n_pad x kp 16-bit
+  const void* queries;  // m_pad x kp 16-bit
+  int kp;
+  int64_t n;
+  int64_t m_pad;
+  int64_t chunk;        // points per chunk (multiple of 32)
+  int l;                // k + m
+  float* part_v;        // chunks x m_pad x 2 x LCAP
+  int32_t* part_i;
+};
+
+// grid: (m_pad / 128, chunks); 4 waves, each 32 queries.
+template <bool F16, int LCAP>
+__global__ __launch_bounds__(256) void k_bf_shortlist(ShortlistParams a) {
+  typedef typename Lp<F16>::T T;
+  typedef typename Lp<F16>::V V;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int half = lane >> 5;
+  const int64_t q = int64_t(blockIdx.x) * 128 + wave * 32 + (lane & 31);
+  const int64_t p0 = int64_t(blockIdx.y) * a.chunk;
+  const int64_t p1 = min(p0 + a.chunk, (a.n + kMfmaTile - 1) / kMfmaTile * kMfmaTile);
+  const int ks_count = a.kp / 16;
+  const T* qrow = static_cast<const T*>(a.queries) + q * a.kp + 8 * half;
+  V b[4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks)
+    if (ks < ks_count) b[ks] = *reinterpret_cast<const V*>(qrow + 16 * ks);
+
+  float ld[LCAP];
+  int32_t li[LCAP];
+#pragma unroll
+  for (int j = 0; j < LCAP; ++j) {
+    ld[j] = j < LCAP - a.l ? -INFINITY : INFINITY;
+    li[j] = -1;
+  }
+  const T* pbase = static_cast<const T*>(a.points) + (lane & 31) * a.kp + 8 * half;
+  for (int64_t t = p0; t < p1; t += kMfmaTile) {
+    const T* prow = pbase + t * a.kp;
+    f32x16 acc = {};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+      if (ks < ks_count) acc = Lp<F16>::mfma(*reinterpret_cast<const V*>(prow + 16 * ks), b[ks], acc);
+    uint32_t pending = 0;
+    const float head = ld[LCAP - 1];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) pending |= (acc[r] < head ? 1u : 0u) << r;
+    while (__ballot(pending != 0)) {
+      if (pending) {
+        const uint32_t r = __builtin_ctz(pending);
+        pending &= pending - 1;
+        const float v = pick16(acc, r);
+        if (v < ld[LCAP - 1]) {
+          const int32_t row = int32_t((r & 3) + 8 * (r >> 2) + 4 * half);
+          list_insert<LCAP>(ld, li, v, int32_t(t + row));
+        }
+      }
+    }
+  }
+  float* ov = a.part_v + ((int64_t(blockIdx.y) * a.m_pad + q) * 2 + half) * a.l;
+  int32_t* oi = a.part_i + ((int64_t(blockIdx.y) * a.m_pad + q) * 2 + half) * a.l;
+#pragma unroll
+  for (int j = 0; j < LCAP; ++j) {
+    if (j >= LCAP - a.l) {
+      ov[j - (LCAP - a.l)] = ld[j];
+      oi[j - (LCAP - a.l)] = li[j];
+    }
+  }
+}
+
+// Lexicographic (distance, index) insertion: the brute-force order.
+template <int K>
+__device__ __forceinline__ void lex_insert(float (&hd)[K], int32_t (&hi)[K], float d,
+                                           int32_t idx) {
+  bool gt[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) gt[j] = hd[j] > d || (hd[j] == d && hi[j] > idx);
+#pragma unroll
+  for (int j = K - 1; j > 0; --j) {
+    const float nd = gt[j - 1] ? hd[j - 1] : (gt[j] ? d : hd[j]);
+    const int32_t ni = gt[j - 1] ? hi[j - 1] : (gt[j] ? idx : hi[j]);
+    hd[j] = nd;
+    hi[j] = ni;
+  }
+  hd[0] = gt[0] ? d : hd[0];
+  hi[0] = gt[0] ? idx : hi[0];
+}
+
+// Shortlist (L smallest ranking values over the 2 x chunks lists of a query),
+// then exact fp32 rescoring and the k best.
+template <int LCAP, int K>
+__global__ __launch_bounds__(256) void k_bf_rescore(ShortlistParams s, BfParams a,
+                                                    int32_t* __restrict__ out_idx,
+                                                    float* __restrict__ out_d2) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= a.m) return;
+  float ld[LCAP];
+  int32_t li[LCAP];
+#pragma unroll
+  for (int j = 0; j < LCAP; ++j) {
+    ld[j] = j < LCAP - s.l ? -INFINITY : INFINITY;
+    li[j] = -1;
+  }
+  for (int c = 0; c < a.chunks * 2; ++c) {
+    const int64_t o = ((int64_t(c >> 1) * s.m_pad + q) * 2 + (c & 1)) * s.l;
+    for (int j = 0; j < s.l; ++j) {
+      const float v = s.part_v[o + j];
+      if (!(v < ld[LCAP - 1])) break;  // lists are ascending
+      list_insert<LCAP>(ld, li, v, s.part_i[o + j]);
+    }
+  }
+  float r2 = a.max_r2;
+  if (a.radii) {
+    const float r = a.radii[q];
+    r2 = r * r;
+  }
+  float hd[K];
+  int32_t hi[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    hd[j] = j < K - a.k ? -INFINITY : INFINITY;
+    hi[j] = j < K - a.k ? INT32_MIN : -1;
+  }
+  const float* qp = a.query + q * a.qstride;
+#pragma unroll
+  for (int j = 0; j < LCAP; ++j) {
+    const int32_t idx = li[j];
+    if (j >= LCAP - s.l && idx >= 0 && idx < a.n) {
+      const float* pp = a.cloud + int64_t(idx) * a.cstride;
+      float dist = 0.f;
+      for (int d = 0; d < a.dim; ++d) {
+        const float diff = pp[d] - qp[d];
+        dist = dist + diff * diff;
+      }
+      const bool better = dist < hd[K - 1] || (dist == hd[K - 1] && idx < hi[K - 1]);
+      if (dist <= r2 && better && (a.allow_self || dist > FLT_EPSILON))
+        lex_insert<K>(hd, hi, dist, idx);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    if (j >= K - a.k) {
+      out_idx[q * a.k + j - (K - a.k)] = hd[j] == INFINITY ? -1 : hi[j];
+      out_d2[q * a.k + j - (K - a.k)] = hd[j];
+    }
+  }
+}
+
+template <bool F16, int LCAP, int K>
+void launch_shortlist(const ShortlistParams& s, const BfParams& p, hipStream_t st,
+                      int32_t* oi, float* od) {
+  k_bf_shortlist<F16, LCAP><<<dim3(unsigned(s.m_pad / 128), unsigned(p.chunks)), 256, 0, st>>>(s);
+  k_bf_rescore<LCAP, K><<<unsigned(ceil_div(p.m, 256)), 256, 0, st>>>(s, p, oi, od);
+}
+
+template <bool F16, int LCAP>
+void shortlist_k(const ShortlistParams& s, const BfParams& p, hipStream_t st, int32_t* oi,
+                 float* od) {
+  if (p.k <= 1)
+    launch_shortlist<F16, LCAP, 1>(s, p, st, oi, od);
+  else if (p.k <= 4)
+    launch_shortlist<F16, LCAP, 4>(s, p, st, oi, od);
+  else if (p.k <= 8)
+    launch_shortlist<F16, LCAP, 8>(s, p, st, oi, od);
+  else if (p.k <= 16)
+    launch_shortlist<F16, LCAP, 16>(s, p, st, oi, od);
+  else
+    launch_shortlist<F16, LCAP, 32>(s, p, st, oi, od);
+}
+
+template <bool F16>
+void shortlist_l(const ShortlistParams& s, const BfParams& p, hipStream_t st, int32_t* oi,
+                 float* od) {
+  if (s.l <= 16)
+    shortlist_k<F16, 16>(s, p, st, oi, od);
+  else if (s.l <= 32)
+    shortlist_k<F16, 32>(s, p, st, oi, od);
+  else
+    shortlist_k<F16, 64>(s, p, st, oi, od);
+}
+
 }  // namespace
 
 constexpr int kBfMaxK = 64;
 constexpr int kBfMaxDim = 128;
+
+constexpr int kLpMaxDim = 62;  // dim + 2 norm columns within 4 MFMA K-steps
 
 void bf_prepare(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s) {
   if (ix.dim > kBfMaxDim)
     throw Error(NABO_HIP_ERR_UNSUPPORTED, "dim = " + std::to_string(ix.dim) +
                                               " exceeds the HIP brute-force limit of " +
                                               std::to_string(kBfMaxDim));
+  const int prec = ix.params.bf_precision;
+  if (prec != NABO_HIP_BF_FP32 && prec != NABO_HIP_BF_BF16 && prec != NABO_HIP_BF_FP16)
+    throw Error(NABO_HIP_ERR_ARGUMENT, "unknown brute-force precision");
+  if (prec != NABO_HIP_BF_FP32) {
+    if (ix.dim > kLpMaxDim)
+      throw Error(NABO_HIP_ERR_UNSUPPORTED, "the bf16/fp16 shortlist supports dim <= 62");
+    if (ix.params.bf_extra < 0) throw Error(NABO_HIP_ERR_ARGUMENT, "bfExtra must be >= 0");
+    ix.lp_kp = int(ceil_div(ix.dim + 2, 16) * 16);
+    ix.lp_n_pad = ceil_div(ix.n, kMfmaTile) * kMfmaTile;
+    NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_cloud_lp),
+                       sizeof(uint16_t) * size_t(ix.lp_n_pad) * ix.lp_kp));
+    const unsigned g = unsigned(ceil_div(ix.lp_n_pad, kBlock));
+    if (prec == NABO_HIP_BF_FP16)
+      k_lp_points<true><<<g, kBlock, 0, s>>>(ix.n, ix.lp_n_pad, ix.dim, ix.lp_kp, d_cloud, stride,
+                                              reinterpret_cast<_Float16*>(ix.d_cloud_lp));
+    else
+      k_lp_points<false><<<g, kBlock, 0, s>>>(ix.n, ix.lp_n_pad, ix.dim, ix.lp_kp, d_cloud, stride,
+                                               reinterpret_cast<__bf16*>(ix.d_cloud_lp));
+    NH_LAUNCH_CHECK();
+  }
   ix.bf_stride = ix.dim;
   NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_cloud), sizeof(float) * size_t(ix.n) * ix.dim));
   const int64_t total = ix.n * ix.dim;
@@ -233,6 +552,44 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   chunks = std::min<int64_t>(chunks, 65535);
   p.chunk = ceil_div(ix.n, chunks);
   p.chunks = int(ceil_div(ix.n, p.chunk));
+  if (ix.params.bf_precision != NABO_HIP_BF_FP32) {
+    const int l = a.k + ix.params.bf_extra;
+    if (l > 64)
+      throw Error(NABO_HIP_ERR_UNSUPPORTED, "k + bfExtra must not exceed 64 for the shortlist");
+    const bool f16 = ix.params.bf_precision == NABO_HIP_BF_FP16;
+    ShortlistParams sp{};
+    sp.kp = ix.lp_kp;
+    sp.n = ix.n;
+    sp.m_pad = ceil_div(a.m, 128) * 128;
+    sp.l = l;
+    sp.points = ix.d_cloud_lp;
+    DevBuf<uint16_t> qlp(size_t(sp.m_pad) * sp.kp, s);
+    const unsigned gq = unsigned(ceil_div(sp.m_pad, kBlock));
+    if (f16)
+      k_lp_queries<true><<<gq, kBlock, 0, s>>>(a.m, sp.m_pad, ix.dim, sp.kp, a.query, a.qstride,
+                                                reinterpret_cast<_Float16*>(qlp.p));
+    else
+      k_lp_queries<false><<<gq, kBlock, 0, s>>>(a.m, sp.m_pad, ix.dim, sp.kp, a.query, a.qstride,
+                                                 reinterpret_cast<__bf16*>(qlp.p));
+    sp.queries = qlp.p;
+    // Chunks of points so that the grid has a few thousand workgroups.
+    const int64_t qb = sp.m_pad / 128;
+    int64_t chunks_lp = std::max<int64_t>(1, ceil_div(2048, qb));
+    chunks_lp = std::min<int64_t>(chunks_lp, std::max<int64_t>(1, ix.lp_n_pad / 1024));
+    sp.chunk = ceil_div(ceil_div(ix.lp_n_pad, chunks_lp), kMfmaTile) * kMfmaTile;
+    p.chunks = int(ceil_div(ix.lp_n_pad, sp.chunk));
+    DevBuf<float> pv(size_t(p.chunks) * sp.m_pad * 2 * l, s);
+    DevBuf<int32_t> pidx(size_t(p.chunks) * sp.m_pad * 2 * l, s);
+    sp.part_v = pv.p;
+    sp.part_i = pidx.p;
+    KernelTimer timer(s);
+    if (f16)
+      shortlist_l<true>(sp, p, s, a.out_idx, a.out_d2);
+    else
+      shortlist_l<false>(sp, p, s, a.out_idx, a.out_d2);
+    NH_LAUNCH_CHECK();
+    return;
+  }
   DevBuf<float> pd(size_t(p.chunks) * a.m * a.k, s);
   DevBuf<int32_t> pi(size_t(p.chunks) * a.m * a.k, s);
   p.part_d = pd.p;

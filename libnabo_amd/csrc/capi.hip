@@ -214,6 +214,8 @@ void run_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
 
 }  // namespace
 
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
 void throw_hip(hipError_t e, const char* what, const char* file, int line) {
   const int code = (e == hipErrorOutOfMemory) ? NABO_HIP_ERR_MEMORY : NABO_HIP_ERR_HIP;
   throw Error(code, std::string(what) + " failed: " + hipGetErrorString(e) + " (" + file + ":" +

@@ -18,6 +18,8 @@ struct Error : std::runtime_error {
 };
 
 [[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line);
+// Sets the message returned by nabo_hip_last_error() on this thread.
+void set_last_error(const std::string& msg);
 
 #define NH_CHECK(expr)                                              \
   do {                                                              \
@@ -82,7 +84,9 @@ struct Index {
   // brute force
   float* d_cloud = nullptr;   // dim x n, column stride bf_stride
   int bf_stride = 0;
-  uint16_t* d_cloud_lp = nullptr;  // low-precision copy for the MFMA shortlist
+  uint16_t* d_cloud_lp = nullptr;  // 16-bit rows for the MFMA shortlist
+  int lp_kp = 0;                   // 16-bit row length (dim + 2 norm columns, padded)
+  int64_t lp_n_pad = 0;            // rows, padded to a multiple of 32
   float* d_norms = nullptr;        // |p|^2 per point (low-precision path)
 
   // Batched trees: clouds > 1 trees stored one after the other in d_nodes /
