@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-query-sort", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline line (no brute-force / batched / served sections)")
     return ap.parse_args()
 
 
@@ -80,6 +82,287 @@ def cpu_baseline(cloud, query, k, eps, radius, bucket, budget_s):
                   f"(median), build {build_s:.3f}s excluded",
         "build_s": build_s,
     }, (oi, od, touched, visits)
+
+
+def bench_brute_force(log_cpu: bool):
+    """Brute force, 200k x 33-D Dirichlet descriptors, 50k queries, k=8: exact
+    fp32 path and bf16 / fp16 MFMA shortlists (m = 8, 32) with exact
+    rescoring. Miss rate = share of queries whose k-best differs from the
+    exact fp32 k-best; error = max relative |dists2 - FP64| over a sample."""
+    import torch
+
+    import libnabo_amd as nabo
+    import oracle as orc
+    from libnabo_amd import datasets
+
+    cloud, query = datasets.dirichlet_descriptors(200_000, 50_000, seed=3)
+    m, dim, k = len(query), cloud.shape[1], 8
+    pairs = float(len(cloud)) * m
+    dq = torch.from_numpy(query).cuda()
+    out = {"workload": "dirichlet_200k_x_50k_d33_k8"}
+    exact = None
+    for prec, extra in (("fp32", 0), ("bf16", 8), ("fp16", 8), ("bf16", 32), ("fp16", 32)):
+        nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, dim, 0,
+                                          {"bfPrecision": prec, "bfExtra": extra})
+        di = torch.empty((m, k), dtype=torch.int32, device="cuda")
+        dd = torch.empty((m, k), dtype=torch.float32, device="cuda")
+        nns.knn_device(dq, k, di, dd)
+        torch.cuda.synchronize()
+        nabo.kernel_timing(True)
+        nabo.kernel_timing_collect()
+        t0 = time.perf_counter()
+        reps = 2
+        for _ in range(reps):
+            nns.knn_device(dq, k, di, dd)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        kms, launches = nabo.kernel_timing_collect()
+        nabo.kernel_timing(False)
+        gi, gd = di.cpu().numpy(), dd.cpu().numpy()
+        key = prec if prec == "fp32" else f"{prec}_m{extra}"
+        entry = {"ms": round(ms, 3), "queries_per_s": round(m / (ms / 1e3), 1)}
+        if prec == "fp32":
+            exact = (gi, gd)
+            flops = 3.0 * dim * pairs
+            entry["roofline"] = {"bound": "valu_fp32", "achieved": round(flops / (ms / 1e3) / 1e12, 2),
+                                 "peak": 157.3, "unit": "TFLOP/s",
+                                 "frac": round(flops / (ms / 1e3) / 1e12 / 157.3, 4)}
+        else:
+            kp = -(-(dim + 2) // 16) * 16
+            flops = 2.0 * kp * pairs
+            entry["mfma_tflops"] = round(flops / (ms / 1e3) / 1e12, 1)
+            entry["roofline"] = {"bound": "mfma", "achieved": entry["mfma_tflops"], "peak": 2500.0,
+                                 "unit": "TFLOP/s",
+                                 "frac": round(flops / (ms / 1e3) / 1e12 / 2500.0, 4)}
+            miss = np.any(gd.view(np.uint32) != exact[1].view(np.uint32), axis=1)
+            entry["miss_rate"] = float(miss.mean())
+            entry["speedup_vs_fp32"] = round(out["fp32"]["ms"] / ms, 2)
+        sample = slice(0, 2000)
+        f64 = orc.default().dist2_f64(cloud, query[sample], gi[sample])
+        entry["max_rel_err_vs_fp64"] = float(np.max(np.abs(gd[sample] - f64) / np.maximum(f64, 1e-30)))
+        out[key] = entry
+        nns.close()
+        log(f"brute force {key}: {entry}")
+    if log_cpu:
+        o = orc.default()
+        threads = os.cpu_count() or 1
+        nq = 500
+        t0 = time.perf_counter()
+        oi, od = o.bf_knn(cloud, query[:nq], k, flags=2, threads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(nq / dt, 1), "unit": "queries/s", "cores": threads,
+                               "kind": "port", "sample": f"oracle brute force, {nq} queries"}
+        out["fp32_parity_sample"] = bool(np.array_equal(exact[1][:nq].view(np.uint32),
+                                                        od.view(np.uint32)))
+    return out
+
+
+def bench_batched():
+    """512 uneven clouds (2k..200k points, k=8, eps 0): one batched build and
+    one search launch (autotuned bucket size / query order) against
+    per-cloud plain create + knn calls; both on device-resident data."""
+    import ctypes as C
+
+    import torch
+
+    import libnabo_amd as nabo
+    from libnabo_amd import autotune, datasets
+
+    pairs = datasets.uneven_clouds(512, seed=4)
+    clouds = [c for c, _ in pairs]
+    queries = [q for _, q in pairs]
+    off = np.concatenate([[0], np.cumsum([len(c) for c in clouds])]).astype(np.int64)
+    qoff = np.concatenate([[0], np.cumsum([len(q) for q in queries])]).astype(np.int64)
+    pts = torch.from_numpy(np.concatenate(clouds)).cuda()
+    qs = torch.from_numpy(np.concatenate(queries)).cuda()
+    dqoff = torch.from_numpy(qoff).cuda()
+    k = 8
+    best, table = autotune.tune_batch(pts, off, qs, qoff, k)
+    log(f"autotune best {best}")
+    params = {"bucketSize": best["bucket_size"], "querySort": best["query_sort"]}
+    # batched: build + search
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b = nabo.BatchSearch(pts, off, 3, 0, params)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    bi, bd = b.knn(qs, dqoff, k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bi, bd = b.knn(qs, dqoff, k, indices=bi, dists2=bd)
+    torch.cuda.synchronize()
+    knn_ms = (time.perf_counter() - t0) * 1e3
+    # plain: one index and one knn call per cloud, same parameters
+    lib = nabo.load_library()
+    p = nabo.make_params(bucket_size=best["bucket_size"], query_sort=bool(best["query_sort"]))
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    pi = torch.empty_like(bi)
+    pd = torch.empty_like(bd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    handles = []
+    for c in range(len(clouds)):
+        h = C.c_void_p()
+        rc = lib.nabo_hip_create_device(C.c_void_p(pts[off[c]:].data_ptr()), int(off[c + 1] - off[c]),
+                                        3, 3, 3, int(nabo.SearchType.KDTREE_HIP), 0, C.byref(p),
+                                        stream, C.byref(h))
+        assert rc == 0, lib.nabo_hip_last_error()
+        handles.append(h)
+    torch.cuda.synchronize()
+    plain_build_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    for c, h in enumerate(handles):
+        q0, q1 = int(qoff[c]), int(qoff[c + 1])
+        rc = lib.nabo_hip_knn_device(h, C.c_void_p(qs[q0:].data_ptr()), 3, 3, q1 - q0, k, 0.0, 0,
+                                     float("inf"), None, C.c_void_p(pi[q0:].data_ptr()),
+                                     C.c_void_p(pd[q0:].data_ptr()), None, stream)
+        assert rc == 0, lib.nabo_hip_last_error()
+    torch.cuda.synchronize()
+    plain_knn_ms = (time.perf_counter() - t0) * 1e3
+    for h in handles:
+        lib.nabo_hip_destroy(h)
+    same = bool(torch.equal(bi, pi) and torch.equal(bd, pd))
+    total_q = int(qoff[-1])
+    batched_total = build_ms + knn_ms
+    plain_total = plain_build_ms + plain_knn_ms
+    return {"workload": "512_uneven_clouds_2k_200k_k8", "clouds": len(clouds),
+            "points": int(off[-1]), "queries": total_q,
+            "autotune": {"best": best, "table": table},
+            "batched_ms": {"build": round(build_ms, 2), "knn": round(knn_ms, 2),
+                           "total": round(batched_total, 2)},
+            "plain_ms": {"build": round(plain_build_ms, 2), "knn": round(plain_knn_ms, 2),
+                         "total": round(plain_total, 2)},
+            "speedup_total": round(plain_total / batched_total, 2),
+            "speedup_knn": round(plain_knn_ms / knn_ms, 2),
+            "batched_queries_per_s": round(total_q / (knn_ms / 1e3), 1),
+            "results_equal_plain": same}
+
+
+def _serial_queue(arrivals, service):
+    """Latencies (ms) and makespan (s) of a one-at-a-time server."""
+    done = 0.0
+    lat = []
+    for a, s in zip(arrivals, service):
+        start = max(a, done)
+        done = start + s
+        lat.append((done - a) * 1e3)
+    return lat, done - arrivals[0]
+
+
+def _served_run(nabo, pool, reqs, arrivals, budget, slots, warm):
+    import threading
+
+    srv = nabo.Server(budget_bytes=budget, slots=slots)
+    ids = [srv.add_cloud(c, 3) for c in pool]
+    if warm:  # every tree resident before the stream starts
+        for cid, c in zip(ids, pool):
+            srv.submit(cid, c[:1], 1, 0.0, 1).wait()
+    base = srv.stats()
+    handles = [None] * len(reqs)
+    submitted = [0.0] * len(reqs)
+
+    def submitter():
+        t_start = time.perf_counter()
+        for i, (c, q, k, r) in enumerate(reqs):
+            delay = arrivals[i] - arrivals[0] - (time.perf_counter() - t_start)
+            if delay > 0:
+                time.sleep(delay)
+            submitted[i] = time.perf_counter()
+            handles[i] = srv.submit(ids[c], q, k, 0.0, 0, r)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    lat = []
+    for h in handles:
+        h.wait()
+        lat.append(h.latency_ms)
+    # makespan: first submit to last completion on the device (the callers'
+    # copy-out of results is not part of the service)
+    makespan = max(s + l / 1e3 for s, l in zip(submitted, lat)) - submitted[0]
+    st = srv.stats()
+    st = {key: st[key] - (base[key] if key in ("builds", "evictions", "submitted", "completed",
+                                                "launches", "merged_requests") else 0)
+          for key in st}
+    srv.close()
+    return lat, makespan, st, handles
+
+
+def bench_served(requests: int = 120, slots: int = 4):
+    """Served stream over 64 clouds (10k..2M points): Poisson arrivals,
+    batches of 1k..500k queries, k in {1, 8, 32}, maxRadius 0.5.
+
+    warm: every tree resident; the server (4 stream slots, continuous
+    admission, same-cloud merging) against the same requests served one at
+    a time by plain knn calls on pre-built trees (a serial queue with the
+    same arrival times, fed with each call's measured service time).
+    budget: HBM budget of half the trees' total bytes, cold start: the server
+    with LRU eviction against serial plain calls under the same budget and
+    LRU policy (trees built on a miss, evicted least recently used first)."""
+    import libnabo_amd as nabo
+    from libnabo_amd import datasets
+
+    pool = datasets.served_pool(64, seed=5)
+    reqs = datasets.served_requests(pool, requests, seed=6)
+    total_q = sum(len(q) for _, q, _, _ in reqs)
+    plain = {c: nabo.NearestNeighbourSearch(pool[c], nabo.SearchType.KDTREE_HIP, 3)
+             for c in range(len(pool))}
+    tree_bytes = {c: plain[c].info().device_bytes for c in plain}
+    service, ref = [], []
+    for c, q, k, r in reqs:
+        t0 = time.perf_counter()
+        ref.append(plain[c].knn(q, k, 0.0, 0, r))
+        service.append(time.perf_counter() - t0)
+    for v in plain.values():
+        v.close()
+    mean_s = float(np.mean(service))
+    rate = 0.9 / mean_s  # offered load: 90% of what the warm serial server takes
+    rng = np.random.default_rng(7)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
+    serial_lat, serial_span = _serial_queue(arrivals, service)
+    lat, span, st, handles = _served_run(nabo, pool, reqs, arrivals, 0, slots, True)
+    same = all(np.array_equal(h.indices, ri) and np.array_equal(h.dists2, rd)
+               for h, (ri, rd) in zip(handles, ref))
+    pct = lambda v, p: round(float(np.percentile(v, p)), 2)  # noqa: E731
+    out = {"workload": "served_64_clouds_10k_2M_poisson", "requests": len(reqs),
+           "queries": int(total_q), "offered_load_req_per_s": round(rate, 2), "slots": slots,
+           "warm": {"served": {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                               "queries_per_s": round(total_q / span, 1), "stats": st},
+                    "serial_plain": {"p50_ms": pct(serial_lat, 50), "p99_ms": pct(serial_lat, 99),
+                                     "queries_per_s": round(total_q / serial_span, 1),
+                                     "mean_service_ms": round(mean_s * 1e3, 2)},
+                    "results_equal_plain": bool(same)}}
+    # budget: half of the trees' bytes, cold start, LRU on both sides
+    budget = sum(tree_bytes.values()) // 2
+    lat_b, span_b, st_b, handles_b = _served_run(nabo, pool, reqs, arrivals, budget, slots, False)
+    cache, order, used, service_b = {}, [], 0, []
+    for c, q, k, r in reqs:
+        t0 = time.perf_counter()
+        if c not in cache:
+            while used + tree_bytes[c] > budget and order:
+                victim = order.pop(0)
+                used -= tree_bytes[victim]
+                cache.pop(victim).close()
+            cache[c] = nabo.NearestNeighbourSearch(pool[c], nabo.SearchType.KDTREE_HIP, 3)
+            used += tree_bytes[c]
+        else:
+            order.remove(c)
+        order.append(c)
+        cache[c].knn(q, k, 0.0, 0, r)
+        service_b.append(time.perf_counter() - t0)
+    for v in cache.values():
+        v.close()
+    serial_lat_b, serial_span_b = _serial_queue(arrivals, service_b)
+    same_b = all(np.array_equal(h.indices, ri) and np.array_equal(h.dists2, rd)
+                 for h, (ri, rd) in zip(handles_b, ref))
+    out["budget"] = {"budget_mb": round(budget / 2**20, 1),
+                     "served": {"p50_ms": pct(lat_b, 50), "p99_ms": pct(lat_b, 99),
+                                "queries_per_s": round(total_q / span_b, 1), "stats": st_b},
+                     "serial_plain_lru": {"p50_ms": pct(serial_lat_b, 50),
+                                          "p99_ms": pct(serial_lat_b, 99),
+                                          "queries_per_s": round(total_q / serial_span_b, 1)},
+                     "results_equal_plain": bool(same_b)}
+    return out
 
 
 def main():
@@ -209,6 +492,18 @@ def main():
                      "device_cloud": None if build_ms_dev is None else round(build_ms_dev, 2)},
         "parity_vs_oracle": parity,
     }
+    if not args.no_extras:
+        nns.close()
+        del d_query, d_idx, d_d2, d_cloud
+        torch.cuda.empty_cache()
+        log("brute force (C3)")
+        out["brute_force"] = bench_brute_force(not args.no_cpu)
+        torch.cuda.empty_cache()
+        log("batched clouds (C4)")
+        out["batched"] = bench_batched()
+        torch.cuda.empty_cache()
+        log("served stream (C5)")
+        out["served"] = bench_served()
     print(json.dumps(out), flush=True)
 
 

@@ -271,19 +271,25 @@ __global__ void k_lp_queries(int64_t m, int64_t m_pad, int dim, int kp,
   }
 }
 
-// Sorted insertion of a ranking value (ties keep the earlier entry first).
+// Sorted insertion of a ranking value (ties keep the earlier entry first),
+// in place from the top: entry j takes entry j-1 when that is greater than
+// value, else value when entry j is, else stays. The index selects are bit
+// blends so that they stay branch-free without temporaries.
+__device__ __forceinline__ int32_t blend(uint32_t mask, int32_t a, int32_t b) {
+  return int32_t((uint32_t(a) & mask) | (uint32_t(b) & ~mask));
+}
 template <int L>
 __device__ __forceinline__ void list_insert(float (&ld)[L], int32_t (&li)[L], float value,
                                             int32_t index) {
-  int32_t moved[L];
-#pragma unroll
-  for (int j = 0; j < L; ++j) moved[j] = ld[j] > value ? index : li[j];
+  uint32_t cur = 0u - uint32_t(ld[L - 1] > value);
 #pragma unroll
   for (int j = L - 1; j > 0; --j) {
-    li[j] = ld[j - 1] > value ? li[j - 1] : moved[j];
+    const uint32_t prev = 0u - uint32_t(ld[j - 1] > value);
+    li[j] = blend(prev, li[j - 1], blend(cur, index, li[j]));
     ld[j] = __builtin_amdgcn_fmed3f(ld[j - 1], ld[j], value);
+    cur = prev;
   }
-  li[0] = moved[0];
+  li[0] = blend(cur, index, li[0]);
   ld[0] = fminf(ld[0], value);
 }
 
@@ -470,8 +476,14 @@ void shortlist_l(const ShortlistParams& s, const BfParams& p, hipStream_t st, in
                  float* od) {
   if (s.l <= 16)
     shortlist_k<F16, 16>(s, p, st, oi, od);
+  else if (s.l <= 24)
+    shortlist_k<F16, 24>(s, p, st, oi, od);
   else if (s.l <= 32)
     shortlist_k<F16, 32>(s, p, st, oi, od);
+  else if (s.l <= 40)
+    shortlist_k<F16, 40>(s, p, st, oi, od);
+  else if (s.l <= 48)
+    shortlist_k<F16, 48>(s, p, st, oi, od);
   else
     shortlist_k<F16, 64>(s, p, st, oi, od);
 }

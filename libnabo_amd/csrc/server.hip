@@ -6,17 +6,24 @@
 // searches are nabo_hip_knn_device launches, so a served result is the
 // result of nabo_hip_knn on the same cloud alone.
 //
+// Host copies stay off the worker: the submitting thread stages a request's
+// queries into a pinned buffer (from a pool), the device writes its results
+// into pinned buffers, and the waiting thread copies them out.
+//
 // Worker loop, one pass:
-//   1. retire finished slots (event query): copy the pinned outputs into the
-//      callers' buffers, wake the waiters;
-//   2. while a slot is free and requests are pending (arrival order): make
-//      the request's tree resident (evicting least recently used trees that
-//      no slot is using until the budget holds), merge the pending requests
-//      on the same cloud with the same k / epsilon / flags / radius behind
-//      it (up to max_merge_queries), and enqueue copy-in, search and
-//      copy-out on the slot's stream.
-// Slots run concurrently on the device; a request that arrives while others
-// run is admitted as soon as a slot and memory allow.
+//   1. retire finished slots (event query): mark their requests done, wake
+//      the waiters;
+//   2. scan the pending requests in arrival order; a request whose tree is
+//      resident takes a free slot, together with the pending requests on the
+//      same cloud with the same k / epsilon / flags / radius (up to
+//      max_merge_queries): copy-in, search and copy-out are enqueued on the
+//      slot's stream; a request whose tree is not resident is handed to the
+//      builder thread and stays pending, so it does not hold back the
+//      requests behind it.
+// Builder thread: evicts least recently used trees that no slot is using
+// until the budget holds, then builds on its own stream, concurrently with
+// the searches. Slots run concurrently on the device; a request that arrives
+// while others run is admitted as soon as a slot and its tree are ready.
 
 #include <algorithm>
 #include <atomic>
@@ -24,6 +31,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -51,10 +59,13 @@ struct Cloud {
   int64_t stride = 0;
   int32_t dim = 0;
   nabo_hip_params params{};
-  nabo_hip_index* index = nullptr;
+  nabo_hip_index* index = nullptr;  // set once resident
   int64_t bytes = 0;
   uint64_t last_use = 0;
-  int in_use = 0;
+  int in_use = 0;          // slots searching it
+  bool building = false;   // queued for, or in, the builder thread
+  int build_status = NABO_HIP_OK;
+  std::string build_error;
 };
 
 struct Request {
@@ -70,6 +81,10 @@ struct Request {
   float radius = 0.f;
   int32_t* out_idx = nullptr;
   float* out_d2 = nullptr;
+  float* h_q = nullptr;     // pinned, m x dim (compacted queries)
+  int32_t* h_i = nullptr;   // pinned, m x k
+  float* h_d = nullptr;     // pinned, m x k
+  size_t cap_q = 0, cap_i = 0, cap_d = 0;
   double t_submit = 0.0;
   double latency = 0.0;
   int status = NABO_HIP_OK;
@@ -83,13 +98,47 @@ struct Slot {
   bool busy = false;
   int32_t cloud = -1;
   std::vector<Request*> reqs;
-  int64_t cap_q = 0, cap_r = 0;  // floats of query staging, entries of result staging
+  int64_t cap_q = 0, cap_r = 0;  // floats of the query buffer, entries of the result buffers
   float* d_q = nullptr;
   int32_t* d_i = nullptr;
   float* d_d = nullptr;
-  float* h_q = nullptr;
-  int32_t* h_i = nullptr;
-  float* h_d = nullptr;
+};
+
+// Pinned host buffers by power-of-two size class, reused across requests
+// (hipHostMalloc is far slower than a pool hit).
+class PinnedPool {
+ public:
+  void* get(size_t bytes, size_t* cap) {
+    size_t c = 256;
+    while (c < bytes) c <<= 1;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      auto& v = free_[c];
+      if (!v.empty()) {
+        void* p = v.back();
+        v.pop_back();
+        *cap = c;
+        return p;
+      }
+    }
+    void* p = nullptr;
+    NH_CHECK(hipHostMalloc(&p, c));
+    *cap = c;
+    return p;
+  }
+  void put(void* p, size_t cap) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lock(mu_);
+    free_[cap].push_back(p);
+  }
+  ~PinnedPool() {
+    for (auto& kv : free_)
+      for (void* p : kv.second) (void)hipHostFree(p);
+  }
+
+ private:
+  std::mutex mu_;
+  std::unordered_map<size_t, std::vector<void*>> free_;
 };
 
 }  // namespace
@@ -102,9 +151,13 @@ struct nabo_hip_server {
   std::condition_variable finished;   // waiters
   std::deque<Request*> pending;
   std::unordered_map<int64_t, Request*> requests;
-  std::vector<Cloud> clouds;
+  std::vector<std::unique_ptr<Cloud>> clouds;  // stable addresses
   std::vector<Slot> slots;
   std::thread worker;
+  std::thread builder;
+  std::condition_variable build_wake;
+  std::deque<int32_t> build_queue;
+  PinnedPool pinned;
   bool stop = false;
   int64_t next_id = 1;
   uint64_t clock = 0;
@@ -112,15 +165,23 @@ struct nabo_hip_server {
   nabo_hip_server_stats stats{};
 
   void run();
+  void run_builder();
   bool retire();
   bool admit();
-  void make_resident(Cloud& c, int32_t self);
   void ensure_capacity(Slot& s, int64_t q_floats, int64_t results);
   void release_slot_buffers(Slot& s);
   int64_t slot_bytes() const {
     int64_t b = 0;
     for (const auto& s : slots) b += s.cap_q * 4 + s.cap_r * 8;
     return b;
+  }
+  void release(Request* r) {
+    pinned.put(r->h_q, r->cap_q);
+    pinned.put(r->h_i, r->cap_i);
+    pinned.put(r->h_d, r->cap_d);
+    r->h_q = nullptr;
+    r->h_i = nullptr;
+    r->h_d = nullptr;
   }
 };
 
@@ -148,26 +209,17 @@ void check(int rc) {
 void nabo_hip_server::ensure_capacity(Slot& s, int64_t q_floats, int64_t results) {
   if (q_floats > s.cap_q) {
     if (s.d_q) (void)hipFree(s.d_q);
-    if (s.h_q) (void)hipHostFree(s.h_q);
     s.d_q = nullptr;
-    s.h_q = nullptr;
     NH_CHECK(hipMalloc(reinterpret_cast<void**>(&s.d_q), sizeof(float) * q_floats));
-    NH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s.h_q), sizeof(float) * q_floats));
     s.cap_q = q_floats;
   }
   if (results > s.cap_r) {
     if (s.d_i) (void)hipFree(s.d_i);
     if (s.d_d) (void)hipFree(s.d_d);
-    if (s.h_i) (void)hipHostFree(s.h_i);
-    if (s.h_d) (void)hipHostFree(s.h_d);
     s.d_i = nullptr;
     s.d_d = nullptr;
-    s.h_i = nullptr;
-    s.h_d = nullptr;
     NH_CHECK(hipMalloc(reinterpret_cast<void**>(&s.d_i), sizeof(int32_t) * results));
     NH_CHECK(hipMalloc(reinterpret_cast<void**>(&s.d_d), sizeof(float) * results));
-    NH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s.h_i), sizeof(int32_t) * results));
-    NH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&s.h_d), sizeof(float) * results));
     s.cap_r = results;
   }
 }
@@ -176,41 +228,67 @@ void nabo_hip_server::release_slot_buffers(Slot& s) {
   if (s.d_q) (void)hipFree(s.d_q);
   if (s.d_i) (void)hipFree(s.d_i);
   if (s.d_d) (void)hipFree(s.d_d);
-  if (s.h_q) (void)hipHostFree(s.h_q);
-  if (s.h_i) (void)hipHostFree(s.h_i);
-  if (s.h_d) (void)hipHostFree(s.h_d);
   s = Slot{s.stream, s.event};
 }
 
-// Builds the tree of `c` if needed, first evicting least recently used trees
+// Builder thread: for each queued cloud, evicts least recently used trees
 // that no slot uses until the budget holds (the budget is a target: a tree
-// larger than it is still built when nothing is left to evict).
-void nabo_hip_server::make_resident(Cloud& c, int32_t self) {
-  if (c.index) return;
-  const int64_t estimate = c.n * int64_t(sizeof(float)) * 4 + c.n * 8;
-  const int64_t budget = cfg.hbm_budget_bytes;
-  while (budget > 0 && resident + slot_bytes() + estimate > budget) {
-    int32_t victim = -1;
-    for (int32_t i = 0; i < int32_t(clouds.size()); ++i) {
-      const Cloud& o = clouds[i];
-      if (i == self || !o.index || o.in_use) continue;
-      if (victim < 0 || o.last_use < clouds[victim].last_use) victim = i;
+// larger than it is still built when nothing is left to evict), then builds
+// the tree on this thread's stream.
+void nabo_hip_server::run_builder() {
+  (void)hipSetDevice(device);
+  std::unique_lock<std::mutex> lock(mu);
+  for (;;) {
+    build_wake.wait(lock, [&] { return stop || !build_queue.empty(); });
+    if (build_queue.empty()) {
+      if (stop) break;
+      continue;
     }
-    if (victim < 0) break;
-    check(nabo_hip_destroy(clouds[victim].index));
-    clouds[victim].index = nullptr;
-    resident -= clouds[victim].bytes;
-    clouds[victim].bytes = 0;
-    ++stats.evictions;
+    const int32_t id = build_queue.front();
+    build_queue.pop_front();
+    Cloud& c = *clouds[id];
+    const int64_t estimate = c.n * int64_t(sizeof(float)) * 4 + c.n * 8;
+    // Trees with pending requests are not evicted: a tree built for waiting
+    // requests stays until they have run, so builds cannot undo each other.
+    std::vector<char> wanted(clouds.size(), 0);
+    for (const Request* r : pending) wanted[size_t(r->cloud)] = 1;
+    std::vector<nabo_hip_index*> victims;
+    while (cfg.hbm_budget_bytes > 0 && resident + slot_bytes() + estimate > cfg.hbm_budget_bytes) {
+      Cloud* victim = nullptr;
+      for (size_t i = 0; i < clouds.size(); ++i) {
+        Cloud* o = clouds[i].get();
+        if (o != &c && o->index && !o->in_use && !wanted[i] &&
+            (!victim || o->last_use < victim->last_use))
+          victim = o;
+      }
+      if (!victim) break;
+      victims.push_back(victim->index);
+      victim->index = nullptr;
+      resident -= victim->bytes;
+      victim->bytes = 0;
+      ++stats.evictions;
+    }
+    lock.unlock();
+    for (auto* v : victims) nabo_hip_destroy(v);
+    nabo_hip_index* index = nullptr;
+    const int rc = nabo_hip_create(c.data.data(), c.n, c.rows, c.stride, c.dim,
+                                   NABO_HIP_SEARCH_KDTREE, 0, &c.params, &index);
+    std::string err = rc == NABO_HIP_OK ? "" : nabo_hip_last_error();
+    nabo_hip_info info{};
+    if (rc == NABO_HIP_OK) nabo_hip_get_info(index, &info);
+    lock.lock();
+    c.building = false;
+    c.build_status = rc;
+    c.build_error = err;
+    if (rc == NABO_HIP_OK) {
+      c.index = index;
+      c.bytes = info.device_bytes;
+      resident += c.bytes;
+      ++stats.builds;
+      stats.peak_bytes = std::max(stats.peak_bytes, resident + slot_bytes());
+    }
+    wake.notify_all();
   }
-  check(nabo_hip_create(c.data.data(), c.n, c.rows, c.stride, c.dim, NABO_HIP_SEARCH_KDTREE, 0,
-                        &c.params, &c.index));
-  nabo_hip_info info;
-  check(nabo_hip_get_info(c.index, &info));
-  c.bytes = info.device_bytes;
-  resident += c.bytes;
-  ++stats.builds;
-  stats.peak_bytes = std::max(stats.peak_bytes, resident + slot_bytes());
 }
 
 bool nabo_hip_server::retire() {
@@ -219,22 +297,17 @@ bool nabo_hip_server::retire() {
     if (!s.busy) continue;
     const hipError_t q = hipEventQuery(s.event);
     if (q == hipErrorNotReady) continue;
-    int64_t off = 0;
     const double t = now_ms();
     for (Request* r : s.reqs) {
       if (q != hipSuccess) {
         r->status = NABO_HIP_ERR_HIP;
         r->error = hipGetErrorString(q);
-      } else {
-        std::memcpy(r->out_idx, s.h_i + off * r->k, sizeof(int32_t) * r->m * r->k);
-        std::memcpy(r->out_d2, s.h_d + off * r->k, sizeof(float) * r->m * r->k);
       }
-      off += r->m;
       r->latency = t - r->t_submit;
       r->done = true;
       ++stats.completed;
     }
-    clouds[s.cloud].in_use--;
+    clouds[s.cloud]->in_use--;
     s.reqs.clear();
     s.busy = false;
     any = true;
@@ -245,61 +318,90 @@ bool nabo_hip_server::retire() {
 
 bool nabo_hip_server::admit() {
   bool any = false;
-  for (auto& s : slots) {
-    if (s.busy || pending.empty()) continue;
-    Request* head = pending.front();
-    Cloud& c = clouds[head->cloud];
+  auto fail = [&](Request* r, int code, const std::string& msg) {
+    r->status = code;
+    r->error = msg;
+    r->done = true;
+    r->latency = now_ms() - r->t_submit;
+    ++stats.completed;
+    finished.notify_all();
+  };
+  for (auto it = pending.begin(); it != pending.end();) {
+    Request* head = *it;
+    Cloud& c = *clouds[head->cloud];
+    if (!c.index) {
+      if (c.build_status != NABO_HIP_OK && !c.building) {  // a failed build fails its requests
+        fail(head, c.build_status, c.build_error);
+        it = pending.erase(it);
+        any = true;
+        continue;
+      }
+      if (!c.building) {
+        c.building = true;
+        build_queue.push_back(head->cloud);
+        build_wake.notify_one();
+      }
+      ++it;
+      continue;
+    }
+    Slot* s = nullptr;
+    for (auto& sl : slots)
+      if (!sl.busy) {
+        s = &sl;
+        break;
+      }
+    if (!s) break;
+    // The group: head and the pending requests it can share a launch with.
+    // Erasing from the middle of a deque invalidates every iterator, so the
+    // scan of `pending` restarts from its beginning afterwards.
     std::vector<Request*> group;
     int64_t m = 0;
-    for (auto it = pending.begin(); it != pending.end();) {
-      Request* r = *it;
+    for (auto jt = it; jt != pending.end();) {
+      Request* r = *jt;
       const bool same = r->cloud == head->cloud && r->k == head->k && r->eps == head->eps &&
                         r->flags == head->flags && r->radius == head->radius;
       if (same && (group.empty() || m + r->m <= cfg.max_merge_queries)) {
         group.push_back(r);
         m += r->m;
-        it = pending.erase(it);
+        jt = pending.erase(jt);
       } else {
-        ++it;
+        ++jt;
       }
     }
+    it = pending.begin();
     try {
-      make_resident(c, head->cloud);
       const int32_t dim = c.dim;
-      ensure_capacity(s, std::max<int64_t>(1, m) * dim, std::max<int64_t>(1, m) * head->k);
+      const int32_t kk = head->k;
+      ensure_capacity(*s, std::max<int64_t>(1, m) * dim, std::max<int64_t>(1, m) * kk);
       int64_t off = 0;
-      for (Request* r : group) {  // compact each query set to dim floats per point
-        for (int64_t i = 0; i < r->m; ++i)
-          std::memcpy(s.h_q + (off + i) * dim, r->query + i * r->qstride, sizeof(float) * dim);
+      for (Request* r : group) {
+        NH_CHECK(hipMemcpyAsync(s->d_q + off * dim, r->h_q, sizeof(float) * r->m * dim,
+                                hipMemcpyHostToDevice, s->stream));
         off += r->m;
       }
-      NH_CHECK(hipMemcpyAsync(s.d_q, s.h_q, sizeof(float) * m * dim, hipMemcpyHostToDevice,
-                              s.stream));
-      check(nabo_hip_knn_device(c.index, s.d_q, dim, dim, m, head->k, head->eps, head->flags,
-                                head->radius, nullptr, s.d_i, s.d_d, nullptr, s.stream));
-      NH_CHECK(hipMemcpyAsync(s.h_i, s.d_i, sizeof(int32_t) * m * head->k,
-                              hipMemcpyDeviceToHost, s.stream));
-      NH_CHECK(hipMemcpyAsync(s.h_d, s.d_d, sizeof(float) * m * head->k, hipMemcpyDeviceToHost,
-                              s.stream));
-      NH_CHECK(hipEventRecord(s.event, s.stream));
-    } catch (const Error& e) {
+      check(nabo_hip_knn_device(c.index, s->d_q, dim, dim, m, kk, head->eps, head->flags,
+                                head->radius, nullptr, s->d_i, s->d_d, nullptr, s->stream));
+      off = 0;
       for (Request* r : group) {
-        r->status = e.code;
-        r->error = e.what();
-        r->done = true;
-        r->latency = now_ms() - r->t_submit;
-        ++stats.completed;
+        NH_CHECK(hipMemcpyAsync(r->h_i, s->d_i + off * kk, sizeof(int32_t) * r->m * kk,
+                                hipMemcpyDeviceToHost, s->stream));
+        NH_CHECK(hipMemcpyAsync(r->h_d, s->d_d + off * kk, sizeof(float) * r->m * kk,
+                                hipMemcpyDeviceToHost, s->stream));
+        off += r->m;
       }
-      finished.notify_all();
+      NH_CHECK(hipEventRecord(s->event, s->stream));
+    } catch (const Error& e) {
+      for (Request* r : group) fail(r, e.code, e.what());
+      any = true;
       continue;
     }
     c.in_use++;
     c.last_use = ++clock;
-    s.cloud = head->cloud;
-    s.reqs = std::move(group);
-    s.busy = true;
+    s->cloud = head->cloud;
+    s->reqs = std::move(group);
+    s->busy = true;
     ++stats.launches;
-    stats.merged_requests += int64_t(s.reqs.size()) - 1;
+    stats.merged_requests += int64_t(s->reqs.size()) - 1;
     stats.peak_bytes = std::max(stats.peak_bytes, resident + slot_bytes());
     any = true;
   }
@@ -314,6 +416,7 @@ void nabo_hip_server::run() {
       return s.busy;
     });
     if (stop && pending.empty() && !busy_slots) break;
+    (void)busy_slots;
     const bool progressed = retire() | admit();
     if (progressed) continue;
     if (busy_slots) {
@@ -321,8 +424,12 @@ void nabo_hip_server::run() {
       lock.unlock();
       std::this_thread::sleep_for(std::chrono::microseconds(20));
       lock.lock();
+    } else if (std::any_of(pending.begin(), pending.end(),
+                           [&](Request* r) { return clouds[r->cloud]->index != nullptr; })) {
+      continue;
     } else {
-      wake.wait(lock, [&] { return stop || !pending.empty(); });
+      // nothing runnable: wait for a submit, a finished build or shutdown
+      wake.wait_for(lock, std::chrono::milliseconds(50));
     }
   }
 }
@@ -360,6 +467,7 @@ int nabo_hip_server_create(const nabo_hip_server_config* config, nabo_hip_server
       NH_CHECK(hipEventCreateWithFlags(&sl.event, hipEventDisableTiming));
     }
     s->worker = std::thread([s] { s->run(); });
+    s->builder = std::thread([s] { s->run_builder(); });
     *out = s;
   });
 }
@@ -373,6 +481,8 @@ int nabo_hip_server_destroy(nabo_hip_server* s) {
     }
     s->wake.notify_all();
     if (s->worker.joinable()) s->worker.join();
+    s->build_wake.notify_all();
+    if (s->builder.joinable()) s->builder.join();
     (void)hipSetDevice(s->device);
     for (auto& sl : s->slots) {
       s->release_slot_buffers(sl);
@@ -380,8 +490,11 @@ int nabo_hip_server_destroy(nabo_hip_server* s) {
       (void)hipStreamDestroy(sl.stream);
     }
     for (auto& c : s->clouds)
-      if (c.index) nabo_hip_destroy(c.index);
-    for (auto& kv : s->requests) delete kv.second;
+      if (c->index) nabo_hip_destroy(c->index);
+    for (auto& kv : s->requests) {
+      s->release(kv.second);
+      delete kv.second;
+    }
     delete s;
   });
 }
@@ -393,16 +506,16 @@ int nabo_hip_server_add_cloud(nabo_hip_server* s, const float* cloud, int64_t n,
     if (!s || !cloud || !cloud_id) throw Error(NABO_HIP_ERR_ARGUMENT, "null argument");
     if (n <= 0 || rows <= 0 || stride < rows || dim <= 0)
       throw Error(NABO_HIP_ERR_ARGUMENT, "invalid cloud shape");
-    Cloud c;
-    c.n = n;
-    c.rows = rows;
-    c.stride = rows;
-    c.dim = std::min(dim, rows);
-    nabo_hip_params_default(&c.params);
-    if (params) c.params = *params;
-    c.data.resize(size_t(n) * rows);
+    auto c = std::make_unique<Cloud>();
+    c->n = n;
+    c->rows = rows;
+    c->stride = rows;
+    c->dim = std::min(dim, rows);
+    nabo_hip_params_default(&c->params);
+    if (params) c->params = *params;
+    c->data.resize(size_t(n) * rows);
     for (int64_t i = 0; i < n; ++i)
-      std::memcpy(c.data.data() + i * rows, cloud + i * stride, sizeof(float) * rows);
+      std::memcpy(c->data.data() + i * rows, cloud + i * stride, sizeof(float) * rows);
     std::lock_guard<std::mutex> lock(s->mu);
     *cloud_id = int32_t(s->clouds.size());
     s->clouds.push_back(std::move(c));
@@ -416,13 +529,34 @@ int nabo_hip_server_submit(nabo_hip_server* s, int32_t cloud_id, const float* qu
   return server_guard([&] {
     if (!s || !request_id) throw Error(NABO_HIP_ERR_ARGUMENT, "null argument");
     auto* r = new Request();
+    int32_t dim = 0;
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      if (cloud_id >= 0 && cloud_id < int32_t(s->clouds.size())) dim = s->clouds[cloud_id]->dim;
+    }
+    // Stage the compacted queries and take the result buffers here, in the
+    // caller's thread.
+    if (m > 0 && k > 0 && query && dim > 0 && qrows >= dim && qstride >= qrows) {
+      try {
+        r->h_q = static_cast<float*>(s->pinned.get(sizeof(float) * m * dim, &r->cap_q));
+        r->h_i = static_cast<int32_t*>(s->pinned.get(sizeof(int32_t) * m * k, &r->cap_i));
+        r->h_d = static_cast<float*>(s->pinned.get(sizeof(float) * m * k, &r->cap_d));
+      } catch (...) {
+        s->release(r);
+        delete r;
+        throw;
+      }
+      for (int64_t i = 0; i < m; ++i)
+        std::memcpy(r->h_q + i * dim, query + i * qstride, sizeof(float) * dim);
+    }
     {
       std::lock_guard<std::mutex> lock(s->mu);
       if (cloud_id < 0 || cloud_id >= int32_t(s->clouds.size())) {
+        s->release(r);
         delete r;
         throw Error(NABO_HIP_ERR_ARGUMENT, "unknown cloud id");
       }
-      const Cloud& c = s->clouds[cloud_id];
+      const Cloud& c = *s->clouds[cloud_id];
       const bool self = option_flags & NABO_HIP_ALLOW_SELF_MATCH;
       std::string err;
       if (self && k > c.n)
@@ -441,6 +575,7 @@ int nabo_hip_server_submit(nabo_hip_server* s, int32_t cloud_id, const float* qu
       else if (k < 1 || m < 0 || (m > 0 && (!query || !indices || !dists2 || qstride < qrows)))
         err = "invalid request arguments";
       if (!err.empty()) {
+        s->release(r);
         delete r;
         throw Error(NABO_HIP_ERR_ARGUMENT, err);
       }
@@ -484,6 +619,12 @@ int nabo_hip_server_wait(nabo_hip_server* s, int64_t request_id, double* latency
     status = r->status;
     const std::string err = r->error;
     s->requests.erase(it);
+    lock.unlock();
+    if (status == NABO_HIP_OK && r->m > 0) {
+      std::memcpy(r->out_idx, r->h_i, sizeof(int32_t) * r->m * r->k);
+      std::memcpy(r->out_d2, r->h_d, sizeof(float) * r->m * r->k);
+    }
+    s->release(r);
     delete r;
     if (status != NABO_HIP_OK) throw Error(status, err);
   });
@@ -497,7 +638,7 @@ int nabo_hip_server_get_stats(const nabo_hip_server* s, nabo_hip_server_stats* o
     std::lock_guard<std::mutex> lock(ms->mu);
     *out = s->stats;
     int64_t trees = 0;
-    for (const auto& c : s->clouds) trees += c.index != nullptr;
+    for (const auto& c : s->clouds) trees += c->index != nullptr;
     out->resident_trees = trees;
     out->resident_bytes = s->resident;
   });

@@ -8,6 +8,8 @@ column-major d x n matrix libnabo takes (nabo/nabo.h:260-264).
                        queries = the cloud under a small rigid motion + noise
     dirichlet_descriptors  200k x 33 histogram-like descriptors + jittered queries
     uneven_clouds      512 clouds of log-uniform size 2k..200k for batching
+    served_pool        64 clouds of log-uniform size 10k..2M and a Poisson
+                       request stream over them (query batches 1k..500k)
 """
 from __future__ import annotations
 
@@ -121,3 +123,25 @@ def load_reference_scan(name: str) -> np.ndarray:
 
     here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     return np.load(os.path.join(here, "tests", "golden", f"{name}.npy"))
+
+
+def served_pool(count: int = 64, lo: int = 10_000, hi: int = 2_000_000, seed: int = 5):
+    """Clouds U[0,1)^3 x scale U[1,50] m, sizes log-uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(math.log(lo), math.log(hi), size=count))).astype(int)
+    return [(rng.random((s, 3)) * rng.uniform(1, 50)).astype(np.float32) for s in sizes]
+
+
+def served_requests(pool, count: int = 150, lo: int = 1_000, hi: int = 500_000,
+                    ks=(1, 8, 32), radius: float = 0.5, seed: int = 6):
+    """Requests (cloud id, queries, k, radius): each query batch is a
+    log-uniform number of cloud points perturbed by N(0, 0.01 m)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        c = int(rng.integers(0, len(pool)))
+        m = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        base = pool[c][rng.integers(0, len(pool[c]), m)]
+        q = (base + rng.normal(scale=0.01, size=base.shape)).astype(np.float32)
+        out.append((c, q, int(rng.choice(ks)), radius))
+    return out
