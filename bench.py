@@ -27,6 +27,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per search launch on the default workload (lidar 1M x 1M, k=16,
+# bucket 8), from two rocprofv3 counter passes scaled by a 256 MiB copy:
+# FETCH_SIZE 50,833 KiB x 2.000 + WRITE_SIZE 128,711 KiB x 1.000
+# (profiles/kd_search_traffic_lidar1M_k16.md).
+MEASURED_TRAFFIC_BYTES = (50_833 * 2.000 + 128_711 * 1.000) * 1024
 METRIC = "exact kNN queries/s, 1M 3D pts k=16; batched+served vs plain GPU; p99 ms"
 
 
@@ -43,6 +48,8 @@ def parse():
     ap.add_argument("--no-query-sort", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (counter calibration)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline line (no brute-force / batched / served sections)")
     return ap.parse_args()
@@ -444,6 +451,16 @@ def main():
     log(f"ms/step {ms_per_step:.3f} (wall {wall * 1e3 / args.steps:.3f}), search kernel "
         f"{kernel_ms:.3f} ms over {launches} launches")
 
+    if args.calibrate_copy:
+        # A plain copy of known size: 256 MiB read + 256 MiB written per
+        # launch, for scaling FETCH_SIZE / WRITE_SIZE counters.
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     gi = d_idx.cpu().numpy()
     gd = d_d2.cpu().numpy()
 
@@ -464,9 +481,12 @@ def main():
     if p_sum is not None:
         algo_bytes = m * (12 + 8 * k) + 16 * p_sum + 8 * v_sum
         achieved = algo_bytes / (kernel_ms / 1e3) / 1e9
+        default_cfg = (args.n == 1_000_000 and k == 16 and args.bucket == 8 and args.eps == 0.0
+                       and args.radius == 1.0 and not args.no_query_sort)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": None, "algorithmic_bytes": int(algo_bytes),
+                    "traffic": int(MEASURED_TRAFFIC_BYTES) if default_cfg else None,
+                    "algorithmic_bytes": int(algo_bytes),
                     "kernel": "k_kd_search<3,16>", "kernel_ms": round(kernel_ms, 4)}
 
     out = {
