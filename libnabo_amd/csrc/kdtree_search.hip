@@ -93,6 +93,7 @@ struct SearchParams {
   int32_t* out_idx;
   float* out_d2;
   unsigned long long* touched;
+  uint32_t* work;  // kXcds chunk counters (zeroed before the launch)
   uint2* stack;
   int64_t stack_lanes;
   int allow_self;
@@ -186,9 +187,11 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
 
 // Work mapping: the queries (in leaf order) are cut into chunks of 64, one
 // per wave at a time. Workgroups are dispatched round-robin over the 8 XCDs,
-// so workgroup b runs on XCD b % 8; each XCD is given one contiguous eighth
-// of the chunks, which keeps the tree region it touches small enough for
-// its own L2.
+// so workgroup b runs on XCD b % 8; each XCD owns one contiguous eighth of
+// the chunks, which keeps the tree region it touches small, and its waves
+// take chunks from it through an atomic counter; a wave whose XCD's eighth
+// is exhausted takes chunks from the other eighths, so no XCD idles while
+// another still has work.
 template <int DIM, int K, bool LDS_STACK>
 __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
   // Per-lane stack: in LDS when the tree is shallow enough ([slot][lane] of
@@ -196,15 +199,9 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
   extern __shared__ uint2 lds_stack[];
   uint2* const stack_base = LDS_STACK ? lds_stack : a.stack;
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  constexpr int kWavesPerBlock = kBlock / kWave;
   const int64_t lane_id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int xcd = blockIdx.x % kXcds;
-  const int64_t local = blockIdx.x / kXcds;
-  const int64_t xcd_blocks = gridDim.x / kXcds;
   const int64_t chunks = (a.m + kWave - 1) / kWave;
-  const int64_t c_begin = chunks * xcd / kXcds;
-  const int64_t c_end = chunks * (xcd + 1) / kXcds;
 
   const int dim = DIM > 0 ? DIM : a.dim;
   const uint32_t leaf_tag = uint32_t(dim);
@@ -215,8 +212,24 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
   const uint32_t sstride = LDS_STACK ? uint32_t(kBlock) : uint32_t(a.stack_lanes);
   uint64_t touched = 0;
 
-  for (int64_t c = c_begin + local * kWavesPerBlock + wave; c < c_end;
-       c += xcd_blocks * kWavesPerBlock) {
+  int owner = xcd;   // eighth being drained
+  int visited = 0;   // eighths found exhausted
+  for (;;) {
+    int64_t c = -1;
+    while (visited < kXcds) {
+      uint32_t got = 0;
+      if (lane == 0) got = atomicAdd(&a.work[owner], 1u);
+      got = __shfl(got, 0, kWave);
+      const int64_t begin = chunks * owner / kXcds;
+      const int64_t end = chunks * (owner + 1) / kXcds;
+      if (begin + int64_t(got) < end) {
+        c = begin + got;
+        break;
+      }
+      owner = (owner + 1) % kXcds;
+      ++visited;
+    }
+    if (c < 0) break;
     const int64_t i = c * kWave + lane;
     if (i >= a.m) continue;
     const int64_t qi = a.order ? a.order[i] : i;
@@ -346,35 +359,62 @@ __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nod
   vals[i] = int32_t(i);
 }
 
+__global__ void k_zero_u32(uint32_t* p, int n) {
+  if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
+}
+
+// Persistent grid: as many workgroups as the device holds at once (from
+// the occupancy of this instantiation), a multiple of the XCD count.
+template <int DIM, int K, bool LDS>
+unsigned resident_grid(size_t lds, int cus) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kd_search<DIM, K, LDS>, kBlock,
+                                                   lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return unsigned(std::max(kXcds, (cus * per_cu) / kXcds * kXcds));
+}
+
 template <int DIM, int K>
-void launch_search(const SearchParams& p, unsigned grid, int depth, hipStream_t s) {
+void launch_search(SearchParams p, int cus, int depth, DevBuf<uint2>& stack, hipStream_t s) {
+  k_zero_u32<<<1, 64, 0, s>>>(p.work, kXcds);
   if (depth <= kLdsStackSlots) {
     const size_t lds = size_t(depth) * kBlock * sizeof(uint2);
+    const unsigned grid = resident_grid<DIM, K, true>(lds, cus);
+    KernelTimer timer(s);
     k_kd_search<DIM, K, true><<<grid, kBlock, lds, s>>>(p);
   } else {
+    const unsigned grid = resident_grid<DIM, K, false>(0, cus);
+    const int64_t lanes = int64_t(grid) * kBlock;
+    if (uint64_t(lanes) * uint64_t(depth + 1) >= (uint64_t(1) << 32))
+      throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
+    stack.alloc(size_t(lanes) * size_t(depth), s);
+    p.stack = stack.p;
+    p.stack_lanes = lanes;
+    KernelTimer timer(s);
     k_kd_search<DIM, K, false><<<grid, kBlock, 0, s>>>(p);
   }
 }
 
 template <int DIM>
-void dispatch_k(const SearchParams& p, unsigned grid, int depth, hipStream_t s) {
+void dispatch_k(const SearchParams& p, int cus, int depth, DevBuf<uint2>& stack, hipStream_t s) {
   const int k = p.k;
   if (k <= 1)
-    launch_search<DIM, 1>(p, grid, depth, s);
+    launch_search<DIM, 1>(p, cus, depth, stack, s);
   else if (k <= 2)
-    launch_search<DIM, 2>(p, grid, depth, s);
+    launch_search<DIM, 2>(p, cus, depth, stack, s);
   else if (k <= 4)
-    launch_search<DIM, 4>(p, grid, depth, s);
+    launch_search<DIM, 4>(p, cus, depth, stack, s);
   else if (k <= 8)
-    launch_search<DIM, 8>(p, grid, depth, s);
+    launch_search<DIM, 8>(p, cus, depth, stack, s);
   else if (k <= 16)
-    launch_search<DIM, 16>(p, grid, depth, s);
+    launch_search<DIM, 16>(p, cus, depth, stack, s);
   else if (k <= 32)
-    launch_search<DIM, 32>(p, grid, depth, s);
+    launch_search<DIM, 32>(p, cus, depth, stack, s);
   else if (k <= 64)
-    launch_search<DIM, 64>(p, grid, depth, s);
+    launch_search<DIM, 64>(p, cus, depth, stack, s);
   else
-    launch_search<DIM, 128>(p, grid, depth, s);
+    launch_search<DIM, 128>(p, cus, depth, stack, s);
 }
 
 uint32_t bits_needed(uint64_t v) {
@@ -458,32 +498,20 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     p.order = order.p;
   }
 
-  // Persistent grid: enough lanes to fill the device, one stack per lane.
   int dev = 0, cus = 256;
   NH_CHECK(hipGetDevice(&dev));
   NH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int64_t max_blocks = int64_t(cus) * 8;
-  int64_t blocks = std::min<int64_t>(ceil_div(a.m, kBlock), max_blocks);
-  blocks = std::max<int64_t>(kXcds, ceil_div(blocks, kXcds) * kXcds);  // multiple of the XCDs
-  const int64_t lanes = blocks * kBlock;
   const int depth = std::max(1, ix.depth);
-  if (uint64_t(lanes) * uint64_t(depth + 1) >= (uint64_t(1) << 32))
-    throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
+  DevBuf<uint32_t> work(kXcds, s);
+  p.work = work.p;
   DevBuf<uint2> stack;
-  if (depth > kLdsStackSlots) stack.alloc(size_t(lanes) * size_t(depth), s);
-  p.stack = stack.p;
-  p.stack_lanes = lanes;
-
-  {
-    KernelTimer timer(s);
-    if (ix.dim == 3)
-      dispatch_k<3>(p, unsigned(blocks), depth, s);
-    else if (ix.dim == 2)
-      dispatch_k<2>(p, unsigned(blocks), depth, s);
-    else
-      dispatch_k<0>(p, unsigned(blocks), depth, s);
-    NH_LAUNCH_CHECK();
-  }
+  if (ix.dim == 3)
+    dispatch_k<3>(p, cus, depth, stack, s);
+  else if (ix.dim == 2)
+    dispatch_k<2>(p, cus, depth, stack, s);
+  else
+    dispatch_k<0>(p, cus, depth, stack, s);
+  NH_LAUNCH_CHECK();
 }
 
 }  // namespace nabo_hip
