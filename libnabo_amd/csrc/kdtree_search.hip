@@ -27,7 +27,6 @@
 
 #include <cstring>
 
-#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include <cfloat>
@@ -340,7 +339,8 @@ __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nod
                                                    const int32_t* __restrict__ roots,
                                                    const int32_t* __restrict__ qcloud,
                                                    uint32_t* __restrict__ keys,
-                                                   int32_t* __restrict__ vals) {
+                                                   int32_t* __restrict__ vals,
+                                                   uint32_t* __restrict__ hist) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const int dim = DIM > 0 ? DIM : dim_rt;
@@ -350,13 +350,30 @@ __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nod
     const uint2 nd = nodes[n];
     const uint32_t cd = nd.x & dim_mask;
     if (cd == uint32_t(dim)) {
-      keys[i] = nd.y >> shift;
+      const uint32_t key = nd.y >> shift;
+      keys[i] = key;
+      vals[i] = int32_t(atomicAdd(&hist[key], 1u));  // rank within its bucket
       break;
     }
     const float new_off = qp[cd] - __uint_as_float(nd.y);
     n = new_off > 0 ? (nd.x >> dim_bits) : n + 1;
   }
-  vals[i] = int32_t(i);
+}
+
+// Counting sort of the queries by key: position = start of the key's bucket
+// + rank of the query in it (the order inside a bucket is arbitrary; results
+// do not depend on the order in which queries are searched).
+__global__ void k_zero_hist(uint32_t* hist, int64_t n) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < n) hist[e] = 0;
+}
+
+__global__ void k_place(int64_t m, const uint32_t* __restrict__ keys,
+                        const int32_t* __restrict__ ranks, const uint32_t* __restrict__ starts,
+                        int32_t* __restrict__ order) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  order[starts[keys[i]] + uint32_t(ranks[i])] = int32_t(i);
 }
 
 __global__ void k_zero_u32(uint32_t* p, int n) {
@@ -464,37 +481,43 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   }
 
   // Query order by home leaf (bucket starts are global, so this also groups
-  // the queries of batched trees by cloud).
-  DevBuf<uint32_t> keys, keys_sorted;
-  DevBuf<int32_t> vals, order;
-  DevBuf<uint8_t> sort_tmp;
+  // the queries of batched trees by cloud): key = the leaf's bucket start cut
+  // to 16 bits, counting sort.
+  DevBuf<uint32_t> keys, hist, starts;
+  DevBuf<int32_t> ranks, order;
+  DevBuf<uint8_t> scan_tmp;
   if (ix.params.query_sort && a.m > 1 && ix.node_count > 1) {
-    keys.alloc(size_t(a.m), s);
-    keys_sorted.alloc(size_t(a.m), s);
-    vals.alloc(size_t(a.m), s);
-    order.alloc(size_t(a.m), s);
-    // Key: the leaf's bucket start, cut to 16 bits (two radix passes); the
-    // dropped low bits only reorder queries of neighbouring leaves.
     const uint32_t full_bits = std::max(1u, bits_needed(uint64_t(ix.n)));
     const uint32_t shift = full_bits > 16 ? full_bits - 16 : 0;
+    const int64_t buckets = int64_t(1) << (full_bits - shift);
+    keys.alloc(size_t(a.m), s);
+    ranks.alloc(size_t(a.m), s);
+    order.alloc(size_t(a.m), s);
+    hist.alloc(size_t(buckets), s);
+    starts.alloc(size_t(buckets), s);
+    k_zero_hist<<<unsigned(ceil_div(buckets, kBlock)), kBlock, 0, s>>>(hist.p, buckets);
     const unsigned g = unsigned(ceil_div(a.m, kBlock));
     if (ix.dim == 3)
       k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, vals.p);
+                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, ranks.p,
+                                       hist.p);
     else if (ix.dim == 2)
       k_locate<2><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, vals.p);
+                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, ranks.p,
+                                       hist.p);
     else
       k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, vals.p);
+                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, ranks.p,
+                                       hist.p);
     NH_LAUNCH_CHECK();
-    const unsigned end_bit = full_bits - shift;
     size_t bytes = 0;
-    NH_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, keys.p, keys_sorted.p, vals.p, order.p,
-                                       size_t(a.m), 0, end_bit, s));
-    sort_tmp.alloc(bytes + 16, s);
-    NH_CHECK(rocprim::radix_sort_pairs(sort_tmp.p, bytes, keys.p, keys_sorted.p, vals.p, order.p,
-                                       size_t(a.m), 0, end_bit, s));
+    NH_CHECK(rocprim::exclusive_scan(nullptr, bytes, hist.p, starts.p, uint32_t(0),
+                                     size_t(buckets), rocprim::plus<uint32_t>(), s));
+    scan_tmp.alloc(bytes + 16, s);
+    NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, hist.p, starts.p, uint32_t(0),
+                                     size_t(buckets), rocprim::plus<uint32_t>(), s));
+    k_place<<<g, kBlock, 0, s>>>(a.m, keys.p, ranks.p, starts.p, order.p);
+    NH_LAUNCH_CHECK();
     p.order = order.p;
   }
 
