@@ -120,14 +120,16 @@ EXPORTED = [
     "nabo_hip_server_submit", "nabo_hip_server_wait", "nabo_hip_server_get_stats",
 ]
 
-_LIB = None
+_LIBS: dict = {}
 
 
-def load_library(path: str = LIB_PATH):
-    """Loads libnabo_hip.so (raises when it has not been built)."""
-    global _LIB
-    if _LIB is not None and path == LIB_PATH:
-        return _LIB
+def load_library(path: str | None = None):
+    """Loads libnabo_hip.so (raises when it has not been built). The default
+    path is the module's LIB_PATH as it is at the time of the call."""
+    if path is None:
+        path = LIB_PATH
+    if path in _LIBS:
+        return _LIBS[path]
     if not os.path.exists(path):
         raise NaboError(2, f"{path} is missing: build it with `python -m libnabo_amd.build`")
     # torch bundles its own HIP runtime under the same soname
@@ -176,8 +178,7 @@ def load_library(path: str = LIB_PATH):
         if name != "nabo_hip_server_config_default":
             getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.nabo_hip_last_error.restype = C.c_char_p
-    if path == LIB_PATH:
-        _LIB = lib
+    _LIBS[path] = lib
     return lib
 
 

@@ -14,16 +14,17 @@
 //            it (:419-424 / :437-442).
 // The far child's rd is formed from the same operands, in the same order,
 // as the reference, so the visit order, the pruning decisions, ties and
-// dists2 are bit-identical, for epsilon > 0 as well. The stack lives in HBM,
-// slot-major ([slot][lane]), so lanes at the same depth touch one line.
+// dists2 are bit-identical, for epsilon > 0 as well. The stack is
+// slot-major ([slot][lane]): in LDS when the tree is at most 32 levels deep,
+// else in HBM.
 //
 // The k-best array is K = next supported size >= k entries in VGPRs; the
 // first K - k entries hold -inf and never move, so the head (largest kept
 // distance) is always entry K - 1 and no register is indexed dynamically.
 //
 // Queries are first ordered by the leaf their greedy descent reaches, so the
-// lanes of a wave walk neighbouring paths (a radix sort on the leaf's bucket
-// start); results are written to each query's own column.
+// lanes of a wave walk neighbouring paths (a counting sort on the leaf's
+// bucket start); results are written to each query's own column.
 
 #include <cstring>
 
@@ -38,6 +39,9 @@ namespace nabo_hip {
 namespace {
 
 constexpr int kBlock = 256;
+// Search workgroups are one wave: the LDS stack of a wave is depth x 512 B,
+// and one-wave workgroups let the CU fill its LDS to the last wave's worth.
+constexpr int kSearchBlock = 64;
 constexpr uint32_t kRestore = 0x80000000u;
 constexpr int kMaxGenericDim = 64;
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
@@ -191,8 +195,10 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
 // take chunks from it through an atomic counter; a wave whose XCD's eighth
 // is exhausted takes chunks from the other eighths, so no XCD idles while
 // another still has work.
+// Up to K = 16 the k-best fits in 128 VGPRs without spills: ask for four
+// waves per SIMD.
 template <int DIM, int K, bool LDS_STACK>
-__global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
+__global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 4 : 1)) void k_kd_search(SearchParams a) {
   // Per-lane stack: in LDS when the tree is shallow enough ([slot][lane] of
   // this workgroup), else in HBM ([slot][lane] over the grid).
   extern __shared__ uint2 lds_stack[];
@@ -208,17 +214,25 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
   // running offset stays in 32 bits (lanes * depth < 2^32 is checked on the
   // host).
   const uint32_t base_off = LDS_STACK ? uint32_t(threadIdx.x) : uint32_t(lane_id);
-  const uint32_t sstride = LDS_STACK ? uint32_t(kBlock) : uint32_t(a.stack_lanes);
+  const uint32_t sstride = LDS_STACK ? uint32_t(kSearchBlock) : uint32_t(a.stack_lanes);
   uint64_t touched = 0;
 
+  // Output rows are written as 16-byte vectors when k = K and both output
+  // arrays are 16-byte aligned (rows are then 16-byte aligned too).
+  const bool vec_out =
+      K >= 4 && a.k == K &&
+      ((reinterpret_cast<uintptr_t>(a.out_idx) | reinterpret_cast<uintptr_t>(a.out_d2)) & 15) == 0;
+
+  // The claim of the next chunk is issued before the current chunk's
+  // results are stored, so its round trip overlaps the stores.
   int owner = xcd;   // eighth being drained
   int visited = 0;   // eighths found exhausted
+  uint32_t claimed = 0;
+  if (lane == 0) claimed = atomicAdd(&a.work[owner], 1u);
   for (;;) {
     int64_t c = -1;
+    uint32_t got = __shfl(claimed, 0, kWave);
     while (visited < kXcds) {
-      uint32_t got = 0;
-      if (lane == 0) got = atomicAdd(&a.work[owner], 1u);
-      got = __shfl(got, 0, kWave);
       const int64_t begin = chunks * owner / kXcds;
       const int64_t end = chunks * (owner + 1) / kXcds;
       if (begin + int64_t(got) < end) {
@@ -226,11 +240,18 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
         break;
       }
       owner = (owner + 1) % kXcds;
-      ++visited;
+      if (++visited < kXcds) {
+        got = 0;
+        if (lane == 0) got = atomicAdd(&a.work[owner], 1u);
+        got = __shfl(got, 0, kWave);
+      }
     }
     if (c < 0) break;
     const int64_t i = c * kWave + lane;
-    if (i >= a.m) continue;
+    if (i >= a.m) {
+      // Only the last chunk has such lanes, and its lane 0 is never one.
+      continue;
+    }
     const int64_t qi = a.order ? a.order[i] : i;
     Vec<DIM> q, off;
     const float* qp = a.query + qi * a.qstride;
@@ -303,13 +324,24 @@ __global__ __launch_bounds__(kBlock) void k_kd_search(SearchParams a) {
       if (!descend) break;
     }
 
+    if (lane == 0) claimed = atomicAdd(&a.work[owner], 1u);
     int32_t* oi = a.out_idx + qi * a.k;
     float* od = a.out_d2 + qi * a.k;
+    if (vec_out) {
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-      if (j >= K - a.k) {
-        oi[j - (K - a.k)] = hi[j];
-        od[j - (K - a.k)] = hd[j];
+      for (int j = 0; j < K / 4; ++j) {
+        reinterpret_cast<int4*>(oi)[j] =
+            make_int4(hi[4 * j], hi[4 * j + 1], hi[4 * j + 2], hi[4 * j + 3]);
+        reinterpret_cast<float4*>(od)[j] =
+            make_float4(hd[4 * j], hd[4 * j + 1], hd[4 * j + 2], hd[4 * j + 3]);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        if (j >= K - a.k) {
+          oi[j - (K - a.k)] = hi[j];
+          od[j - (K - a.k)] = hd[j];
+        }
       }
     }
   }
@@ -385,7 +417,7 @@ __global__ void k_zero_u32(uint32_t* p, int n) {
 template <int DIM, int K, bool LDS>
 unsigned resident_grid(size_t lds, int cus) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kd_search<DIM, K, LDS>, kBlock,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kd_search<DIM, K, LDS>, kSearchBlock,
                                                    lds) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -396,20 +428,20 @@ template <int DIM, int K>
 void launch_search(SearchParams p, int cus, int depth, DevBuf<uint2>& stack, hipStream_t s) {
   k_zero_u32<<<1, 64, 0, s>>>(p.work, kXcds);
   if (depth <= kLdsStackSlots) {
-    const size_t lds = size_t(depth) * kBlock * sizeof(uint2);
+    const size_t lds = size_t(depth) * kSearchBlock * sizeof(uint2);
     const unsigned grid = resident_grid<DIM, K, true>(lds, cus);
     KernelTimer timer(s);
-    k_kd_search<DIM, K, true><<<grid, kBlock, lds, s>>>(p);
+    k_kd_search<DIM, K, true><<<grid, kSearchBlock, lds, s>>>(p);
   } else {
     const unsigned grid = resident_grid<DIM, K, false>(0, cus);
-    const int64_t lanes = int64_t(grid) * kBlock;
+    const int64_t lanes = int64_t(grid) * kSearchBlock;
     if (uint64_t(lanes) * uint64_t(depth + 1) >= (uint64_t(1) << 32))
       throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
     stack.alloc(size_t(lanes) * size_t(depth), s);
     p.stack = stack.p;
     p.stack_lanes = lanes;
     KernelTimer timer(s);
-    k_kd_search<DIM, K, false><<<grid, kBlock, 0, s>>>(p);
+    k_kd_search<DIM, K, false><<<grid, kSearchBlock, 0, s>>>(p);
   }
 }
 
