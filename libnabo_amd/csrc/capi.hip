@@ -141,7 +141,8 @@ void build_index(nabo_hip_index* h, const float* d_cloud, int64_t n, int32_t row
   ix.dim_mask = (1u << ix.dim_bits) - 1u;
   if (type == NABO_HIP_SEARCH_KDTREE) {
     kd_build(ix, d_cloud, stride, offsets ? *offsets : std::vector<int64_t>{0, n}, s);
-    ix.device_bytes = int64_t(sizeof(uint2)) * ix.node_count +
+    ix.device_bytes = int64_t(sizeof(uint2)) * (ix.node_count + 1) +
+                      int64_t(sizeof(uint4)) * ix.node_count +
                       int64_t(sizeof(float)) * ix.pts_stride * ix.n;
   } else {
     bf_prepare(ix, d_cloud, stride, s);
@@ -249,6 +250,7 @@ KernelTimer::~KernelTimer() {
 
 Index::~Index() {
   if (d_nodes) (void)hipFree(d_nodes);
+  if (d_links) (void)hipFree(d_links);
   if (d_pts) (void)hipFree(d_pts);
   if (d_cloud) (void)hipFree(d_cloud);
   if (d_cloud_lp) (void)hipFree(d_cloud_lp);

@@ -76,7 +76,10 @@ struct Index {
   uint32_t dim_mask = 0;
   int depth = 0;
   int64_t node_count = 0;
-  uint2* d_nodes = nullptr;   // {dim | child << dim_bits, cut bits or bucket start}
+  uint2* d_nodes = nullptr;   // {dim | child << dim_bits, cut bits or bucket start},
+                              // node_count entries + one padding leaf
+  uint4* d_links = nullptr;   // per node: the right child's node word (internal
+                              // nodes) and {cut dim, cut bits} of the parent's split
   float* d_pts = nullptr;     // leaf-ordered points, pts_stride floats each,
                               // slot `dim` holds the point index bits
   int pts_stride = 4;

@@ -466,6 +466,39 @@ __global__ void k_pack_points(int64_t n, const int32_t* __restrict__ perm,
   for (int d = dim + 1; d < pts_stride; ++d) dst[d] = 0.f;
 }
 
+// Search-side links, by pre-order node p: links[p].xy is the node word of
+// p's right child (the left child is node p + 1), so one round trip reads a
+// node and both its children; links[c].zw is {cut dim, cut bits} of the
+// split of c's parent, so a far child is entered without reading its parent
+// again. nodes[count] is a padding leaf, so nodes[p + 1] exists for every p.
+__global__ void k_links_self(int64_t count, const uint2* __restrict__ nodes, uint32_t dim_bits,
+                             uint32_t dim_mask, int dim, uint4* __restrict__ links,
+                             uint2* __restrict__ pad) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= count) return;
+  const uint2 nd = nodes[p];
+  uint4 l = make_uint4(0u, 0u, 0u, 0u);
+  if ((nd.x & dim_mask) != uint32_t(dim)) {
+    const uint2 r = nodes[nd.x >> dim_bits];
+    l.x = r.x;
+    l.y = r.y;
+  }
+  links[p] = l;
+  if (p == 0) *pad = make_uint2(uint32_t(dim), 0u);
+}
+
+__global__ void k_links_parent(int64_t count, const uint2* __restrict__ nodes, uint32_t dim_bits,
+                               uint32_t dim_mask, int dim, uint4* __restrict__ links) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= count) return;
+  const uint2 nd = nodes[p];
+  const uint32_t cd = nd.x & dim_mask;
+  if (cd == uint32_t(dim)) return;
+  const uint2 split = make_uint2(cd, nd.y);
+  reinterpret_cast<uint2*>(links + p + 1)[1] = split;
+  reinterpret_cast<uint2*>(links + (nd.x >> dim_bits))[1] = split;
+}
+
 __global__ void k_roots_out(int32_t clouds, const uint32_t* __restrict__ root_base,
                             int32_t* __restrict__ roots) {
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -606,8 +639,13 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
   for (int L = 0; L < levels; ++L)
     k_preorder<<<grid_for(level_size[L]), kBlock, 0, s>>>(level_base[L], level_size[L], bfs, dim,
                                                           L == 0 ? root_base.p : nullptr);
-  NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_nodes), sizeof(uint2) * count));
+  NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_nodes), sizeof(uint2) * (count + 1)));
+  NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_links), sizeof(uint4) * count));
   k_encode<<<grid_for(count), kBlock, 0, s>>>(count, bfs, ix.dim_bits, dim, ix.d_nodes);
+  k_links_self<<<grid_for(count), kBlock, 0, s>>>(count, ix.d_nodes, ix.dim_bits, ix.dim_mask,
+                                                  dim, ix.d_links, ix.d_nodes + count);
+  k_links_parent<<<grid_for(count), kBlock, 0, s>>>(count, ix.d_nodes, ix.dim_bits, ix.dim_mask,
+                                                    dim, ix.d_links);
   k_pack_points<<<grid_for(n), kBlock, 0, s>>>(n, perm.p, cid.p, d_offsets.p, d_cloud, stride,
                                                 dim, ix.pts_stride, pts);
   if (clouds > 1) {

@@ -5,10 +5,10 @@
 //
 // One query per lane. The recursion becomes a per-lane explicit stack with
 // one 8-byte entry per split ancestor of the current node:
-//   pending  {parent node, rd of its far child}: pushed when the lane
-//            descends into the near child; popped after the near subtree,
-//            when the reference tests `rd <= maxRadius2 && rd*maxError2 <
-//            head` (:415-417 / :433-435) against the k-best as it is then;
+//   pending  {far child, its rd}: pushed when the lane descends into the
+//            near child; popped after the near subtree, when the reference
+//            tests `rd <= maxRadius2 && rd*maxError2 < head` (:415-417 /
+//            :433-435) against the k-best as it is then;
 //   restore  {cut dim | flag, previous off[cd]}: replaces the pending entry
 //            while the far subtree is visited and puts off[cd] back after
 //            it (:419-424 / :437-442).
@@ -78,6 +78,7 @@ struct Vec<0> {
 
 struct SearchParams {
   const uint2* nodes;
+  const uint4* links;
   const float* pts;
   int pts_stride;
   uint32_t dim_bits;
@@ -121,6 +122,10 @@ __device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], fl
   }
   hi[0] = moved[0];
   hd[0] = fminf(hd[0], value);
+}
+
+__device__ __forceinline__ uint2 right_word(const uint4* links, uint32_t n) {
+  return reinterpret_cast<const uint2*>(links)[2 * size_t(n)];
 }
 
 template <int DIM>
@@ -272,27 +277,43 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 4 : 1)) void k_kd_search(S
       r2 = r * r;
     }
 
-    uint32_t n = a.roots ? uint32_t(a.roots[a.qcloud[qi]]) : 0u;
     float rd = 0.f;
     uint32_t top = base_off;  // offset of the next free slot
+
+    // The split at internal node `node` (word w): pushes the far child when
+    // it passes the test and returns the near child; go_right tells which.
+    // The far child is tested again after the near subtree, against a head
+    // that can only have shrunk: a test failed now fails then.
+    auto split = [&](uint32_t node, uint2 w, bool& go_right) -> uint32_t {
+      const uint32_t cd = w.x & a.dim_mask;
+      const float new_off = q.get(cd) - __uint_as_float(w.y);
+      const float old_off = off.get(cd);
+      const float rd_far = rd + (-old_off * old_off + new_off * new_off);
+      const uint32_t right = w.x >> a.dim_bits;
+      go_right = new_off > 0;
+      if (rd_far <= r2 && rd_far * a.max_err2 < hd[K - 1]) {
+        stack_base[top] = make_uint2(go_right ? node + 1 : right, __float_as_uint(rd_far));
+        top += sstride;
+      }
+      return go_right ? right : node + 1;
+    };
+    // Node n's word and both its children's words (left child = n + 1).
+    uint32_t n = a.roots ? uint32_t(a.roots[a.qcloud[qi]]) : 0u;
+    uint2 nd = a.nodes[n], lf = a.nodes[n + 1], rt = right_word(a.links, n);
     for (;;) {
-      // Descend to a leaf. The wave leaves this loop only when every lane
-      // stands on a leaf, so the leaf scan below runs once for all lanes
-      // instead of once per node step of the slowest lane.
-      uint2 nd = a.nodes[n];
+      // Descend to a leaf, two levels per round trip to memory. The wave
+      // leaves this loop only when every lane stands on a leaf, so the leaf
+      // scan below runs once for all lanes instead of once per node step of
+      // the slowest lane.
       while ((nd.x & a.dim_mask) != leaf_tag) {
-        const uint32_t cd = nd.x & a.dim_mask;
-        const float new_off = q.get(cd) - __uint_as_float(nd.y);
-        const float old_off = off.get(cd);
-        const float rd_far = rd + (-old_off * old_off + new_off * new_off);
-        // The far child is tested again after the near subtree, against a
-        // head that can only have shrunk: a test failed now fails then.
-        if (rd_far <= r2 && rd_far * a.max_err2 < hd[K - 1]) {
-          stack_base[top] = make_uint2(n, __float_as_uint(rd_far));
-          top += sstride;
-        }
-        n = new_off > 0 ? (nd.x >> a.dim_bits) : n + 1;
+        bool go_right;
+        n = split(n, nd, go_right);
+        nd = go_right ? rt : lf;
+        if ((nd.x & a.dim_mask) == leaf_tag) break;
+        n = split(n, nd, go_right);
         nd = a.nodes[n];
+        lf = a.nodes[n + 1];
+        rt = right_word(a.links, n);
       }
       const uint32_t count = nd.x >> a.dim_bits;
       scan_leaf<DIM, K>(a, q, count, int64_t(nd.y), r2, hd, hi);
@@ -308,14 +329,18 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 4 : 1)) void k_kd_search(S
         }
         const float rdf = __uint_as_float(e.y);
         if (rdf <= r2 && rdf * a.max_err2 < hd[K - 1]) {
-          const uint2 pn = a.nodes[e.x];
-          const uint32_t pcd = pn.x & a.dim_mask;
-          const float new_off = q.get(pcd) - __uint_as_float(pn.y);
-          const uint32_t far = new_off > 0 ? e.x + 1 : (pn.x >> a.dim_bits);
+          // Enter far child e.x: its split comes from the links, with its
+          // own word and its children's words in the same round trip.
+          n = e.x;
+          const uint4 lk = a.links[n];
+          nd = a.nodes[n];
+          lf = a.nodes[n + 1];
+          rt = make_uint2(lk.x, lk.y);
+          const uint32_t pcd = lk.z;
+          const float new_off = q.get(pcd) - __uint_as_float(lk.w);
           stack_base[top] = make_uint2(kRestore | pcd, __float_as_uint(off.get(pcd)));
           top += sstride;
           off.set(pcd, new_off);
-          n = far;
           rd = rdf;
           descend = true;
           break;
@@ -490,6 +515,7 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
 
   SearchParams p{};
   p.nodes = ix.d_nodes;
+  p.links = ix.d_links;
   p.pts = ix.d_pts;
   p.pts_stride = ix.pts_stride;
   p.dim_bits = ix.dim_bits;

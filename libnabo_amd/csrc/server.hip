@@ -247,7 +247,10 @@ void nabo_hip_server::run_builder() {
     const int32_t id = build_queue.front();
     build_queue.pop_front();
     Cloud& c = *clouds[id];
-    const int64_t estimate = c.n * int64_t(sizeof(float)) * 4 + c.n * 8;
+    // Leaf-ordered points (16 B each) plus 24 B per node; a bucket of b
+    // points makes about 4n/b nodes at most in practice.
+    const int64_t bucket = std::max<int64_t>(1, int64_t(c.params.bucket_size));
+    const int64_t estimate = c.n * int64_t(sizeof(float)) * 4 + 24 * (4 * c.n / bucket + 2);
     // Trees with pending requests are not evicted: a tree built for waiting
     // requests stays until they have run, so builds cannot undo each other.
     std::vector<char> wanted(clouds.size(), 0);
