@@ -47,6 +47,7 @@ constexpr int kMaxGenericDim = 64;
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kLeafBatch = 8;  // bucket entries loaded together
 constexpr int kLdsStackSlots = 32;  // deepest tree whose stacks fit in LDS
+constexpr int kPopBatch = 4;        // stack entries read per round trip
 
 // Coordinates / offsets of one query in VGPRs, indexed by a runtime cut
 // dimension through selects (DIM > 0), or a local array (DIM == 0).
@@ -319,34 +320,48 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 4 : 1)) void k_kd_search(S
       scan_leaf<DIM, K>(a, q, count, int64_t(nd.y), r2, hd, hi);
       touched += count;
 
+      // Pop: kPopBatch entries per round trip to the stack. Restore entries
+      // put their offset back, pending entries that fail the test are
+      // dropped, and the first one that passes is entered.
       bool descend = false;
-      while (top != base_off) {
-        top -= sstride;
-        const uint2 e = stack_base[top];
-        if (e.x & kRestore) {
-          off.set(e.x & ~kRestore, __uint_as_float(e.y));
-          continue;
+      while (!descend && top != base_off) {
+        uint2 e[kPopBatch];
+#pragma unroll
+        for (int j = 0; j < kPopBatch; ++j) {
+          const bool has = top - base_off >= uint32_t(j + 1) * sstride;
+          e[j] = stack_base[has ? top - uint32_t(j + 1) * sstride : top - sstride];
         }
-        const float rdf = __uint_as_float(e.y);
-        if (rdf <= r2 && rdf * a.max_err2 < hd[K - 1]) {
-          // Enter far child e.x: its split comes from the links, with its
-          // own word and its children's words in the same round trip.
-          n = e.x;
-          const uint4 lk = a.links[n];
-          nd = a.nodes[n];
-          lf = a.nodes[n + 1];
-          rt = make_uint2(lk.x, lk.y);
-          const uint32_t pcd = lk.z;
-          const float new_off = q.get(pcd) - __uint_as_float(lk.w);
-          stack_base[top] = make_uint2(kRestore | pcd, __float_as_uint(off.get(pcd)));
-          top += sstride;
-          off.set(pcd, new_off);
-          rd = rdf;
-          descend = true;
-          break;
+#pragma unroll
+        for (int j = 0; j < kPopBatch; ++j) {
+          if (!descend && top != base_off) {
+            top -= sstride;
+            if (e[j].x & kRestore) {
+              off.set(e[j].x & ~kRestore, __uint_as_float(e[j].y));
+            } else {
+              const float rdf = __uint_as_float(e[j].y);
+              if (rdf <= r2 && rdf * a.max_err2 < hd[K - 1]) {
+                descend = true;
+                n = e[j].x;
+                rd = rdf;
+              }
+            }
+          }
         }
       }
       if (!descend) break;
+      {
+        // Enter far child n: its split comes from the links, with its own
+        // word and its children's words in the same round trip.
+        const uint4 lk = a.links[n];
+        nd = a.nodes[n];
+        lf = a.nodes[n + 1];
+        rt = make_uint2(lk.x, lk.y);
+        const uint32_t pcd = lk.z;
+        const float new_off = q.get(pcd) - __uint_as_float(lk.w);
+        stack_base[top] = make_uint2(kRestore | pcd, __float_as_uint(off.get(pcd)));
+        top += sstride;
+        off.set(pcd, new_off);
+      }
     }
 
     if (lane == 0) claimed = atomicAdd(&a.work[owner], 1u);
