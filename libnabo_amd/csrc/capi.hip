@@ -143,7 +143,7 @@ void build_index(nabo_hip_index* h, const float* d_cloud, int64_t n, int32_t row
     kd_build(ix, d_cloud, stride, offsets ? *offsets : std::vector<int64_t>{0, n}, s);
     ix.device_bytes = int64_t(sizeof(uint2)) * (ix.node_count + 1) +
                       int64_t(sizeof(uint4)) * ix.node_count +
-                      int64_t(sizeof(float)) * ix.pts_stride * ix.n;
+                      int64_t(sizeof(float)) * ix.pts_stride * (ix.n + kPtsPad);
   } else {
     bf_prepare(ix, d_cloud, stride, s);
     ix.device_bytes = int64_t(sizeof(float)) * ix.n * ix.dim;

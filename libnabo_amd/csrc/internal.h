@@ -143,6 +143,9 @@ bool kernel_timing_enabled();
 
 // Largest k the register-resident k-best arrays support.
 constexpr int kMaxK = 128;
+// Zeroed entries after the leaf-ordered points, so a leaf scan can load a
+// batch of up to kPtsPad entries past its bucket's end.
+constexpr int kPtsPad = 8;
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

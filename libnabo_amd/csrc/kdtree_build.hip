@@ -457,7 +457,11 @@ __global__ void k_pack_points(int64_t n, const int32_t* __restrict__ perm,
                               const float* __restrict__ cloud, int64_t stride, int dim,
                               int pts_stride, float* __restrict__ pts) {
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  if (p >= n + kPtsPad) return;
+  if (p >= n) {
+    for (int d = 0; d < pts_stride; ++d) pts[p * pts_stride + d] = 0.f;
+    return;
+  }
   const int32_t idx = perm[p];
   float* dst = pts + p * pts_stride;
   const float* src = cloud + int64_t(idx) * stride;
@@ -519,7 +523,8 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
   ix.cloud_offsets = offsets;
   ix.pts_stride = int((dim + 1 + 3) / 4 * 4);
 
-  NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_pts), sizeof(float) * size_t(n) * ix.pts_stride));
+  NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_pts),
+                     sizeof(float) * size_t(n + kPtsPad) * ix.pts_stride));
   float* pts = ix.d_pts;
 
   const int64_t max_nodes = 2 * n;
@@ -646,7 +651,7 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
                                                   dim, ix.d_links, ix.d_nodes + count);
   k_links_parent<<<grid_for(count), kBlock, 0, s>>>(count, ix.d_nodes, ix.dim_bits, ix.dim_mask,
                                                     dim, ix.d_links);
-  k_pack_points<<<grid_for(n), kBlock, 0, s>>>(n, perm.p, cid.p, d_offsets.p, d_cloud, stride,
+  k_pack_points<<<grid_for(n + kPtsPad), kBlock, 0, s>>>(n, perm.p, cid.p, d_offsets.p, d_cloud, stride,
                                                 dim, ix.pts_stride, pts);
   if (clouds > 1) {
     NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_roots), sizeof(int32_t) * clouds));

@@ -46,21 +46,31 @@ constexpr uint32_t kRestore = 0x80000000u;
 constexpr int kMaxGenericDim = 64;
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kLeafBatch = 8;  // bucket entries loaded together
+static_assert(kLeafBatch <= kPtsPad, "a leaf batch may read kLeafBatch - 1 entries past a bucket");
 constexpr int kLdsStackSlots = 32;  // deepest tree whose stacks fit in LDS
 constexpr int kPopBatch = 4;        // stack entries read per round trip
 
 // Coordinates / offsets of one query in VGPRs, indexed by a runtime cut
 // dimension through selects (DIM > 0), or a local array (DIM == 0).
-// The selects are written as bit blends: a select chain over an array is
-// turned back into an indexed private array (and then into LDS) otherwise.
 template <int DIM>
 struct Vec {
   float v[DIM];
   __device__ __forceinline__ float get(uint32_t i) const {
-    uint32_t r = 0;
+    if constexpr (DIM == 3) {
+      // Selects of values, not of elements: a select of two elements is a
+      // select of their addresses, which puts the array in scratch memory.
+      const float v0 = v[0], v1 = v[1], v2 = v[2];
+      const float lo = i == 1u ? v1 : v0;
+      return i == 2u ? v2 : lo;
+    } else if constexpr (DIM == 2) {
+      const float v0 = v[0], v1 = v[1];
+      return i == 1u ? v1 : v0;
+    } else {
+      uint32_t r = 0;
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) r |= __float_as_uint(v[d]) & (0u - uint32_t(i == uint32_t(d)));
-    return __uint_as_float(r);
+      for (int d = 0; d < DIM; ++d) r |= __float_as_uint(v[d]) & (0u - uint32_t(i == uint32_t(d)));
+      return __uint_as_float(r);
+    }
   }
   __device__ __forceinline__ void set(uint32_t i, float x) {
 #pragma unroll
@@ -165,10 +175,11 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
   if constexpr (DIM > 0 && DIM <= 3) {
     const float4* pts4 = reinterpret_cast<const float4*>(a.pts) + start;
     for (uint32_t j0 = 0; j0 < count; j0 += kLeafBatch) {
+      // Unconditional loads: entries past the bucket belong to the next
+      // buckets or to the zeroed padding after the last one.
       float4 pv[kLeafBatch];
 #pragma unroll
-      for (int j = 0; j < kLeafBatch; ++j)
-        if (j0 + j < count) pv[j] = pts4[j0 + j];
+      for (int j = 0; j < kLeafBatch; ++j) pv[j] = pts4[j0 + j];
 #pragma unroll
       for (int j = 0; j < kLeafBatch; ++j) {
         if (j0 + j < count) {
@@ -201,14 +212,17 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
 // take chunks from it through an atomic counter; a wave whose XCD's eighth
 // is exhausted takes chunks from the other eighths, so no XCD idles while
 // another still has work.
-// Up to K = 16 the k-best fits in 128 VGPRs without spills: ask for four
-// waves per SIMD.
+// Up to K = 16 three waves per SIMD fit without spills (at most 168 VGPRs),
+// one more than the LDS stack of a 28-level tree lets a CU hold.
 template <int DIM, int K, bool LDS_STACK>
-__global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 4 : 1)) void k_kd_search(SearchParams a) {
+__global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(SearchParams a) {
   // Per-lane stack: in LDS when the tree is shallow enough ([slot][lane] of
   // this workgroup), else in HBM ([slot][lane] over the grid).
   extern __shared__ uint2 lds_stack[];
-  uint2* const stack_base = LDS_STACK ? lds_stack : a.stack;
+  char* const stack_base = reinterpret_cast<char*>(LDS_STACK ? lds_stack : a.stack);
+  auto slot = [stack_base](uint32_t off) -> uint2& {
+    return *reinterpret_cast<uint2*>(stack_base + off);
+  };
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t lane_id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int xcd = blockIdx.x % kXcds;
@@ -216,11 +230,13 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 4 : 1)) void k_kd_search(S
 
   const int dim = DIM > 0 ? DIM : a.dim;
   const uint32_t leaf_tag = uint32_t(dim);
-  // Slot s of this lane's stack is a.stack[s * lanes + lane_id]; the
-  // running offset stays in 32 bits (lanes * depth < 2^32 is checked on the
-  // host).
-  const uint32_t base_off = LDS_STACK ? uint32_t(threadIdx.x) : uint32_t(lane_id);
-  const uint32_t sstride = LDS_STACK ? uint32_t(kSearchBlock) : uint32_t(a.stack_lanes);
+  // Slot s of this lane's stack is entry s * lanes + lane of the stack; the
+  // running offset is in bytes and stays in 32 bits (8 * lanes * depth <
+  // 2^32 is checked on the host).
+  const uint32_t base_off =
+      uint32_t(sizeof(uint2)) * (LDS_STACK ? uint32_t(threadIdx.x) : uint32_t(lane_id));
+  const uint32_t sstride =
+      uint32_t(sizeof(uint2)) * (LDS_STACK ? uint32_t(kSearchBlock) : uint32_t(a.stack_lanes));
   uint64_t touched = 0;
 
   // Output rows are written as 16-byte vectors when k = K and both output
@@ -293,14 +309,14 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 4 : 1)) void k_kd_search(S
       const uint32_t right = w.x >> a.dim_bits;
       go_right = new_off > 0;
       if (rd_far <= r2 && rd_far * a.max_err2 < hd[K - 1]) {
-        stack_base[top] = make_uint2(go_right ? node + 1 : right, __float_as_uint(rd_far));
+        slot(top) = make_uint2(go_right ? node + 1 : right, __float_as_uint(rd_far));
         top += sstride;
       }
       return go_right ? right : node + 1;
     };
     // Node n's word and both its children's words (left child = n + 1).
     uint32_t n = a.roots ? uint32_t(a.roots[a.qcloud[qi]]) : 0u;
-    uint2 nd = a.nodes[n], lf = a.nodes[n + 1], rt = right_word(a.links, n);
+    uint2 nd = a.nodes[n], lf = (a.nodes + n)[1], rt = right_word(a.links, n);
     for (;;) {
       // Descend to a leaf, two levels per round trip to memory. The wave
       // leaves this loop only when every lane stands on a leaf, so the leaf
@@ -313,7 +329,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 4 : 1)) void k_kd_search(S
         if ((nd.x & a.dim_mask) == leaf_tag) break;
         n = split(n, nd, go_right);
         nd = a.nodes[n];
-        lf = a.nodes[n + 1];
+        lf = (a.nodes + n)[1];
         rt = right_word(a.links, n);
       }
       const uint32_t count = nd.x >> a.dim_bits;
@@ -329,7 +345,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 4 : 1)) void k_kd_search(S
 #pragma unroll
         for (int j = 0; j < kPopBatch; ++j) {
           const bool has = top - base_off >= uint32_t(j + 1) * sstride;
-          e[j] = stack_base[has ? top - uint32_t(j + 1) * sstride : top - sstride];
+          e[j] = slot(has ? top - uint32_t(j + 1) * sstride : top - sstride);
         }
 #pragma unroll
         for (int j = 0; j < kPopBatch; ++j) {
@@ -354,11 +370,11 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 4 : 1)) void k_kd_search(S
         // word and its children's words in the same round trip.
         const uint4 lk = a.links[n];
         nd = a.nodes[n];
-        lf = a.nodes[n + 1];
+        lf = (a.nodes + n)[1];
         rt = make_uint2(lk.x, lk.y);
         const uint32_t pcd = lk.z;
         const float new_off = q.get(pcd) - __uint_as_float(lk.w);
-        stack_base[top] = make_uint2(kRestore | pcd, __float_as_uint(off.get(pcd)));
+        slot(top) = make_uint2(kRestore | pcd, __float_as_uint(off.get(pcd)));
         top += sstride;
         off.set(pcd, new_off);
       }
@@ -475,7 +491,7 @@ void launch_search(SearchParams p, int cus, int depth, DevBuf<uint2>& stack, hip
   } else {
     const unsigned grid = resident_grid<DIM, K, false>(0, cus);
     const int64_t lanes = int64_t(grid) * kSearchBlock;
-    if (uint64_t(lanes) * uint64_t(depth + 1) >= (uint64_t(1) << 32))
+    if (uint64_t(lanes) * uint64_t(depth + 1) * sizeof(uint2) >= (uint64_t(1) << 32))
       throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
     stack.alloc(size_t(lanes) * size_t(depth), s);
     p.stack = stack.p;
