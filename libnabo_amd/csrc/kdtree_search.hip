@@ -22,10 +22,12 @@
 // first K - k entries hold -inf and never move, so the head (largest kept
 // distance) is always entry K - 1 and no register is indexed dynamically.
 //
-// Queries are first ordered by the leaf their greedy descent reaches, so the
-// lanes of a wave walk neighbouring paths (a counting sort on the leaf's
-// bucket start); results are written to each query's own column.
+// Queries are first put in a spatial order (Morton order in the cloud's box
+// for one 2-D / 3-D tree, else the order of the leaf their greedy descent
+// reaches), so the lanes of a wave walk neighbouring paths; results are
+// written to each query's own row.
 
+#include <atomic>
 #include <cstring>
 
 #include <rocprim/device/device_scan.hpp>
@@ -48,7 +50,8 @@ constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-ro
 constexpr int kLeafBatch = 8;  // bucket entries loaded together
 static_assert(kLeafBatch <= kPtsPad, "a leaf batch may read kLeafBatch - 1 entries past a bucket");
 constexpr int kLdsStackSlots = 32;  // deepest tree whose stacks fit in LDS
-constexpr int kPopBatch = 4;        // stack entries read per round trip
+constexpr int kPopBatch = 4;
+constexpr int kMaxCachedDevices = 64;        // stack entries read per round trip
 
 // Coordinates / offsets of one query in VGPRs, indexed by a runtime cut
 // dimension through selects (DIM > 0), or a local array (DIM == 0).
@@ -418,9 +421,11 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
 }
 
 // Greedy descent to the leaf a query falls into; its bucket start orders the
-// queries.
+// queries. For 2-D and 3-D the query sits in registers and each round trip
+// reads a node with both children (two levels per round trip).
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nodes,
+                                                   const uint4* __restrict__ links,
                                                    uint32_t dim_bits, uint32_t dim_mask, int dim_rt,
                                                    const float* __restrict__ query,
                                                    int64_t qstride, int64_t m, uint32_t shift,
@@ -432,29 +437,77 @@ __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nod
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const int dim = DIM > 0 ? DIM : dim_rt;
+  const uint32_t leaf_tag = uint32_t(dim);
   const float* qp = query + i * qstride;
   uint32_t n = roots ? uint32_t(roots[qcloud[i]]) : 0u;
-  for (;;) {
-    const uint2 nd = nodes[n];
-    const uint32_t cd = nd.x & dim_mask;
-    if (cd == uint32_t(dim)) {
-      const uint32_t key = nd.y >> shift;
-      keys[i] = key;
-      vals[i] = int32_t(atomicAdd(&hist[key], 1u));  // rank within its bucket
-      break;
+  uint2 nd;
+  if constexpr (DIM == 2 || DIM == 3) {
+    Vec<DIM> q;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) q.v[d] = qp[d];
+    nd = nodes[n];
+    uint2 lf = (nodes + n)[1], rt = right_word(links, n);
+    while ((nd.x & dim_mask) != leaf_tag) {
+      bool go_right = q.get(nd.x & dim_mask) - __uint_as_float(nd.y) > 0;
+      n = go_right ? (nd.x >> dim_bits) : n + 1;
+      nd = go_right ? rt : lf;
+      if ((nd.x & dim_mask) == leaf_tag) break;
+      go_right = q.get(nd.x & dim_mask) - __uint_as_float(nd.y) > 0;
+      n = go_right ? (nd.x >> dim_bits) : n + 1;
+      nd = nodes[n];
+      lf = (nodes + n)[1];
+      rt = right_word(links, n);
     }
-    const float new_off = qp[cd] - __uint_as_float(nd.y);
-    n = new_off > 0 ? (nd.x >> dim_bits) : n + 1;
+  } else {
+    for (;;) {
+      nd = nodes[n];
+      const uint32_t cd = nd.x & dim_mask;
+      if (cd == leaf_tag) break;
+      const float new_off = qp[cd] - __uint_as_float(nd.y);
+      n = new_off > 0 ? (nd.x >> dim_bits) : n + 1;
+    }
   }
+  const uint32_t key = nd.y >> shift;
+  keys[i] = key;
+  vals[i] = int32_t(atomicAdd(&hist[key], 1u));  // rank within its bucket
+}
+
+// Morton code of a 2-D / 3-D query in the cloud's bounding box with
+// axis_bits bits per axis (coordinates outside the box clamp to it, NaN to
+// its low corner), and the query's rank among the queries of its cell.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ query,
+                                                   int64_t qstride, int64_t m, int axis_bits,
+                                                   float lo0, float lo1, float lo2, float sc0,
+                                                   float sc1, float sc2,
+                                                   uint32_t* __restrict__ keys,
+                                                   int32_t* __restrict__ ranks,
+                                                   uint32_t* __restrict__ hist) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const float top = float((1u << axis_bits) - 1u);
+  const float* qp = query + i * qstride;
+  const float lo[3] = {lo0, lo1, lo2}, sc[3] = {sc0, sc1, sc2};
+  uint32_t code = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const uint32_t g = uint32_t(fminf(fmaxf((qp[d] - lo[d]) * sc[d], 0.f), top));
+    for (int b = 0; b < axis_bits; ++b) code |= ((g >> b) & 1u) << (b * DIM + d);
+  }
+  keys[i] = code;
+  ranks[i] = int32_t(atomicAdd(&hist[code], 1u));
+}
+
+// Zeroes the search's chunk counters and the sort's histogram (kernels
+// rather than memsets, so the launch sequence can be captured in a graph).
+__global__ void k_zero_counters(uint32_t* counters, int64_t n) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < n) counters[e] = 0;
 }
 
 // Counting sort of the queries by key: position = start of the key's bucket
 // + rank of the query in it (the order inside a bucket is arbitrary; results
 // do not depend on the order in which queries are searched).
-__global__ void k_zero_hist(uint32_t* hist, int64_t n) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e < n) hist[e] = 0;
-}
 
 __global__ void k_place(int64_t m, const uint32_t* __restrict__ keys,
                         const int32_t* __restrict__ ranks, const uint32_t* __restrict__ starts,
@@ -464,32 +517,34 @@ __global__ void k_place(int64_t m, const uint32_t* __restrict__ keys,
   order[starts[keys[i]] + uint32_t(ranks[i])] = int32_t(i);
 }
 
-__global__ void k_zero_u32(uint32_t* p, int n) {
-  if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
-}
-
 // Persistent grid: as many workgroups as the device holds at once (from
 // the occupancy of this instantiation), a multiple of the XCD count.
+// The occupancy is cached per device and LDS size (tree depth).
 template <int DIM, int K, bool LDS>
-unsigned resident_grid(size_t lds, int cus) {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kd_search<DIM, K, LDS>, kSearchBlock,
-                                                   lds) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
+unsigned resident_grid(size_t lds, int cus, int dev) {
+  static std::atomic<int> cache[kMaxCachedDevices][kLdsStackSlots + 2];
+  const size_t slot = std::min<size_t>(lds / (kSearchBlock * sizeof(uint2)), kLdsStackSlots + 1);
+  int per_cu = dev >= 0 && dev < kMaxCachedDevices ? cache[dev][slot].load() : 0;
+  if (per_cu < 1) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kd_search<DIM, K, LDS>,
+                                                     kSearchBlock, lds) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    if (dev >= 0 && dev < kMaxCachedDevices) cache[dev][slot].store(per_cu);
+  }
   return unsigned(std::max(kXcds, (cus * per_cu) / kXcds * kXcds));
 }
 
 template <int DIM, int K>
-void launch_search(SearchParams p, int cus, int depth, DevBuf<uint2>& stack, hipStream_t s) {
-  k_zero_u32<<<1, 64, 0, s>>>(p.work, kXcds);
+void launch_search(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& stack,
+                   hipStream_t s) {
   if (depth <= kLdsStackSlots) {
     const size_t lds = size_t(depth) * kSearchBlock * sizeof(uint2);
-    const unsigned grid = resident_grid<DIM, K, true>(lds, cus);
+    const unsigned grid = resident_grid<DIM, K, true>(lds, cus, dev);
     KernelTimer timer(s);
     k_kd_search<DIM, K, true><<<grid, kSearchBlock, lds, s>>>(p);
   } else {
-    const unsigned grid = resident_grid<DIM, K, false>(0, cus);
+    const unsigned grid = resident_grid<DIM, K, false>(0, cus, dev);
     const int64_t lanes = int64_t(grid) * kSearchBlock;
     if (uint64_t(lanes) * uint64_t(depth + 1) * sizeof(uint2) >= (uint64_t(1) << 32))
       throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
@@ -502,24 +557,25 @@ void launch_search(SearchParams p, int cus, int depth, DevBuf<uint2>& stack, hip
 }
 
 template <int DIM>
-void dispatch_k(const SearchParams& p, int cus, int depth, DevBuf<uint2>& stack, hipStream_t s) {
+void dispatch_k(const SearchParams& p, int cus, int dev, int depth, DevBuf<uint2>& stack,
+                hipStream_t s) {
   const int k = p.k;
   if (k <= 1)
-    launch_search<DIM, 1>(p, cus, depth, stack, s);
+    launch_search<DIM, 1>(p, cus, dev, depth, stack, s);
   else if (k <= 2)
-    launch_search<DIM, 2>(p, cus, depth, stack, s);
+    launch_search<DIM, 2>(p, cus, dev, depth, stack, s);
   else if (k <= 4)
-    launch_search<DIM, 4>(p, cus, depth, stack, s);
+    launch_search<DIM, 4>(p, cus, dev, depth, stack, s);
   else if (k <= 8)
-    launch_search<DIM, 8>(p, cus, depth, stack, s);
+    launch_search<DIM, 8>(p, cus, dev, depth, stack, s);
   else if (k <= 16)
-    launch_search<DIM, 16>(p, cus, depth, stack, s);
+    launch_search<DIM, 16>(p, cus, dev, depth, stack, s);
   else if (k <= 32)
-    launch_search<DIM, 32>(p, cus, depth, stack, s);
+    launch_search<DIM, 32>(p, cus, dev, depth, stack, s);
   else if (k <= 64)
-    launch_search<DIM, 64>(p, cus, depth, stack, s);
+    launch_search<DIM, 64>(p, cus, dev, depth, stack, s);
   else
-    launch_search<DIM, 128>(p, cus, depth, stack, s);
+    launch_search<DIM, 128>(p, cus, dev, depth, stack, s);
 }
 
 uint32_t bits_needed(uint64_t v) {
@@ -569,41 +625,76 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     p.qcloud = a.qcloud;
   }
 
-  // Query order by home leaf (bucket starts are global, so this also groups
-  // the queries of batched trees by cloud): key = the leaf's bucket start cut
-  // to 16 bits, counting sort.
-  DevBuf<uint32_t> keys, hist, starts;
+  // Query order, by a counting sort (rank = atomic count within the key's
+  // bucket, so the order inside a bucket is arbitrary). One 2-D / 3-D tree:
+  // key = Morton code of the query in the cloud's box, with up to a few
+  // cells per query (at most 2^21 cells); lanes of a wave then walk
+  // neighbouring paths.
+  // Batched trees and other dimensions: key = bucket start of the leaf the
+  // greedy descent reaches, cut to 16 bits (bucket starts are global, so
+  // this also groups the queries of batched trees by cloud). One zeroed
+  // buffer holds the search's chunk counters and the histogram.
+  DevBuf<uint32_t> counters, keys, starts;
   DevBuf<int32_t> ranks, order;
   DevBuf<uint8_t> scan_tmp;
-  if (ix.params.query_sort && a.m > 1 && ix.node_count > 1) {
+  const bool sort = ix.params.query_sort && a.m > 1 && ix.node_count > 1;
+  const bool morton = sort && ix.clouds == 1 && (ix.dim == 2 || ix.dim == 3);
+  int axis_bits = 0;
+  uint32_t shift = 0;
+  int64_t buckets = 0;
+  if (morton) {
+    const int max_bits = ix.dim == 3 ? 7 : 10;
+    axis_bits = std::max(1, std::min(max_bits, (int(bits_needed(uint64_t(a.m))) + 2) / ix.dim));
+    buckets = int64_t(1) << (axis_bits * ix.dim);
+  } else if (sort) {
     const uint32_t full_bits = std::max(1u, bits_needed(uint64_t(ix.n)));
-    const uint32_t shift = full_bits > 16 ? full_bits - 16 : 0;
-    const int64_t buckets = int64_t(1) << (full_bits - shift);
+    shift = full_bits > 16 ? full_bits - 16 : 0;
+    buckets = int64_t(1) << (full_bits - shift);
+  }
+  counters.alloc(size_t(kXcds + buckets), s);
+  k_zero_counters<<<unsigned(ceil_div(kXcds + buckets, kBlock)), kBlock, 0, s>>>(counters.p,
+                                                                                 kXcds + buckets);
+  p.work = counters.p;
+  if (sort) {
+    uint32_t* const hist = counters.p + kXcds;
+    const unsigned g = unsigned(ceil_div(a.m, kBlock));
     keys.alloc(size_t(a.m), s);
     ranks.alloc(size_t(a.m), s);
     order.alloc(size_t(a.m), s);
-    hist.alloc(size_t(buckets), s);
     starts.alloc(size_t(buckets), s);
-    k_zero_hist<<<unsigned(ceil_div(buckets, kBlock)), kBlock, 0, s>>>(hist.p, buckets);
-    const unsigned g = unsigned(ceil_div(a.m, kBlock));
-    if (ix.dim == 3)
-      k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, ranks.p,
-                                       hist.p);
-    else if (ix.dim == 2)
-      k_locate<2><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, ranks.p,
-                                       hist.p);
-    else
-      k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.dim_bits, ix.dim_mask, ix.dim, a.query,
-                                       a.qstride, a.m, shift, p.roots, p.qcloud, keys.p, ranks.p,
-                                       hist.p);
+    if (morton) {
+      float lo[3] = {0.f, 0.f, 0.f}, sc[3] = {0.f, 0.f, 0.f};
+      const float top = float((1u << axis_bits) - 1u);
+      for (int d = 0; d < ix.dim; ++d) {
+        lo[d] = ix.min_bound[size_t(d)];
+        const float ext = ix.max_bound[size_t(d)] - lo[d];
+        sc[d] = ext > 0.f ? top / ext : 0.f;
+      }
+      if (ix.dim == 3)
+        k_morton<3><<<g, kBlock, 0, s>>>(a.query, a.qstride, a.m, axis_bits, lo[0], lo[1], lo[2],
+                                         sc[0], sc[1], sc[2], keys.p, ranks.p, hist);
+      else
+        k_morton<2><<<g, kBlock, 0, s>>>(a.query, a.qstride, a.m, axis_bits, lo[0], lo[1], lo[2],
+                                         sc[0], sc[1], sc[2], keys.p, ranks.p, hist);
+    } else if (ix.dim == 3) {
+      k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
+                                       a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p,
+                                       ranks.p, hist);
+    } else if (ix.dim == 2) {
+      k_locate<2><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
+                                       a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p,
+                                       ranks.p, hist);
+    } else {
+      k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
+                                       a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p,
+                                       ranks.p, hist);
+    }
     NH_LAUNCH_CHECK();
     size_t bytes = 0;
-    NH_CHECK(rocprim::exclusive_scan(nullptr, bytes, hist.p, starts.p, uint32_t(0),
+    NH_CHECK(rocprim::exclusive_scan(nullptr, bytes, hist, starts.p, uint32_t(0),
                                      size_t(buckets), rocprim::plus<uint32_t>(), s));
     scan_tmp.alloc(bytes + 16, s);
-    NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, hist.p, starts.p, uint32_t(0),
+    NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, hist, starts.p, uint32_t(0),
                                      size_t(buckets), rocprim::plus<uint32_t>(), s));
     k_place<<<g, kBlock, 0, s>>>(a.m, keys.p, ranks.p, starts.p, order.p);
     NH_LAUNCH_CHECK();
@@ -614,15 +705,13 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   NH_CHECK(hipGetDevice(&dev));
   NH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int depth = std::max(1, ix.depth);
-  DevBuf<uint32_t> work(kXcds, s);
-  p.work = work.p;
   DevBuf<uint2> stack;
   if (ix.dim == 3)
-    dispatch_k<3>(p, cus, depth, stack, s);
+    dispatch_k<3>(p, cus, dev, depth, stack, s);
   else if (ix.dim == 2)
-    dispatch_k<2>(p, cus, depth, stack, s);
+    dispatch_k<2>(p, cus, dev, depth, stack, s);
   else
-    dispatch_k<0>(p, cus, depth, stack, s);
+    dispatch_k<0>(p, cus, dev, depth, stack, s);
   NH_LAUNCH_CHECK();
 }
 
