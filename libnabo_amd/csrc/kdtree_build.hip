@@ -65,59 +65,178 @@ struct Bfs {  // all nodes in BFS order
   uint32_t* pre;    // pre-order id
 };
 
-// Tiled segmented min/max/add: each thread folds the points of one segment
-// at a time into a register and emits an atomic when its segment changes;
-// the last partial results are combined per wave. Segments are contiguous
-// ranges of positions, so a thread rarely changes segment inside a tile.
+constexpr int kTile = 2048;              // points per workgroup in the tiled passes
+constexpr int kItems = kTile / kBlock;   // consecutive points per thread
+
 template <int OP>
-struct SegAcc {
-  int seg = -1;
-  uint32_t v = 0;
-  __device__ __forceinline__ static uint32_t op(uint32_t a, uint32_t b) {
-    return OP == 0 ? min(a, b) : (OP == 1 ? max(a, b) : a + b);
-  }
-  __device__ __forceinline__ void add(int s, uint32_t x, uint32_t* target) {
-    if (s != seg) {
-      if (seg >= 0) atomic(target);
-      seg = s;
-      v = x;
-    } else {
-      v = op(v, x);
-    }
-  }
-  __device__ __forceinline__ void atomic(uint32_t* target) {
-    if (OP == 0)
-      atomicMin(&target[seg], v);
-    else if (OP == 1)
-      atomicMax(&target[seg], v);
-    else
-      atomicAdd(&target[seg], v);
-  }
-  // All lanes of the wave call this once, at the end.
-  __device__ __forceinline__ void finish(uint32_t* target) {
-    wave_segmented_atomic<OP>(seg >= 0, seg, v, target);
-  }
+__device__ __forceinline__ uint32_t red(uint32_t a, uint32_t b) {
+  return OP == 0 ? min(a, b) : (OP == 1 ? max(a, b) : a + b);
+}
+template <int OP>
+__device__ __forceinline__ uint32_t red_identity() {
+  return OP == 0 ? 0xffffffffu : 0u;
+}
+template <int OP>
+__device__ __forceinline__ void red_atomic(uint32_t* p, uint32_t v) {
+  if (OP == 0)
+    atomicMin(p, v);
+  else if (OP == 1)
+    atomicMax(p, v);
+  else
+    atomicAdd(p, v);
+}
+
+// Per-tile partial results of the two segments a tile may share with its
+// neighbours (those of its first and last items): slot 2t and 2t + 1 of
+// tile t. A tile made of one segment writes it to both slots (the second
+// with identity values), so the partials of a segment stay contiguous.
+struct TileParts {
+  int32_t* seg;
+  uint32_t* a;
+  uint32_t* b;
 };
 
-constexpr int kTile = 2048;  // points per workgroup in the tiled passes
+// Reduction by segment of two quantities over one tile, without contention:
+// thread t holds items [kItems t, kItems (t + 1)) of the tile in position
+// order, seg < 0 meaning no item; equal segments are contiguous. A segment
+// that lies inside the tile gets one atomic per quantity (no other tile
+// touches it); the tile's first and last segments go to its partials, or to
+// atomics as well when parts is null. Targets: ta[seg * stride],
+// tb[seg * stride]. Same-address atomics from many workgroups serialize at
+// the memory side, which made the top levels of the build slow.
+template <int OPA, int OPB>
+__device__ void tile_reduce(const int32_t (&seg)[kItems], const uint32_t (&va)[kItems],
+                            const uint32_t (&vb)[kItems], uint32_t* ta, uint32_t* tb,
+                            int64_t stride, const TileParts* parts, int64_t tile) {
+  __shared__ int32_t s_first[kBlock], s_last[kBlock], s_flag[kBlock];
+  __shared__ uint32_t s_a[kBlock], s_b[kBlock];
+  const int t = threadIdx.x;
+  // Runs inside the thread: the first one (head), interior ones (complete,
+  // emitted here) and the last one (tail).
+  int32_t cs = seg[0];
+  uint32_t ca = va[0], cb = vb[0];
+  int32_t hs = seg[0];
+  uint32_t ha = 0, hb = 0;
+  bool single = true;
+  for (int j = 1; j < kItems; ++j) {
+    if (seg[j] == cs) {
+      ca = red<OPA>(ca, va[j]);
+      cb = red<OPB>(cb, vb[j]);
+      continue;
+    }
+    if (single) {
+      ha = ca;
+      hb = cb;
+      single = false;
+    } else if (cs >= 0) {
+      red_atomic<OPA>(ta + cs * stride, ca);
+      red_atomic<OPB>(tb + cs * stride, cb);
+    }
+    cs = seg[j];
+    ca = va[j];
+    cb = vb[j];
+  }
+  s_first[t] = seg[0];
+  s_last[t] = cs;
+  __syncthreads();
+  const int32_t prev_last = t > 0 ? s_last[t - 1] : -1;
+  const int32_t next_first = t + 1 < kBlock ? s_first[t + 1] : -1;
+  const int32_t tile_first = s_first[0], tile_last = s_last[kBlock - 1];
+  // Inclusive segmented scan of the tails over the threads: a tail
+  // continues the previous thread's tail when the thread is one run of
+  // that segment.
+  int32_t flag = !(single && cs >= 0 && cs == prev_last);
+  uint32_t sa = ca, sb = cb;
+  for (int d = 1; d < kBlock; d <<= 1) {
+    s_flag[t] = flag;
+    s_a[t] = sa;
+    s_b[t] = sb;
+    __syncthreads();
+    if (t >= d && !flag) {
+      const int32_t of = s_flag[t - d];
+      sa = red<OPA>(s_a[t - d], sa);
+      sb = red<OPB>(s_b[t - d], sb);
+      flag = of;
+    }
+    __syncthreads();
+  }
+  // Scanned value of the previous thread's tail, for the head run.
+  s_a[t] = sa;
+  s_b[t] = sb;
+  __syncthreads();
+  auto emit = [&](int32_t sgm, uint32_t a, uint32_t b) {
+    if (sgm < 0) return;
+    if (parts && (sgm == tile_first || sgm == tile_last)) {
+      const int slot = sgm == tile_first ? 0 : 1;
+      parts->seg[2 * tile + slot] = sgm;
+      parts->a[2 * tile + slot] = a;
+      parts->b[2 * tile + slot] = b;
+    } else {
+      red_atomic<OPA>(ta + sgm * stride, a);
+      red_atomic<OPB>(tb + sgm * stride, b);
+    }
+  };
+  if (!single) {
+    uint32_t a = ha, b = hb;
+    if (hs >= 0 && hs == prev_last) {
+      a = red<OPA>(s_a[t - 1], a);
+      b = red<OPB>(s_b[t - 1], b);
+    }
+    emit(hs, a, b);
+  }
+  if (cs != next_first) emit(cs, sa, sb);
+  if (parts) {
+    if (t == 0 && tile_first < 0) parts->seg[2 * tile] = -1;
+    if (t == kBlock - 1 && (tile_last < 0 || tile_last == tile_first)) {
+      parts->seg[2 * tile + 1] = tile_last < 0 ? -1 : tile_last;
+      parts->a[2 * tile + 1] = red_identity<OPA>();
+      parts->b[2 * tile + 1] = red_identity<OPB>();
+    }
+  }
+}
+
+// Second stage: the partials of `groups` reductions ([group][2 * tiles]),
+// reduced the same way with atomics only (targets ta / tb + group * gstride).
+template <int OPA, int OPB>
+__global__ __launch_bounds__(kBlock) void k_tile_partials(int64_t nparts, TileParts parts,
+                                                          uint32_t* ta, uint32_t* tb,
+                                                          int64_t stride, int64_t gstride) {
+  const int64_t g = blockIdx.y;
+  const int64_t base = int64_t(blockIdx.x) * kTile + int64_t(threadIdx.x) * kItems;
+  int32_t sg[kItems];
+  uint32_t a[kItems], b[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t e = base + j;
+    const bool ok = e < nparts;
+    sg[j] = ok ? parts.seg[g * nparts + e] : -1;
+    a[j] = ok ? parts.a[g * nparts + e] : red_identity<OPA>();
+    b[j] = ok ? parts.b[g * nparts + e] : red_identity<OPB>();
+  }
+  tile_reduce<OPA, OPB>(sg, a, b, ta + g * gstride, tb + g * gstride, stride, nullptr, 0);
+}
 
 // Per-cloud bounds (kdtree_cpu.cpp:256-267): keys[c*2*dim + d] holds the min
-// key of dimension d, keys[c*2*dim + dim + d] the max key.
-__global__ void k_bounds(int64_t n, const float* __restrict__ cloud, int64_t stride, int dim,
-                         const int32_t* __restrict__ cid, uint32_t* __restrict__ keys) {
-  const int64_t t0 = int64_t(blockIdx.x) * kTile;
-  const int64_t t1 = min(n, t0 + kTile);
+// key of dimension d, keys[c*2*dim + dim + d] the max key. Partials per
+// dimension: parts [d][2 * tiles].
+__global__ __launch_bounds__(kBlock) void k_bounds(int64_t n, const float* __restrict__ cloud,
+                                                   int64_t stride, int dim,
+                                                   const int32_t* __restrict__ cid,
+                                                   uint32_t* __restrict__ keys, TileParts parts,
+                                                   int64_t nparts) {
+  const int64_t tile = blockIdx.x;
+  const int64_t base = tile * kTile + int64_t(threadIdx.x) * kItems;
+  int32_t sg[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) sg[j] = base + j < n ? cid[base + j] : -1;
   for (int d = 0; d < dim; ++d) {
-    SegAcc<0> mn;
-    SegAcc<1> mx;
-    for (int64_t p = t0 + threadIdx.x; p < t1; p += blockDim.x) {
-      const uint32_t kv = float_key(cloud[p * stride + d]);
-      const int c = cid[p];
-      mn.add(c * 2 * dim + d, kv, keys);
-      mx.add(c * 2 * dim + dim + d, kv, keys);
-    }
-    mn.finish(keys);
-    mx.finish(keys);
+    uint32_t kv[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+      kv[j] = base + j < n ? float_key(cloud[(base + j) * stride + d]) : 0u;
+    const TileParts pd{parts.seg + d * nparts, parts.a + d * nparts, parts.b + d * nparts};
+    tile_reduce<0, 1>(sg, kv, kv, keys + d, keys + dim + d, 2 * dim, &pd, tile);
+    __syncthreads();
   }
 }
 
@@ -209,28 +328,32 @@ __global__ void k_split_total(int32_t S, const uint32_t* split, const uint32_t* 
 
 // Gathers the cut coordinate of every point still inside a splitting segment
 // and reduces its min/max per segment.
-__global__ void k_pt_gather(int64_t n, int32_t* __restrict__ pseg, const int32_t* __restrict__ perm,
-                            const float* __restrict__ cloud, int64_t stride, SegStat st,
-                            float* __restrict__ xs) {
-  const int64_t t0 = int64_t(blockIdx.x) * kTile;
-  const int64_t t1 = min(n, t0 + kTile);
-  SegAcc<0> mn;
-  SegAcc<1> mx;
-  for (int64_t p = t0 + threadIdx.x; p < t1; p += blockDim.x) {
-    int s = pseg[p];
-    if (s < 0) continue;
-    if (st.split[s] == 0) {
+__global__ __launch_bounds__(kBlock) void k_pt_gather(int64_t n, int32_t* __restrict__ pseg,
+                                                      const int32_t* __restrict__ perm,
+                                                      const float* __restrict__ cloud,
+                                                      int64_t stride, SegStat st,
+                                                      float* __restrict__ xs, TileParts parts) {
+  const int64_t tile = blockIdx.x;
+  const int64_t base = tile * kTile + int64_t(threadIdx.x) * kItems;
+  int32_t sg[kItems];
+  uint32_t kv[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t p = base + j;
+    int s = p < n ? pseg[p] : -1;
+    kv[j] = 0u;
+    if (s >= 0 && st.split[s] == 0) {
       pseg[p] = -1;  // its segment became a leaf
-      continue;
+      s = -1;
     }
-    const float x = cloud[int64_t(perm[p]) * stride + st.cd[s]];
-    xs[p] = x;
-    const uint32_t key = float_key(x);
-    mn.add(s, key, st.minkey);
-    mx.add(s, key, st.maxkey);
+    if (s >= 0) {
+      const float x = cloud[int64_t(perm[p]) * stride + st.cd[s]];
+      xs[p] = x;
+      kv[j] = float_key(x);
+    }
+    sg[j] = s;
   }
-  mn.finish(st.minkey);
-  mx.finish(st.maxkey);
+  tile_reduce<0, 1>(sg, kv, kv, st.minkey, st.maxkey, 1, &parts, tile);
 }
 
 __global__ void k_seg_cut(int32_t S, SegStat st) {
@@ -249,21 +372,28 @@ __global__ void k_seg_cut(int32_t S, SegStat st) {
   st.cut[s] = cut;
 }
 
-__global__ void k_pt_count(int64_t n, const int32_t* __restrict__ pseg,
-                           const float* __restrict__ xs, SegStat st) {
-  const int64_t t0 = int64_t(blockIdx.x) * kTile;
-  const int64_t t1 = min(n, t0 + kTile);
-  SegAcc<2> lt, eq;
-  for (int64_t p = t0 + threadIdx.x; p < t1; p += blockDim.x) {
-    const int s = pseg[p];
-    if (s < 0) continue;
-    const float x = xs[p];
-    const float cut = st.cut[s];
-    lt.add(s, x < cut ? 1u : 0u, st.lt);
-    eq.add(s, x == cut ? 1u : 0u, st.eq);
+__global__ __launch_bounds__(kBlock) void k_pt_count(int64_t n, const int32_t* __restrict__ pseg,
+                                                     const float* __restrict__ xs, SegStat st,
+                                                     TileParts parts) {
+  const int64_t tile = blockIdx.x;
+  const int64_t base = tile * kTile + int64_t(threadIdx.x) * kItems;
+  int32_t sg[kItems];
+  uint32_t lt[kItems], eq[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t p = base + j;
+    const int s = p < n ? pseg[p] : -1;
+    sg[j] = s;
+    lt[j] = 0u;
+    eq[j] = 0u;
+    if (s >= 0) {
+      const float x = xs[p];
+      const float cut = st.cut[s];
+      lt[j] = x < cut ? 1u : 0u;
+      eq[j] = x == cut ? 1u : 0u;
+    }
   }
-  lt.finish(st.lt);
-  eq.finish(st.eq);
+  tile_reduce<2, 2>(sg, lt, eq, st.lt, st.eq, 1, &parts, tile);
 }
 
 // Out-of-place flags of one Hoare pass: low word = belongs right but sits
@@ -563,6 +693,12 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
   NH_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes32, st_split.p, st_childbase.p, uint32_t(0),
                                    size_t(seg_cap), rocprim::plus<uint32_t>(), s));
   DevBuf<uint8_t> scan_tmp(std::max(scan_bytes64, scan_bytes32) + 16, s);
+  // Tile partials of the reductions: 2 per tile, dim groups for the bounds.
+  const int64_t nparts = 2 * int64_t(tiles_for(n));
+  DevBuf<int32_t> part_seg(size_t(nparts) * dim, s);
+  DevBuf<uint32_t> part_a(size_t(nparts) * dim, s), part_b(size_t(nparts) * dim, s);
+  const TileParts parts{part_seg.p, part_a.p, part_b.p};
+  const unsigned gpart = unsigned(ceil_div(nparts, kTile));
 
   SegStat st{st_cd.p, st_ideal.p, st_cut.p, st_minkey.p, st_maxkey.p, st_lt.p, st_eq.p,
              st_split.p, st_childbase.p, st_left.p};
@@ -575,7 +711,9 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
   const int64_t cd_elems = int64_t(clouds) * dim;
   k_init_points<<<grid_for(n), kBlock, 0, s>>>(n, clouds, d_offsets.p, cid.p, perm.p, pseg.p);
   k_init_keys<<<grid_for(cd_elems), kBlock, 0, s>>>(clouds, dim, keys.p);
-  k_bounds<<<tiles_for(n), kBlock, 0, s>>>(n, d_cloud, stride, dim, cid.p, keys.p);
+  k_bounds<<<tiles_for(n), kBlock, 0, s>>>(n, d_cloud, stride, dim, cid.p, keys.p, parts, nparts);
+  k_tile_partials<0, 1><<<dim3(gpart, unsigned(dim)), kBlock, 0, s>>>(nparts, parts, keys.p,
+                                                                      keys.p + dim, 2 * dim, 1);
   k_roots<<<grid_for(cd_elems), kBlock, 0, s>>>(clouds, dim, d_offsets.p, keys.p, level(0),
                                                  bounds.p);
   NH_LAUNCH_CHECK();
@@ -608,9 +746,11 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
     if (T > 0) {
       const unsigned gp = grid_for(n);
       const unsigned gt = tiles_for(n);
-      k_pt_gather<<<gt, kBlock, 0, s>>>(n, pseg.p, perm.p, d_cloud, stride, st, xs.p);
+      k_pt_gather<<<gt, kBlock, 0, s>>>(n, pseg.p, perm.p, d_cloud, stride, st, xs.p, parts);
+      k_tile_partials<0, 1><<<gpart, kBlock, 0, s>>>(nparts, parts, st.minkey, st.maxkey, 1, 0);
       k_seg_cut<<<gs, kBlock, 0, s>>>(S, st);
-      k_pt_count<<<gt, kBlock, 0, s>>>(n, pseg.p, xs.p, st);
+      k_pt_count<<<gt, kBlock, 0, s>>>(n, pseg.p, xs.p, st, parts);
+      k_tile_partials<2, 2><<<gpart, kBlock, 0, s>>>(nparts, parts, st.lt, st.eq, 1, 0);
       for (int pass = 1; pass <= 2; ++pass) {
         k_pt_flags<<<gp, kBlock, 0, s>>>(n, pass, pseg.p, xs.p, sg, st, flags.p);
         bytes = scan_tmp.count;
