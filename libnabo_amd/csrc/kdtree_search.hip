@@ -431,9 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nod
                                                    int64_t qstride, int64_t m, uint32_t shift,
                                                    const int32_t* __restrict__ roots,
                                                    const int32_t* __restrict__ qcloud,
-                                                   uint32_t* __restrict__ keys,
-                                                   int32_t* __restrict__ vals,
-                                                   uint32_t* __restrict__ hist) {
+                                                   uint32_t* __restrict__ keys) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const int dim = DIM > 0 ? DIM : dim_rt;
@@ -467,22 +465,18 @@ __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nod
       n = new_off > 0 ? (nd.x >> dim_bits) : n + 1;
     }
   }
-  const uint32_t key = nd.y >> shift;
-  keys[i] = key;
-  vals[i] = int32_t(atomicAdd(&hist[key], 1u));  // rank within its bucket
+  keys[i] = nd.y >> shift;
 }
 
 // Morton code of a 2-D / 3-D query in the cloud's bounding box with
 // axis_bits bits per axis (coordinates outside the box clamp to it, NaN to
-// its low corner), and the query's rank among the queries of its cell.
+// its low corner).
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ query,
                                                    int64_t qstride, int64_t m, int axis_bits,
                                                    float lo0, float lo1, float lo2, float sc0,
                                                    float sc1, float sc2,
-                                                   uint32_t* __restrict__ keys,
-                                                   int32_t* __restrict__ ranks,
-                                                   uint32_t* __restrict__ hist) {
+                                                   uint32_t* __restrict__ keys) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const float top = float((1u << axis_bits) - 1u);
@@ -495,26 +489,110 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ que
     for (int b = 0; b < axis_bits; ++b) code |= ((g >> b) & 1u) << (b * DIM + d);
   }
   keys[i] = code;
-  ranks[i] = int32_t(atomicAdd(&hist[code], 1u));
 }
 
-// Zeroes the search's chunk counters and the sort's histogram (kernels
-// rather than memsets, so the launch sequence can be captured in a graph).
+__global__ void k_shift_keys(uint32_t* keys, int64_t m, uint32_t shift) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < m) keys[i] >>= shift;
+}
+
+// Zeroes the search's chunk counters (a kernel rather than a memset, so the
+// launch sequence can be captured in a graph).
 __global__ void k_zero_counters(uint32_t* counters, int64_t n) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e < n) counters[e] = 0;
 }
 
-// Counting sort of the queries by key: position = start of the key's bucket
-// + rank of the query in it (the order inside a bucket is arbitrary; results
-// do not depend on the order in which queries are searched).
+// Query ordering: a two-level bucket sort of the keys with workgroup-local
+// (LDS) counters only; global atomics from every query serialize at the
+// memory side. The order inside a bucket is arbitrary: results do not
+// depend on the order in which queries are searched.
+//   1. k_sort_hist: per tile of kSortTile queries, the count of each coarse
+//      bucket (the key's top coarse_bits), written bucket-major
+//      counts[bucket * tiles + tile];
+//   2. exclusive scan of counts: where each (bucket, tile) pair starts;
+//   3. k_sort_scatter: each tile places its queries in their coarse buckets;
+//   4. k_sort_fine: one workgroup per coarse bucket places its queries by
+//      the key's low fine_bits.
+constexpr int kSortTile = 4096;
+constexpr int kCoarseBits = 12;
+constexpr int kFineBitsMax = 9;
 
-__global__ void k_place(int64_t m, const uint32_t* __restrict__ keys,
-                        const int32_t* __restrict__ ranks, const uint32_t* __restrict__ starts,
-                        int32_t* __restrict__ order) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  order[starts[keys[i]] + uint32_t(ranks[i])] = int32_t(i);
+__global__ __launch_bounds__(kBlock) void k_sort_hist(const uint32_t* __restrict__ keys,
+                                                      int64_t m, int fine_bits, int coarse_bins,
+                                                      int64_t tiles, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t h[1 << kCoarseBits];
+  const int64_t tile = blockIdx.x;
+  for (int b = threadIdx.x; b < coarse_bins; b += kBlock) h[b] = 0;
+  __syncthreads();
+  const int64_t end = min(m, (tile + 1) * kSortTile);
+  for (int64_t q = tile * kSortTile + threadIdx.x; q < end; q += kBlock)
+    atomicAdd(&h[keys[q] >> fine_bits], 1u);
+  __syncthreads();
+  for (int b = threadIdx.x; b < coarse_bins; b += kBlock) counts[int64_t(b) * tiles + tile] = h[b];
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint32_t* __restrict__ keys,
+                                                         int64_t m, int fine_bits,
+                                                         int coarse_bins, int64_t tiles,
+                                                         const uint32_t* __restrict__ starts,
+                                                         uint32_t* __restrict__ fkeys,
+                                                         int32_t* __restrict__ fidx) {
+  __shared__ uint32_t cur[1 << kCoarseBits];
+  const int64_t tile = blockIdx.x;
+  for (int b = threadIdx.x; b < coarse_bins; b += kBlock) cur[b] = starts[int64_t(b) * tiles + tile];
+  __syncthreads();
+  const uint32_t fine_mask = (1u << fine_bits) - 1u;
+  const int64_t end = min(m, (tile + 1) * kSortTile);
+  for (int64_t q = tile * kSortTile + threadIdx.x; q < end; q += kBlock) {
+    const uint32_t key = keys[q];
+    const uint32_t pos = atomicAdd(&cur[key >> fine_bits], 1u);
+    fkeys[pos] = key & fine_mask;
+    fidx[pos] = int32_t(q);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_fine(const uint32_t* __restrict__ fkeys,
+                                                      const int32_t* __restrict__ fidx,
+                                                      const uint32_t* __restrict__ starts,
+                                                      int64_t tiles, int64_t m, int fine_bits,
+                                                      int coarse_bins, int32_t* __restrict__ order) {
+  __shared__ uint32_t h[1 << kFineBitsMax];
+  __shared__ uint32_t part[kBlock];
+  const int bin = blockIdx.x;
+  const uint32_t begin = starts[int64_t(bin) * tiles];
+  const uint32_t end = bin + 1 < coarse_bins ? starts[int64_t(bin + 1) * tiles] : uint32_t(m);
+  if (end - begin <= 1 || fine_bits == 0) {
+    for (uint32_t e = begin + threadIdx.x; e < end; e += kBlock) order[e] = fidx[e];
+    return;
+  }
+  const int bins = 1 << fine_bits;
+  for (int b = threadIdx.x; b < bins; b += kBlock) h[b] = 0;
+  __syncthreads();
+  for (uint32_t e = begin + threadIdx.x; e < end; e += kBlock) atomicAdd(&h[fkeys[e]], 1u);
+  __syncthreads();
+  // Exclusive scan of the bins: each thread owns bins [t * per, (t + 1) * per).
+  const int per = (bins + kBlock - 1) / kBlock;
+  const int b0 = threadIdx.x * per;
+  uint32_t sum = 0;
+  for (int b = b0; b < min(bins, b0 + per); ++b) sum += h[b];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int d = 1; d < kBlock; d <<= 1) {
+    const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = begin + part[threadIdx.x] - sum;
+  for (int b = b0; b < min(bins, b0 + per); ++b) {
+    const uint32_t c = h[b];
+    h[b] = run;
+    run += c;
+  }
+  __syncthreads();
+  for (uint32_t e = begin + threadIdx.x; e < end; e += kBlock)
+    order[atomicAdd(&h[fkeys[e]], 1u)] = fidx[e];
 }
 
 // Persistent grid: as many workgroups as the device holds at once (from
@@ -625,44 +703,29 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     p.qcloud = a.qcloud;
   }
 
-  // Query order, by a counting sort (rank = atomic count within the key's
-  // bucket, so the order inside a bucket is arbitrary). One 2-D / 3-D tree:
+  // Query order: keys, then the two-level bucket sort. One 2-D / 3-D tree:
   // key = Morton code of the query in the cloud's box, with up to a few
   // cells per query (at most 2^21 cells); lanes of a wave then walk
-  // neighbouring paths.
-  // Batched trees and other dimensions: key = bucket start of the leaf the
-  // greedy descent reaches, cut to 16 bits (bucket starts are global, so
-  // this also groups the queries of batched trees by cloud). One zeroed
-  // buffer holds the search's chunk counters and the histogram.
-  DevBuf<uint32_t> counters, keys, starts;
-  DevBuf<int32_t> ranks, order;
+  // neighbouring paths. Batched trees and other dimensions: key = bucket
+  // start of the leaf the greedy descent reaches, cut to 16 bits (bucket
+  // starts are global, so this also groups the queries of batched trees by
+  // cloud).
+  DevBuf<uint32_t> counters(kXcds, s), keys, counts, starts, fkeys;
+  DevBuf<int32_t> fidx, order;
   DevBuf<uint8_t> scan_tmp;
-  const bool sort = ix.params.query_sort && a.m > 1 && ix.node_count > 1;
-  const bool morton = sort && ix.clouds == 1 && (ix.dim == 2 || ix.dim == 3);
-  int axis_bits = 0;
-  uint32_t shift = 0;
-  int64_t buckets = 0;
-  if (morton) {
-    const int max_bits = ix.dim == 3 ? 7 : 10;
-    axis_bits = std::max(1, std::min(max_bits, (int(bits_needed(uint64_t(a.m))) + 2) / ix.dim));
-    buckets = int64_t(1) << (axis_bits * ix.dim);
-  } else if (sort) {
-    const uint32_t full_bits = std::max(1u, bits_needed(uint64_t(ix.n)));
-    shift = full_bits > 16 ? full_bits - 16 : 0;
-    buckets = int64_t(1) << (full_bits - shift);
-  }
-  counters.alloc(size_t(kXcds + buckets), s);
-  k_zero_counters<<<unsigned(ceil_div(kXcds + buckets, kBlock)), kBlock, 0, s>>>(counters.p,
-                                                                                 kXcds + buckets);
+  k_zero_counters<<<1, kBlock, 0, s>>>(counters.p, kXcds);
   p.work = counters.p;
+  const bool sort = ix.params.query_sort && a.m > 1 && ix.node_count > 1;
   if (sort) {
-    uint32_t* const hist = counters.p + kXcds;
+    const bool morton = ix.clouds == 1 && (ix.dim == 2 || ix.dim == 3);
     const unsigned g = unsigned(ceil_div(a.m, kBlock));
     keys.alloc(size_t(a.m), s);
-    ranks.alloc(size_t(a.m), s);
-    order.alloc(size_t(a.m), s);
-    starts.alloc(size_t(buckets), s);
+    int key_bits = 0;
     if (morton) {
+      const int max_bits = ix.dim == 3 ? 7 : 10;
+      const int axis_bits =
+          std::max(1, std::min(max_bits, (int(bits_needed(uint64_t(a.m))) + 2) / ix.dim));
+      key_bits = axis_bits * ix.dim;
       float lo[3] = {0.f, 0.f, 0.f}, sc[3] = {0.f, 0.f, 0.f};
       const float top = float((1u << axis_bits) - 1u);
       for (int d = 0; d < ix.dim; ++d) {
@@ -672,31 +735,54 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
       }
       if (ix.dim == 3)
         k_morton<3><<<g, kBlock, 0, s>>>(a.query, a.qstride, a.m, axis_bits, lo[0], lo[1], lo[2],
-                                         sc[0], sc[1], sc[2], keys.p, ranks.p, hist);
+                                         sc[0], sc[1], sc[2], keys.p);
       else
         k_morton<2><<<g, kBlock, 0, s>>>(a.query, a.qstride, a.m, axis_bits, lo[0], lo[1], lo[2],
-                                         sc[0], sc[1], sc[2], keys.p, ranks.p, hist);
-    } else if (ix.dim == 3) {
-      k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
-                                       a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p,
-                                       ranks.p, hist);
-    } else if (ix.dim == 2) {
-      k_locate<2><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
-                                       a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p,
-                                       ranks.p, hist);
+                                         sc[0], sc[1], sc[2], keys.p);
     } else {
-      k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
-                                       a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p,
-                                       ranks.p, hist);
+      const uint32_t full_bits = std::max(1u, bits_needed(uint64_t(ix.n)));
+      const uint32_t shift = full_bits > 16 ? full_bits - 16 : 0;
+      key_bits = int(full_bits - shift);
+      if (ix.dim == 3)
+        k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
+                                         a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p);
+      else if (ix.dim == 2)
+        k_locate<2><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
+                                         a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p);
+      else
+        k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
+                                         a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p);
     }
     NH_LAUNCH_CHECK();
+    const int coarse_bits = std::min(kCoarseBits, key_bits);
+    const int fine_bits = std::min(kFineBitsMax, key_bits - coarse_bits);
+    const int coarse_bins = 1 << coarse_bits;
+    const int shift_fine = key_bits - coarse_bits;  // key bits below the coarse ones
+    const int64_t tiles = ceil_div(a.m, kSortTile);
+    const int64_t ncounts = int64_t(coarse_bins) * tiles;
+    if (shift_fine > fine_bits) {
+      // Drop key bits below the fine ones (only when the key is wider than
+      // coarse + fine bits).
+      k_shift_keys<<<g, kBlock, 0, s>>>(keys.p, a.m, uint32_t(shift_fine - fine_bits));
+    }
+    counts.alloc(size_t(ncounts), s);
+    starts.alloc(size_t(ncounts), s);
+    fkeys.alloc(size_t(a.m), s);
+    fidx.alloc(size_t(a.m), s);
+    order.alloc(size_t(a.m), s);
+    k_sort_hist<<<unsigned(tiles), kBlock, 0, s>>>(keys.p, a.m, fine_bits, coarse_bins, tiles,
+                                                    counts.p);
+    NH_LAUNCH_CHECK();
     size_t bytes = 0;
-    NH_CHECK(rocprim::exclusive_scan(nullptr, bytes, hist, starts.p, uint32_t(0),
-                                     size_t(buckets), rocprim::plus<uint32_t>(), s));
+    NH_CHECK(rocprim::exclusive_scan(nullptr, bytes, counts.p, starts.p, uint32_t(0),
+                                     size_t(ncounts), rocprim::plus<uint32_t>(), s));
     scan_tmp.alloc(bytes + 16, s);
-    NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, hist, starts.p, uint32_t(0),
-                                     size_t(buckets), rocprim::plus<uint32_t>(), s));
-    k_place<<<g, kBlock, 0, s>>>(a.m, keys.p, ranks.p, starts.p, order.p);
+    NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, counts.p, starts.p, uint32_t(0),
+                                     size_t(ncounts), rocprim::plus<uint32_t>(), s));
+    k_sort_scatter<<<unsigned(tiles), kBlock, 0, s>>>(keys.p, a.m, fine_bits, coarse_bins, tiles,
+                                                       starts.p, fkeys.p, fidx.p);
+    k_sort_fine<<<unsigned(coarse_bins), kBlock, 0, s>>>(fkeys.p, fidx.p, starts.p, tiles, a.m,
+                                                         fine_bits, coarse_bins, order.p);
     NH_LAUNCH_CHECK();
     p.order = order.p;
   }
