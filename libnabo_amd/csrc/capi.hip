@@ -517,6 +517,13 @@ int nabo_hip_batch_knn(const nabo_hip_index* index, const float* d_query, int32_
       arg_error("query, offsets or output pointer is null");
     DeviceScope scope(ix.device);
     hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    // A replay of this call captured in a HIP graph made an illegal memory
+    // access (single-tree knn replays correctly); until that is understood,
+    // capture of the batched search is refused.
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    NH_CHECK(hipStreamIsCapturing(s, &capture));
+    if (capture != hipStreamCaptureStatusNone)
+      throw Error(NABO_HIP_ERR_UNSUPPORTED, "batched knn cannot be captured in a HIP graph");
     DevBuf<int32_t> qcloud;
     if (ix.clouds > 1 && m > 0) {
       qcloud.alloc(size_t(m), s);
