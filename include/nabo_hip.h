@@ -168,6 +168,22 @@ int nabo_hip_get_bounds(const nabo_hip_index* index, float* min_bound, float* ma
  * in leaf order (BucketEntry::index, nabo/nabo_private.h:158-168). */
 int nabo_hip_get_tree(const nabo_hip_index* index, uint32_t* nodes, int32_t* bucket_indices);
 
+/* Serialized copy of a built index (kd-tree, batched kd-trees or brute
+ * force) in caller-owned host memory, e.g. to keep an evicted tree in host
+ * RAM and re-admit it later with one copy instead of a rebuild. No
+ * counterpart in the reference, which has no tree (de)serialization
+ * (nabo/nabo_private.h:158-168 keeps the tree in process memory only).
+ * nabo_hip_serialize: with buf null, sets *bytes to the size needed; else
+ * writes the index into buf, which must hold *bytes (capacity). Pinned host
+ * memory (hipHostMalloc) makes the copies faster. */
+int nabo_hip_serialize(const nabo_hip_index* index, void* buf, uint64_t capacity,
+                       uint64_t* bytes);
+/* Recreates an index from nabo_hip_serialize output on a device (-1: the
+ * current one) without a build; searches give the same results as on the
+ * serialized index. Rejects buffers that are truncated or not produced by
+ * this library version. */
+int nabo_hip_deserialize(const void* buf, uint64_t bytes, int32_t device, nabo_hip_index** out);
+
 /* Kernel timing for measurement tools. While enabled, each launch of the
  * search kernels is bracketed by HIP events recorded on its own stream.
  * collect() waits for the recorded events, returns the summed kernel time in
@@ -193,12 +209,17 @@ typedef struct nabo_hip_server_config {
   int32_t slots;              /* concurrent streams (default 4) */
   int64_t max_merge_queries;  /* cap on queries merged into one launch */
   int32_t device;             /* -1: current */
-  int32_t reserved[9];
+  int64_t host_cache_bytes;   /* evicted trees kept serialized in pinned host
+                                 memory up to this size (0: none); a cached
+                                 tree is re-admitted by a copy, not a build */
+  int32_t reserved[6];
 } nabo_hip_server_config;
 
 typedef struct nabo_hip_server_stats {
   int64_t submitted, completed, launches, merged_requests;
   int64_t builds, evictions, resident_trees, resident_bytes, peak_bytes;
+  int64_t readmits;           /* trees re-admitted from the host cache */
+  int64_t host_cached_bytes;  /* serialized trees held in host memory */
 } nabo_hip_server_stats;
 
 void nabo_hip_server_config_default(nabo_hip_server_config* c);

@@ -100,14 +100,15 @@ class ServerConfig(C.Structure):
         ("slots", C.c_int32),
         ("max_merge_queries", C.c_int64),
         ("device", C.c_int32),
-        ("reserved", C.c_int32 * 9),
+        ("host_cache_bytes", C.c_int64),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
 class ServerStats(C.Structure):
     _fields_ = [(name, C.c_int64) for name in (
         "submitted", "completed", "launches", "merged_requests", "builds", "evictions",
-        "resident_trees", "resident_bytes", "peak_bytes")]
+        "resident_trees", "resident_bytes", "peak_bytes", "readmits", "host_cached_bytes")]
 
 
 EXPORTED = [
@@ -118,6 +119,7 @@ EXPORTED = [
     "nabo_hip_batch_knn", "nabo_hip_batch_roots", "nabo_hip_server_config_default",
     "nabo_hip_server_create", "nabo_hip_server_destroy", "nabo_hip_server_add_cloud",
     "nabo_hip_server_submit", "nabo_hip_server_wait", "nabo_hip_server_get_stats",
+    "nabo_hip_serialize", "nabo_hip_deserialize",
 ]
 
 _LIBS: dict = {}
@@ -172,6 +174,8 @@ def load_library(path: str | None = None):
                                            vp, C.POINTER(i64)]
     lib.nabo_hip_server_wait.argtypes = [vp, i64, C.POINTER(C.c_double)]
     lib.nabo_hip_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+    lib.nabo_hip_serialize.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
+    lib.nabo_hip_deserialize.argtypes = [vp, C.c_uint64, i32, C.POINTER(vp)]
     lib.nabo_hip_kernel_timing.argtypes = [i32]
     lib.nabo_hip_kernel_timing_collect.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     for name in EXPORTED:
@@ -240,6 +244,12 @@ _PARAM_KEYS = {"bucketSize": "bucket_size", "querySort": "query_sort",
                "bfPrecision": "bf_precision", "bfExtra": "bf_extra", "device": "device"}
 
 
+class _Serialized:
+    def __init__(self, data, device):
+        self.data = data
+        self.device = device
+
+
 class NearestNeighbourSearch:
     """pynabo.NearestNeighbourSearch on the GPU (python/nabo.cpp:95-163).
 
@@ -252,6 +262,18 @@ class NearestNeighbourSearch:
                  creationOptionFlags: int = 0, params: dict | None = None):
         self._lib = load_library()
         self._h = None
+        if isinstance(cloud, _Serialized):
+            h = C.c_void_p()
+            buf = cloud.data
+            _check(self._lib, self._lib.nabo_hip_deserialize(_ptr(buf), buf.nbytes, int(cloud.device),
+                                                             C.byref(h)))
+            self._h = h
+            inf = self.info()
+            self.cloud = None
+            self.searchType = SearchType(inf.search_type)
+            self.creationOptionFlags = int(inf.creation_flags)
+            self.last_touched = 0
+            return
         kw = {}
         for key, val in (params or {}).items():
             if key not in _PARAM_KEYS:
@@ -302,6 +324,20 @@ class NearestNeighbourSearch:
         hi = np.empty(d, np.float32)
         _check(self._lib, self._lib.nabo_hip_get_bounds(self._h, _ptr(lo), _ptr(hi)))
         return lo, hi
+
+    def serialize(self) -> np.ndarray:
+        """The built index as a uint8 array in host memory (nabo_hip_serialize)."""
+        size = C.c_uint64(0)
+        _check(self._lib, self._lib.nabo_hip_serialize(self._h, None, 0, C.byref(size)))
+        buf = np.empty(size.value, np.uint8)
+        _check(self._lib, self._lib.nabo_hip_serialize(self._h, _ptr(buf), buf.nbytes,
+                                                       C.byref(size)))
+        return buf
+
+    @classmethod
+    def deserialize(cls, data, device: int = -1) -> "NearestNeighbourSearch":
+        """Recreates an index from serialize() output without a build."""
+        return cls(_Serialized(np.ascontiguousarray(data, dtype=np.uint8), device))
 
     def tree(self):
         """(nodes[count, 2] uint32, bucket point indices[n]) as built on the GPU."""
@@ -481,7 +517,7 @@ class Server:
     """
 
     def __init__(self, budget_bytes: int = 0, slots: int = 4, max_merge_queries: int = 1 << 22,
-                 device: int = -1):
+                 device: int = -1, host_cache_bytes: int = 0):
         self._lib = load_library()
         self._h = None
         cfg = ServerConfig()
@@ -490,6 +526,7 @@ class Server:
         cfg.slots = int(slots)
         cfg.max_merge_queries = int(max_merge_queries)
         cfg.device = int(device)
+        cfg.host_cache_bytes = int(host_cache_bytes)
         h = C.c_void_p()
         _check(self._lib, self._lib.nabo_hip_server_create(C.byref(cfg), C.byref(h)))
         self._h = h

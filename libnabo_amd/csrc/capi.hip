@@ -151,6 +151,70 @@ void build_index(nabo_hip_index* h, const float* d_cloud, int64_t n, int32_t row
   NH_CHECK(hipStreamSynchronize(s));
 }
 
+// Serialized index: a fixed header, then the device arrays and the host
+// vectors, each at a 256-byte aligned offset of the buffer.
+constexpr char kSerialMagic[8] = {'N', 'A', 'B', 'O', 'H', 'I', 'P', '1'};
+constexpr uint32_t kSerialVersion = 1;
+
+struct SerialHeader {
+  char magic[8];
+  uint32_t version;
+  uint32_t header_bytes;
+  int32_t search_type, rows, dim, clouds;
+  int64_t n;
+  uint32_t creation_flags, bucket_size, dim_bits, dim_mask;
+  nabo_hip_params params;
+  int32_t depth, pts_stride, bf_stride, lp_kp;
+  int64_t node_count, lp_n_pad, device_bytes;
+  // byte offset and size of each array (size 0: absent)
+  uint64_t off[9], len[9];
+  uint64_t total;
+};
+enum { kArNodes, kArLinks, kArPts, kArRoots, kArCloud, kArCloudLp, kArOffsets, kArMin, kArMax };
+
+uint64_t align256(uint64_t v) { return (v + 255) / 256 * 256; }
+
+SerialHeader serial_layout(const Index& ix) {
+  SerialHeader h{};
+  std::memcpy(h.magic, kSerialMagic, 8);
+  h.version = kSerialVersion;
+  h.header_bytes = sizeof(SerialHeader);
+  h.search_type = ix.search_type;
+  h.rows = ix.rows;
+  h.dim = ix.dim;
+  h.clouds = ix.clouds;
+  h.n = ix.n;
+  h.creation_flags = ix.creation_flags;
+  h.bucket_size = ix.bucket_size;
+  h.dim_bits = ix.dim_bits;
+  h.dim_mask = ix.dim_mask;
+  h.params = ix.params;
+  h.depth = ix.depth;
+  h.pts_stride = ix.pts_stride;
+  h.bf_stride = ix.bf_stride;
+  h.lp_kp = ix.lp_kp;
+  h.node_count = ix.node_count;
+  h.lp_n_pad = ix.lp_n_pad;
+  h.device_bytes = ix.device_bytes;
+  const bool kd = ix.search_type == NABO_HIP_SEARCH_KDTREE;
+  h.len[kArNodes] = kd ? sizeof(uint2) * uint64_t(ix.node_count + 1) : 0;
+  h.len[kArLinks] = kd ? sizeof(uint4) * uint64_t(ix.node_count) : 0;
+  h.len[kArPts] = kd ? sizeof(float) * uint64_t(ix.n + kPtsPad) * ix.pts_stride : 0;
+  h.len[kArRoots] = ix.d_roots ? sizeof(int32_t) * uint64_t(ix.clouds) : 0;
+  h.len[kArCloud] = ix.d_cloud ? sizeof(float) * uint64_t(ix.n) * ix.dim : 0;
+  h.len[kArCloudLp] = ix.d_cloud_lp ? sizeof(uint16_t) * uint64_t(ix.lp_n_pad) * ix.lp_kp : 0;
+  h.len[kArOffsets] = sizeof(int64_t) * ix.cloud_offsets.size();
+  h.len[kArMin] = sizeof(float) * ix.min_bound.size();
+  h.len[kArMax] = sizeof(float) * ix.max_bound.size();
+  uint64_t at = align256(sizeof(SerialHeader));
+  for (int a = 0; a < 9; ++a) {
+    h.off[a] = at;
+    at = align256(at + h.len[a]);
+  }
+  h.total = at;
+  return h;
+}
+
 void check_knn(const Index& ix, int32_t qrows, int64_t qstride, int64_t m, int32_t k,
                uint32_t flags) {
   const bool allow_self = flags & NABO_HIP_ALLOW_SELF_MATCH;
@@ -196,9 +260,15 @@ __global__ void k_query_clouds(int64_t m, int32_t clouds, const int64_t* __restr
   qcloud[i] = lo;
 }
 
-// The touched counter is cleared by a kernel rather than a memset node so
-// that callers may capture the call in a HIP graph.
+// Stream capture is refused: replays of captured knn calls (which allocate
+// their scratch stream-ordered) made illegal memory accesses on the GPU,
+// intermittently, after other captured graphs had run in the process; the
+// cause was not found.
 void run_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  NH_CHECK(hipStreamIsCapturing(s, &capture));
+  if (capture != hipStreamCaptureStatusNone)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED, "knn calls cannot be captured in a HIP graph");
   if (a.touched) {
     k_set_u64<<<1, 1, 0, s>>>(a.touched, 0);
     NH_LAUNCH_CHECK();
@@ -517,13 +587,6 @@ int nabo_hip_batch_knn(const nabo_hip_index* index, const float* d_query, int32_
       arg_error("query, offsets or output pointer is null");
     DeviceScope scope(ix.device);
     hipStream_t s = static_cast<hipStream_t>(hip_stream);
-    // A replay of this call captured in a HIP graph made an illegal memory
-    // access (single-tree knn replays correctly); until that is understood,
-    // capture of the batched search is refused.
-    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
-    NH_CHECK(hipStreamIsCapturing(s, &capture));
-    if (capture != hipStreamCaptureStatusNone)
-      throw Error(NABO_HIP_ERR_UNSUPPORTED, "batched knn cannot be captured in a HIP graph");
     DevBuf<int32_t> qcloud;
     if (ix.clouds > 1 && m > 0) {
       qcloud.alloc(size_t(m), s);
@@ -611,6 +674,117 @@ int nabo_hip_get_tree(const nabo_hip_index* index, uint32_t* nodes, int32_t* buc
       std::memcpy(&v, &pts[size_t(p * ix.pts_stride + ix.dim)], 4);
       bucket_indices[p] = v;
     }
+  });
+}
+
+int nabo_hip_serialize(const nabo_hip_index* index, void* buf, uint64_t capacity,
+                       uint64_t* bytes) {
+  return guarded([&] {
+    if (!index || !bytes) arg_error("null argument");
+    const Index& ix = index->ix;
+    const SerialHeader h = serial_layout(ix);
+    *bytes = h.total;
+    if (!buf) return;
+    if (capacity < h.total)
+      arg_error("serialization buffer holds " + std::to_string(capacity) + " bytes, " +
+                std::to_string(h.total) + " are needed");
+    char* out = static_cast<char*>(buf);
+    // Zeroed gaps, so that equal indices serialize to equal bytes.
+    std::memset(out, 0, h.off[0]);
+    for (int a = 0; a < 9; ++a) {
+      const uint64_t end = a + 1 < 9 ? h.off[a + 1] : h.total;
+      std::memset(out + h.off[a] + h.len[a], 0, end - h.off[a] - h.len[a]);
+    }
+    std::memcpy(out, &h, sizeof(h));
+    DeviceScope scope(ix.device);
+    hipStream_t s = thread_stream(ix.device);
+    const void* dev[6] = {ix.d_nodes, ix.d_links, ix.d_pts, ix.d_roots, ix.d_cloud, ix.d_cloud_lp};
+    for (int a = 0; a < 6; ++a)
+      if (h.len[a])
+        NH_CHECK(hipMemcpyAsync(out + h.off[a], dev[a], h.len[a], hipMemcpyDeviceToHost, s));
+    if (h.len[kArOffsets]) std::memcpy(out + h.off[kArOffsets], ix.cloud_offsets.data(), h.len[kArOffsets]);
+    if (h.len[kArMin]) std::memcpy(out + h.off[kArMin], ix.min_bound.data(), h.len[kArMin]);
+    if (h.len[kArMax]) std::memcpy(out + h.off[kArMax], ix.max_bound.data(), h.len[kArMax]);
+    NH_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int nabo_hip_deserialize(const void* buf, uint64_t bytes, int32_t device, nabo_hip_index** out) {
+  return guarded([&] {
+    if (!buf || !out) arg_error("null argument");
+    *out = nullptr;
+    SerialHeader h;
+    if (bytes < sizeof(h)) arg_error("serialized index is truncated");
+    std::memcpy(&h, buf, sizeof(h));
+    if (std::memcmp(h.magic, kSerialMagic, 8) != 0 || h.version != kSerialVersion ||
+        h.header_bytes != sizeof(SerialHeader))
+      arg_error("buffer is not a serialized index of this library version");
+    if (h.total > bytes) arg_error("serialized index is truncated");
+    for (int a = 0; a < 9; ++a)
+      if (h.off[a] > h.total || h.len[a] > h.total - h.off[a])
+        arg_error("serialized index has an array outside the buffer");
+    const bool kd = h.search_type == NABO_HIP_SEARCH_KDTREE;
+    if (!kd && h.search_type != NABO_HIP_SEARCH_BRUTE_FORCE) arg_error("unknown search type");
+    if (h.n <= 0 || h.dim <= 0 || h.clouds < 1 ||
+        (h.len[kArOffsets] != 0 && h.len[kArOffsets] != sizeof(int64_t) * uint64_t(h.clouds + 1)))
+      arg_error("serialized index header is inconsistent");
+    if (kd && (h.node_count < 1 ||
+               h.len[kArNodes] != sizeof(uint2) * uint64_t(h.node_count + 1) ||
+               h.len[kArLinks] != sizeof(uint4) * uint64_t(h.node_count) ||
+               h.len[kArPts] != sizeof(float) * uint64_t(h.n + kPtsPad) * uint64_t(h.pts_stride) ||
+               (h.clouds > 1) != (h.len[kArRoots] != 0)))
+      arg_error("serialized kd-tree arrays do not match its header");
+    if (!kd && h.len[kArCloud] != sizeof(float) * uint64_t(h.n) * uint64_t(h.dim))
+      arg_error("serialized brute-force arrays do not match its header");
+    int dev = device;
+    if (dev < 0) NH_CHECK(hipGetDevice(&dev));
+    DeviceScope scope(dev);
+    prepare_device(dev);
+    hipStream_t s = thread_stream(dev);
+    auto* hnd = new nabo_hip_index();
+    Index& ix = hnd->ix;
+    try {
+      ix.device = dev;
+      ix.search_type = h.search_type;
+      ix.rows = h.rows;
+      ix.dim = h.dim;
+      ix.clouds = h.clouds;
+      ix.n = h.n;
+      ix.creation_flags = h.creation_flags;
+      ix.bucket_size = h.bucket_size;
+      ix.dim_bits = h.dim_bits;
+      ix.dim_mask = h.dim_mask;
+      ix.params = h.params;
+      ix.depth = h.depth;
+      ix.pts_stride = h.pts_stride;
+      ix.bf_stride = h.bf_stride;
+      ix.lp_kp = h.lp_kp;
+      ix.node_count = h.node_count;
+      ix.lp_n_pad = h.lp_n_pad;
+      ix.device_bytes = h.device_bytes;
+      const char* in = static_cast<const char*>(buf);
+      void** dev_ptr[6] = {reinterpret_cast<void**>(&ix.d_nodes), reinterpret_cast<void**>(&ix.d_links),
+                           reinterpret_cast<void**>(&ix.d_pts), reinterpret_cast<void**>(&ix.d_roots),
+                           reinterpret_cast<void**>(&ix.d_cloud),
+                           reinterpret_cast<void**>(&ix.d_cloud_lp)};
+      for (int a = 0; a < 6; ++a) {
+        if (!h.len[a]) continue;
+        NH_CHECK(hipMalloc(dev_ptr[a], h.len[a]));
+        NH_CHECK(hipMemcpyAsync(*dev_ptr[a], in + h.off[a], h.len[a], hipMemcpyHostToDevice, s));
+      }
+      ix.cloud_offsets.resize(h.len[kArOffsets] / sizeof(int64_t));
+      if (h.len[kArOffsets])
+        std::memcpy(ix.cloud_offsets.data(), in + h.off[kArOffsets], h.len[kArOffsets]);
+      ix.min_bound.resize(h.len[kArMin] / sizeof(float));
+      ix.max_bound.resize(h.len[kArMax] / sizeof(float));
+      if (h.len[kArMin]) std::memcpy(ix.min_bound.data(), in + h.off[kArMin], h.len[kArMin]);
+      if (h.len[kArMax]) std::memcpy(ix.max_bound.data(), in + h.off[kArMax], h.len[kArMax]);
+      NH_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      delete hnd;
+      throw;
+    }
+    *out = hnd;
   });
 }
 

@@ -496,8 +496,7 @@ __global__ void k_shift_keys(uint32_t* keys, int64_t m, uint32_t shift) {
   if (i < m) keys[i] >>= shift;
 }
 
-// Zeroes the search's chunk counters (a kernel rather than a memset, so the
-// launch sequence can be captured in a graph).
+// Zeroes the search's chunk counters.
 __global__ void k_zero_counters(uint32_t* counters, int64_t n) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e < n) counters[e] = 0;

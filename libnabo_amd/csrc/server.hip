@@ -21,7 +21,9 @@
 //      builder thread and stays pending, so it does not hold back the
 //      requests behind it.
 // Builder thread: evicts least recently used trees that no slot is using
-// until the budget holds, then builds on its own stream, concurrently with
+// until the budget holds (with a host cache, an evicted tree is first
+// serialized into pinned host memory), then builds on its own stream, or
+// re-admits the tree from its host copy with one copy, concurrently with
 // the searches. Slots run concurrently on the device; a request that arrives
 // while others run is admitted as soon as a slot and its tree are ready.
 
@@ -52,6 +54,19 @@ double now_ms() {
       .count();
 }
 
+// A serialized tree (nabo_hip_serialize) in pinned host memory.
+struct HostCopy {
+  void* p = nullptr;
+  uint64_t bytes = 0;
+  uint64_t last_use = 0;
+  HostCopy() = default;
+  HostCopy(const HostCopy&) = delete;
+  HostCopy& operator=(const HostCopy&) = delete;
+  ~HostCopy() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 struct Cloud {
   std::vector<float> data;  // rows x n, column stride `stride`
   int64_t n = 0;
@@ -66,6 +81,7 @@ struct Cloud {
   bool building = false;   // queued for, or in, the builder thread
   int build_status = NABO_HIP_OK;
   std::string build_error;
+  std::unique_ptr<HostCopy> host;  // kept while within the host cache budget
 };
 
 struct Request {
@@ -162,6 +178,7 @@ struct nabo_hip_server {
   int64_t next_id = 1;
   uint64_t clock = 0;
   int64_t resident = 0;
+  int64_t host_cached = 0;  // bytes of serialized trees in host memory
   nabo_hip_server_stats stats{};
 
   void run();
@@ -234,7 +251,9 @@ void nabo_hip_server::release_slot_buffers(Slot& s) {
 // Builder thread: for each queued cloud, evicts least recently used trees
 // that no slot uses until the budget holds (the budget is a target: a tree
 // larger than it is still built when nothing is left to evict), then builds
-// the tree on this thread's stream.
+// the tree on this thread's stream, or deserializes its host copy. Host
+// copies are kept up to host_cache_bytes, least recently used dropped first;
+// a copy stays valid for good since a registered cloud never changes.
 void nabo_hip_server::run_builder() {
   (void)hipSetDevice(device);
   std::unique_lock<std::mutex> lock(mu);
@@ -255,7 +274,7 @@ void nabo_hip_server::run_builder() {
     // requests stays until they have run, so builds cannot undo each other.
     std::vector<char> wanted(clouds.size(), 0);
     for (const Request* r : pending) wanted[size_t(r->cloud)] = 1;
-    std::vector<nabo_hip_index*> victims;
+    std::vector<std::pair<Cloud*, nabo_hip_index*>> victims;
     while (cfg.hbm_budget_bytes > 0 && resident + slot_bytes() + estimate > cfg.hbm_budget_bytes) {
       Cloud* victim = nullptr;
       for (size_t i = 0; i < clouds.size(); ++i) {
@@ -265,21 +284,61 @@ void nabo_hip_server::run_builder() {
           victim = o;
       }
       if (!victim) break;
-      victims.push_back(victim->index);
+      victims.emplace_back(victim, victim->index);
       victim->index = nullptr;
       resident -= victim->bytes;
       victim->bytes = 0;
       ++stats.evictions;
     }
     lock.unlock();
-    for (auto* v : victims) nabo_hip_destroy(v);
+    // Evicted trees: keep a serialized copy in pinned host memory when the
+    // host cache allows (only this thread touches the copies).
+    std::vector<std::unique_ptr<HostCopy>> copies(victims.size());
+    for (size_t v = 0; v < victims.size(); ++v) {
+      Cloud* vc = victims[v].first;
+      if (cfg.host_cache_bytes > 0 && !vc->host) {
+        uint64_t bytes = 0;
+        if (nabo_hip_serialize(victims[v].second, nullptr, 0, &bytes) == NABO_HIP_OK &&
+            int64_t(bytes) <= cfg.host_cache_bytes) {
+          auto hc = std::make_unique<HostCopy>();
+          if (hipHostMalloc(&hc->p, bytes, hipHostMallocDefault) == hipSuccess &&
+              nabo_hip_serialize(victims[v].second, hc->p, bytes, &hc->bytes) == NABO_HIP_OK)
+            copies[v] = std::move(hc);
+        }
+      }
+      nabo_hip_destroy(victims[v].second);
+    }
+    // Re-admission from the host copy when there is one, else a build.
     nabo_hip_index* index = nullptr;
-    const int rc = nabo_hip_create(c.data.data(), c.n, c.rows, c.stride, c.dim,
-                                   NABO_HIP_SEARCH_KDTREE, 0, &c.params, &index);
+    const bool readmit = c.host != nullptr;
+    int rc;
+    if (readmit) {
+      rc = nabo_hip_deserialize(c.host->p, c.host->bytes, device, &index);
+    } else {
+      rc = nabo_hip_create(c.data.data(), c.n, c.rows, c.stride, c.dim, NABO_HIP_SEARCH_KDTREE, 0,
+                           &c.params, &index);
+    }
     std::string err = rc == NABO_HIP_OK ? "" : nabo_hip_last_error();
     nabo_hip_info info{};
     if (rc == NABO_HIP_OK) nabo_hip_get_info(index, &info);
+    std::vector<std::unique_ptr<HostCopy>> dropped;
     lock.lock();
+    for (size_t v = 0; v < victims.size(); ++v) {
+      if (!copies[v]) continue;
+      host_cached += int64_t(copies[v]->bytes);
+      copies[v]->last_use = ++clock;
+      victims[v].first->host = std::move(copies[v]);
+    }
+    if (c.host) c.host->last_use = ++clock;
+    // Host cache over budget: drop the least recently used copies.
+    while (host_cached > cfg.host_cache_bytes) {
+      Cloud* oldest = nullptr;
+      for (auto& o : clouds)
+        if (o->host && (!oldest || o->host->last_use < oldest->host->last_use)) oldest = o.get();
+      if (!oldest) break;
+      host_cached -= int64_t(oldest->host->bytes);
+      dropped.push_back(std::move(oldest->host));
+    }
     c.building = false;
     c.build_status = rc;
     c.build_error = err;
@@ -287,10 +346,16 @@ void nabo_hip_server::run_builder() {
       c.index = index;
       c.bytes = info.device_bytes;
       resident += c.bytes;
-      ++stats.builds;
+      if (readmit)
+        ++stats.readmits;
+      else
+        ++stats.builds;
       stats.peak_bytes = std::max(stats.peak_bytes, resident + slot_bytes());
     }
     wake.notify_all();
+    lock.unlock();
+    dropped.clear();  // hipHostFree outside the lock
+    lock.lock();
   }
 }
 
@@ -644,6 +709,7 @@ int nabo_hip_server_get_stats(const nabo_hip_server* s, nabo_hip_server_stats* o
     for (const auto& c : s->clouds) trees += c->index != nullptr;
     out->resident_trees = trees;
     out->resident_bytes = s->resident;
+    out->host_cached_bytes = s->host_cached;
   });
 }
 

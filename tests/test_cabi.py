@@ -42,3 +42,23 @@ def test_params_defaults():
 def test_last_error_is_empty_at_start():
     lib = nabo.load_library()
     assert lib.nabo_hip_last_error() in (b"", None)
+
+
+def test_struct_sizes_match_the_header():
+    # nabo_hip_server_config / _stats as laid out by the C compiler
+    assert ctypes.sizeof(nabo.ServerConfig) == 64
+    assert ctypes.sizeof(nabo.ServerStats) == 11 * 8
+
+
+def test_deserialize_rejects_bad_buffers_before_any_device_call():
+    lib = nabo.load_library()
+    h = ctypes.c_void_p()
+    short = (ctypes.c_uint8 * 16)()
+    rc = lib.nabo_hip_deserialize(ctypes.cast(short, ctypes.c_void_p), 16, -1, ctypes.byref(h))
+    assert rc == 1 and b"truncated" in lib.nabo_hip_last_error()
+    junk = (ctypes.c_uint8 * 4096)(*([7] * 4096))
+    rc = lib.nabo_hip_deserialize(ctypes.cast(junk, ctypes.c_void_p), 4096, -1, ctypes.byref(h))
+    assert rc == 1 and b"not a serialized index" in lib.nabo_hip_last_error()
+    assert not h.value
+    size = ctypes.c_uint64(0)
+    assert lib.nabo_hip_serialize(None, None, 0, ctypes.byref(size)) == 1

@@ -1,19 +1,20 @@
-"""kNN calls captured in a HIP graph (torch.cuda.CUDAGraph) and replayed.
+"""kNN calls and HIP graph (stream) capture.
 
-Single-tree knn_device calls are capturable; the batched search refuses
-capture (test_batched_capture_refused) since a replay of it faulted.
-
-A knn_device call launches its whole sequence (counter zeroing, query sort
-kernels, scan, search) on the caller's stream and allocates its scratch
-stream-ordered, so the call can be captured and replayed. Each replay runs
-with eager GPU work and synchronizations between replays, and every replay
-must reproduce the eager call's indices and dists2 bit for bit (which the
-other GPU tests tie to the oracle).
+Replays of captured knn calls made illegal memory accesses on the GPU,
+intermittently: once for a batched search, once for a single-tree search
+after other captured graphs had run in the same process (eager calls never
+did). The cause was not found, so every knn entry point refuses stream
+capture with NABO_HIP_ERR_UNSUPPORTED before it launches anything; the replay
+test below is kept, skipped, for when capture is supported again.
 """
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+REPLAY_SKIP = ("replays of captured knn calls faulted the GPU intermittently (cause not found); "
+               "knn refuses stream capture")
+UNSUPPORTED = 4  # NABO_HIP_ERR_UNSUPPORTED
 
 
 def _capture(fn):
@@ -43,10 +44,26 @@ def _replays_match(graph, outs, want, reps=3):
         torch.cuda.synchronize()
         for o, w in zip(outs, want):
             assert torch.equal(o, w), f"replay {r} differs from the eager call"
-        # eager work between the replays
         torch.randn(4096, device="cuda").sum().item()
 
 
+def _refused_in_capture(fn, nabo):
+    """Runs fn inside a capture that then ends normally; returns its errors."""
+    import torch
+
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    errors = []
+    with torch.cuda.graph(graph):
+        try:
+            fn()
+        except nabo.NaboError as e:
+            errors.append(e)
+    torch.cuda.synchronize()
+    return errors
+
+
+@pytest.mark.skip(reason=REPLAY_SKIP)
 @pytest.mark.parametrize("k,sort", [(16, 1), (5, 1), (16, 0)])
 def test_kdtree_knn_graph_replay(nabo, k, sort):
     import torch
@@ -67,32 +84,27 @@ def test_kdtree_knn_graph_replay(nabo, k, sort):
     _replays_match(graph, [gi, gd], [ei, ed])
 
 
-@pytest.mark.skip(reason="a graph replay of the batched knn made an illegal memory access on the "
-                         "GPU (cause not found); nabo_hip_batch_knn refuses stream capture")
-def test_batched_knn_graph_replay(nabo):
+def test_kdtree_capture_refused(nabo):
     import torch
 
-    rng = np.random.default_rng(12)
-    sizes = [3000, 17, 9, 12000, 500, 12, 7000]  # each at least k + 1 points
-    clouds = [(rng.random((s, 3)) * rng.uniform(1, 30)).astype(np.float32) for s in sizes]
-    queries = [(c + rng.normal(scale=0.05, size=c.shape)).astype(np.float32) for c in clouds]
-    off = np.concatenate([[0], np.cumsum([len(c) for c in clouds])]).astype(np.int64)
-    qoff = np.concatenate([[0], np.cumsum([len(q) for q in queries])]).astype(np.int64)
-    pts = torch.from_numpy(np.concatenate(clouds)).cuda()
-    qs = torch.from_numpy(np.concatenate(queries)).cuda()
-    dqoff = torch.from_numpy(qoff).cuda()
-    b = nabo.BatchSearch(pts, off, 3, 0, {"bucketSize": 8})
-    k = 8
-    ei, ed = b.knn(qs, dqoff, k)
+    from libnabo_amd import datasets
+
+    cloud, query = datasets.uniform_cube(5_000, 2_000, seed=14)
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0, {"bucketSize": 8})
+    dq = torch.from_numpy(query).cuda()
+    gi = torch.empty((len(query), 4), dtype=torch.int32, device="cuda")
+    gd = torch.empty((len(query), 4), dtype=torch.float32, device="cuda")
+    errors = _refused_in_capture(lambda: nns.knn_device(dq, 4, gi, gd), nabo)
+    assert errors and errors[0].code == UNSUPPORTED and "captured" in str(errors[0])
+    # eager calls still work and match the host path
+    nns.knn_device(dq, 4, gi, gd)
     torch.cuda.synchronize()
-    gi, gd = torch.empty_like(ei), torch.empty_like(ed)
-    graph = _capture(lambda: b.knn(qs, dqoff, k, indices=gi, dists2=gd))
-    _replays_match(graph, [gi, gd], [ei, ed])
-    b.close()
+    hi, hd = nns.knn(query, 4)
+    assert np.array_equal(gi.cpu().numpy(), hi)
+    assert np.array_equal(gd.cpu().numpy().view(np.uint32), hd.view(np.uint32))
 
 
 def test_batched_capture_refused(nabo):
-    """Capture of the batched search is refused on the host, before any launch."""
     import torch
 
     rng = np.random.default_rng(13)
@@ -104,17 +116,8 @@ def test_batched_capture_refused(nabo):
     b = nabo.BatchSearch(pts, off, 3, 0, {"bucketSize": 8})
     gi = torch.empty((800, 4), dtype=torch.int32, device="cuda")
     gd = torch.empty((800, 4), dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
-    graph = torch.cuda.CUDAGraph()
-    errors = []
-    with torch.cuda.graph(graph):  # the capture itself ends normally
-        try:
-            b.knn(qs, dqoff, 4, indices=gi, dists2=gd)
-        except nabo.NaboError as e:
-            errors.append(str(e))
-    torch.cuda.synchronize()
-    assert errors and "captured" in errors[0]
-    # eager calls still work afterwards
+    errors = _refused_in_capture(lambda: b.knn(qs, dqoff, 4, indices=gi, dists2=gd), nabo)
+    assert errors and errors[0].code == UNSUPPORTED and "captured" in str(errors[0])
     i, _ = b.knn(qs, dqoff, 4, optionFlags=int(nabo.SearchOptionFlags.ALLOW_SELF_MATCH))
     torch.cuda.synchronize()
     assert bool((i[:, 0] == torch.from_numpy(np.r_[np.arange(300), np.arange(500)]).cuda()).all())

@@ -1,0 +1,77 @@
+"""Serialized indices (nabo_hip_serialize / nabo_hip_deserialize) on the GPU.
+
+A deserialized index must be the same index: identical tree words and bucket
+order, identical info, and knn results bit-identical to the original's (which
+the other GPU tests tie to the oracle). The served loop with a host cache
+re-admits evicted trees from their host copies, and its results stay
+bit-identical to the plain search.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+from libnabo_amd import datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def test_kdtree_roundtrip(nabo):
+    cloud, query = datasets.lidar_scene(50_000, seed=21)
+    a = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0, {"bucketSize": 8})
+    blob = a.serialize()
+    b = nabo.NearestNeighbourSearch.deserialize(blob)
+    ia, ib = a.info(), b.info()
+    for f in ("n", "dim", "node_count", "depth", "bucket_size", "device_bytes", "clouds"):
+        assert getattr(ia, f) == getattr(ib, f), f
+    na, ba = a.tree()
+    nb, bb = b.tree()
+    assert np.array_equal(na, nb) and np.array_equal(ba, bb)
+    assert all(np.array_equal(x, y) for x, y in zip(a.bounds, b.bounds))
+    for k, eps, r in [(16, 0.0, 1.0), (1, 0.1, np.inf), (5, 0.0, 0.3)]:
+        ai, ad = a.knn(query, k, eps, 0, r)
+        bi, bd = b.knn(query, k, eps, 0, r)
+        assert np.array_equal(ai, bi) and np.array_equal(ad.view(np.uint32), bd.view(np.uint32))
+    # a copy of the copy is byte-identical
+    assert np.array_equal(b.serialize(), blob)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_brute_force_roundtrip(nabo, precision):
+    cloud, query = datasets.dirichlet_descriptors(4_000, 500, seed=22)
+    a = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, 33, 0,
+                                    {"bfPrecision": precision, "bfExtra": 8})
+    b = nabo.NearestNeighbourSearch.deserialize(a.serialize())
+    ai, ad = a.knn(query, 8)
+    bi, bd = b.knn(query, 8)
+    assert np.array_equal(ai, bi) and np.array_equal(ad.view(np.uint32), bd.view(np.uint32))
+
+
+def test_served_host_cache_readmits(nabo):
+    rng = np.random.default_rng(23)
+    clouds = [(rng.random((int(rng.integers(20_000, 50_000)), 3)) * 10).astype(np.float32)
+              for _ in range(5)]
+    reqs = []
+    for _ in range(30):
+        c = int(rng.integers(0, len(clouds)))
+        base = clouds[c][rng.integers(0, len(clouds[c]), int(rng.integers(10, 3000)))]
+        reqs.append((c, (base + rng.normal(scale=0.05, size=base.shape)).astype(np.float32)))
+    print("case served host cache", flush=True)
+    # a budget of about one tree: every build or re-admission evicts
+    srv = nabo.Server(budget_bytes=1_500_000, slots=2, host_cache_bytes=64 << 20)
+    ids = [srv.add_cloud(c, 3, {"bucketSize": 8}) for c in clouds]
+    outs = []
+    for c, q in reqs:  # one at a time, so evictions and re-admissions alternate
+        outs.append(srv.submit(ids[c], q, 8).wait())
+    st = srv.stats()
+    print(st, flush=True)
+    assert st["completed"] == len(reqs)
+    assert st["evictions"] > 0 and st["readmits"] > 0
+    assert st["builds"] == len(set(c for c, _ in reqs))  # each tree built once
+    assert st["host_cached_bytes"] > 0
+    plain = [nabo.NearestNeighbourSearch(c, nabo.SearchType.KDTREE_HIP, 3, 0, {"bucketSize": 8})
+             for c in clouds]
+    for (c, q), (gi, gd) in zip(reqs, outs):
+        oi, od = plain[c].knn(q, 8)
+        assert np.array_equal(gi, oi) and np.array_equal(gd.view(np.uint32), od.view(np.uint32))
+    srv.close()
