@@ -256,10 +256,10 @@ def _serial_queue(arrivals, service):
     return lat, done - arrivals[0]
 
 
-def _served_run(nabo, pool, reqs, arrivals, budget, slots, warm):
+def _served_run(nabo, pool, reqs, arrivals, budget, slots, warm, host_cache=0):
     import threading
 
-    srv = nabo.Server(budget_bytes=budget, slots=slots)
+    srv = nabo.Server(budget_bytes=budget, slots=slots, host_cache_bytes=host_cache)
     ids = [srv.add_cloud(c, 3) for c in pool]
     if warm:  # every tree resident before the stream starts
         for cid, c in zip(ids, pool):
@@ -289,7 +289,8 @@ def _served_run(nabo, pool, reqs, arrivals, budget, slots, warm):
     makespan = max(s + l / 1e3 for s, l in zip(submitted, lat)) - submitted[0]
     st = srv.stats()
     st = {key: st[key] - (base[key] if key in ("builds", "evictions", "submitted", "completed",
-                                                "launches", "merged_requests") else 0)
+                                                "launches", "merged_requests", "readmits")
+                          else 0)
           for key in st}
     srv.close()
     return lat, makespan, st, handles
@@ -305,7 +306,10 @@ def bench_served(requests: int = 120, slots: int = 4):
     same arrival times, fed with each call's measured service time).
     budget: HBM budget of half the trees' total bytes, cold start: the server
     with LRU eviction against serial plain calls under the same budget and
-    LRU policy (trees built on a miss, evicted least recently used first)."""
+    LRU policy (trees built on a miss, evicted least recently used first).
+    budget_host_cache: the same, with the server keeping evicted trees
+    serialized in pinned host memory (room for all of them), so a re-admitted
+    tree costs one copy instead of a build."""
     import libnabo_amd as nabo
     from libnabo_amd import datasets
 
@@ -369,6 +373,17 @@ def bench_served(requests: int = 120, slots: int = 4):
                                           "p99_ms": pct(serial_lat_b, 99),
                                           "queries_per_s": round(total_q / serial_span_b, 1)},
                      "results_equal_plain": bool(same_b)}
+    host_cache = 2 * sum(tree_bytes.values())
+    lat_h, span_h, st_h, handles_h = _served_run(nabo, pool, reqs, arrivals, budget, slots, False,
+                                                 host_cache)
+    same_h = all(np.array_equal(h.indices, ri) and np.array_equal(h.dists2, rd)
+                 for h, (ri, rd) in zip(handles_h, ref))
+    out["budget_host_cache"] = {"budget_mb": round(budget / 2**20, 1),
+                                "host_cache_mb": round(host_cache / 2**20, 1),
+                                "served": {"p50_ms": pct(lat_h, 50), "p99_ms": pct(lat_h, 99),
+                                           "queries_per_s": round(total_q / span_h, 1),
+                                           "stats": st_h},
+                                "results_equal_plain": bool(same_h)}
     return out
 
 
