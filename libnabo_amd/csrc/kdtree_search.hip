@@ -138,6 +138,30 @@ __device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], fl
   hd[0] = fminf(hd[0], value);
 }
 
+// Admission tests of a bucket entry (kdtree_cpu.cpp:392-394) and of a far
+// child (:415-417 / :433-435). With epsilon = 0 (EXACT) the far-child test
+// `rd <= r2 && rd * 1 < head` and the entry test `d <= r2 && d < head` are
+// both `x < min(head, r2n)`, r2n being the next float above r2 (+inf for an
+// infinite radius, -inf for a NaN one, which admits nothing as in the
+// reference): one compare instead of two (and a multiply).
+template <bool EXACT>
+struct Admit {
+  float r2, r2n, err2;
+  __device__ __forceinline__ Admit(float r2_, float err2_) : r2(r2_), err2(err2_) {
+    // r2 = r * r is +0 or more: the next float is the next bit pattern
+    // (FLT_MAX goes to +inf).
+    r2n = r2_ != r2_ ? -INFINITY
+                     : (r2_ == INFINITY ? INFINITY
+                                        : __uint_as_float(__float_as_uint(fmaxf(r2_, 0.f)) + 1u));
+  }
+  __device__ __forceinline__ bool node(float rd, float head) const {
+    return EXACT ? rd < fminf(head, r2n) : (rd <= r2 && rd * err2 < head);
+  }
+  __device__ __forceinline__ bool point(float d, float head) const {
+    return EXACT ? d < fminf(head, r2n) : (d <= r2 && d < head);
+  }
+};
+
 __device__ __forceinline__ uint2 right_word(const uint4* links, uint32_t n) {
   return reinterpret_cast<const uint2*>(links)[2 * size_t(n)];
 }
@@ -171,9 +195,9 @@ __device__ __forceinline__ float point_dist(const SearchParams& a, const Vec<DIM
 
 // Scans one leaf bucket: loads up to kLeafBatch entries at once, then tests
 // them in bucket order (kdtree_cpu.cpp:380-398).
-template <int DIM, int K>
+template <int DIM, int K, bool EXACT>
 __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>& q, uint32_t count,
-                                          int64_t start, float r2, float (&hd)[K],
+                                          int64_t start, const Admit<EXACT>& adm, float (&hd)[K],
                                           int32_t (&hi)[K]) {
   if constexpr (DIM > 0 && DIM <= 3) {
     const float4* pts4 = reinterpret_cast<const float4*>(a.pts) + start;
@@ -193,7 +217,7 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
             const float diff = q.v[d] - c[d];
             dist = dist + diff * diff;
           }
-          if (dist <= r2 && dist < hd[K - 1] && (a.allow_self || dist > FLT_EPSILON))
+          if (adm.point(dist, hd[K - 1]) && (a.allow_self || dist > FLT_EPSILON))
             heap_insert<K>(hd, hi, dist, __float_as_int(c[DIM]));
         }
       }
@@ -202,7 +226,7 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
     for (uint32_t j = 0; j < count; ++j) {
       int32_t index;
       const float dist = point_dist<DIM>(a, q, start + j, &index);
-      if (dist <= r2 && dist < hd[K - 1] && (a.allow_self || dist > FLT_EPSILON))
+      if (adm.point(dist, hd[K - 1]) && (a.allow_self || dist > FLT_EPSILON))
         heap_insert<K>(hd, hi, dist, index);
     }
   }
@@ -217,7 +241,7 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
 // another still has work.
 // Up to K = 16 three waves per SIMD fit without spills (at most 168 VGPRs),
 // one more than the LDS stack of a 28-level tree lets a CU hold.
-template <int DIM, int K, bool LDS_STACK>
+template <int DIM, int K, bool LDS_STACK, bool EXACT>
 __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(SearchParams a) {
   // Per-lane stack: in LDS when the tree is shallow enough ([slot][lane] of
   // this workgroup), else in HBM ([slot][lane] over the grid).
@@ -296,6 +320,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
       const float r = a.radii[qi];
       r2 = r * r;
     }
+    const Admit<EXACT> adm(r2, a.max_err2);
 
     float rd = 0.f;
     uint32_t top = base_off;  // offset of the next free slot
@@ -311,7 +336,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
       const float rd_far = rd + (-old_off * old_off + new_off * new_off);
       const uint32_t right = w.x >> a.dim_bits;
       go_right = new_off > 0;
-      if (rd_far <= r2 && rd_far * a.max_err2 < hd[K - 1]) {
+      if (adm.node(rd_far, hd[K - 1])) {
         slot(top) = make_uint2(go_right ? node + 1 : right, __float_as_uint(rd_far));
         top += sstride;
       }
@@ -336,7 +361,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
         rt = right_word(a.links, n);
       }
       const uint32_t count = nd.x >> a.dim_bits;
-      scan_leaf<DIM, K>(a, q, count, int64_t(nd.y), r2, hd, hi);
+      scan_leaf<DIM, K, EXACT>(a, q, count, int64_t(nd.y), adm, hd, hi);
       touched += count;
 
       // Pop: kPopBatch entries per round trip to the stack. Restore entries
@@ -358,7 +383,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
               off.set(e[j].x & ~kRestore, __uint_as_float(e[j].y));
             } else {
               const float rdf = __uint_as_float(e[j].y);
-              if (rdf <= r2 && rdf * a.max_err2 < hd[K - 1]) {
+              if (adm.node(rdf, hd[K - 1])) {
                 descend = true;
                 n = e[j].x;
                 rd = rdf;
@@ -597,13 +622,13 @@ __global__ __launch_bounds__(kBlock) void k_sort_fine(const uint32_t* __restrict
 // Persistent grid: as many workgroups as the device holds at once (from
 // the occupancy of this instantiation), a multiple of the XCD count.
 // The occupancy is cached per device and LDS size (tree depth).
-template <int DIM, int K, bool LDS>
+template <int DIM, int K, bool LDS, bool EXACT>
 unsigned resident_grid(size_t lds, int cus, int dev) {
   static std::atomic<int> cache[kMaxCachedDevices][kLdsStackSlots + 2];
   const size_t slot = std::min<size_t>(lds / (kSearchBlock * sizeof(uint2)), kLdsStackSlots + 1);
   int per_cu = dev >= 0 && dev < kMaxCachedDevices ? cache[dev][slot].load() : 0;
   if (per_cu < 1) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kd_search<DIM, K, LDS>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kd_search<DIM, K, LDS, EXACT>,
                                                      kSearchBlock, lds) != hipSuccess ||
         per_cu < 1)
       per_cu = 1;
@@ -612,16 +637,16 @@ unsigned resident_grid(size_t lds, int cus, int dev) {
   return unsigned(std::max(kXcds, (cus * per_cu) / kXcds * kXcds));
 }
 
-template <int DIM, int K>
-void launch_search(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& stack,
-                   hipStream_t s) {
+template <int DIM, int K, bool EXACT>
+void launch_search_as(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& stack,
+                      hipStream_t s) {
   if (depth <= kLdsStackSlots) {
     const size_t lds = size_t(depth) * kSearchBlock * sizeof(uint2);
-    const unsigned grid = resident_grid<DIM, K, true>(lds, cus, dev);
+    const unsigned grid = resident_grid<DIM, K, true, EXACT>(lds, cus, dev);
     KernelTimer timer(s);
-    k_kd_search<DIM, K, true><<<grid, kSearchBlock, lds, s>>>(p);
+    k_kd_search<DIM, K, true, EXACT><<<grid, kSearchBlock, lds, s>>>(p);
   } else {
-    const unsigned grid = resident_grid<DIM, K, false>(0, cus, dev);
+    const unsigned grid = resident_grid<DIM, K, false, EXACT>(0, cus, dev);
     const int64_t lanes = int64_t(grid) * kSearchBlock;
     if (uint64_t(lanes) * uint64_t(depth + 1) * sizeof(uint2) >= (uint64_t(1) << 32))
       throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
@@ -629,8 +654,19 @@ void launch_search(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& s
     p.stack = stack.p;
     p.stack_lanes = lanes;
     KernelTimer timer(s);
-    k_kd_search<DIM, K, false><<<grid, kSearchBlock, 0, s>>>(p);
+    k_kd_search<DIM, K, false, EXACT><<<grid, kSearchBlock, 0, s>>>(p);
   }
+}
+
+template <int DIM, int K>
+void launch_search(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& stack,
+                   hipStream_t s) {
+  // epsilon = 0 (max_err2 == 1, so the multiply is exact) takes the
+  // one-compare admission tests.
+  if (p.max_err2 == 1.f)
+    launch_search_as<DIM, K, true>(p, cus, dev, depth, stack, s);
+  else
+    launch_search_as<DIM, K, false>(p, cus, dev, depth, stack, s);
 }
 
 template <int DIM>
