@@ -344,18 +344,39 @@ __global__ __launch_bounds__(256) void k_bf_shortlist(ShortlistParams a) {
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
       if (ks < ks_count) acc = Lp<F16>::mfma(*reinterpret_cast<const V*>(prow + 16 * ks), b[ks], acc);
+    // Candidates of the tile: the values under the head at the tile's start.
     uint32_t pending = 0;
     const float head = ld[LCAP - 1];
 #pragma unroll
     for (int r = 0; r < 16; ++r) pending |= (acc[r] < head ? 1u : 0u) << r;
-    while (__ballot(pending != 0)) {
-      if (pending) {
-        const uint32_t r = __builtin_ctz(pending);
-        pending &= pending - 1;
-        const float v = pick16(acc, r);
-        if (v < ld[LCAP - 1]) {
-          const int32_t row = int32_t((r & 3) + 8 * (r >> 2) + 4 * half);
-          list_insert<LCAP>(ld, li, v, int32_t(t + row));
+    if constexpr (LCAP <= 16) {
+      // Short lists: each lane takes its own next candidate (a select over
+      // the 16 values); the wave loops as often as its busiest lane.
+      while (__ballot(pending != 0)) {
+        if (pending) {
+          const uint32_t r = __builtin_ctz(pending);
+          pending &= pending - 1;
+          const float v = pick16(acc, r);
+          if (v < ld[LCAP - 1]) {
+            const int32_t row = int32_t((r & 3) + 8 * (r >> 2) + 4 * half);
+            list_insert<LCAP>(ld, li, v, int32_t(t + row));
+          }
+        }
+      }
+    } else if (__any(pending != 0)) {
+      // Long lists: value r of every lane in row order, r uniform across
+      // the wave (a relative register read: the select and its
+      // temporaries would not fit beside the lists), r skipped when no lane
+      // has a candidate there.
+#pragma unroll 1
+      for (int r = 0; r < 16; ++r) {
+        const bool mine = (pending >> r) & 1u;
+        if (__any(mine)) {
+          const float v = acc[r];
+          if (mine && v < ld[LCAP - 1]) {
+            const int32_t row = int32_t((r & 3) + 8 * (r >> 2) + 4 * half);
+            list_insert<LCAP>(ld, li, v, int32_t(t + row));
+          }
         }
       }
     }
