@@ -307,7 +307,9 @@ def bench_served(requests: int = 120, slots: int = 4):
     budget: HBM budget of half the trees' total bytes, cold start: the server
     with LRU eviction against serial plain calls under the same budget and
     LRU policy (trees built on a miss, evicted least recently used first).
-    budget_host_cache: the same, with the server keeping evicted trees
+    saturated: all requests at once on the warm server, makespan against
+    the sum of the plain calls' service times (throughput at full load).
+    budget_host_cache: the same budget, with the server keeping evicted trees
     serialized in pinned host memory (room for all of them), so a re-admitted
     tree costs one copy instead of a build."""
     import libnabo_amd as nabo
@@ -373,6 +375,21 @@ def bench_served(requests: int = 120, slots: int = 4):
                                           "p99_ms": pct(serial_lat_b, 99),
                                           "queries_per_s": round(total_q / serial_span_b, 1)},
                      "results_equal_plain": bool(same_b)}
+    # saturated: every request submitted at once to the warm server; its
+    # makespan against the serial sum of the plain calls' service times
+    sat_arrivals = np.zeros(len(reqs))
+    lat_s, span_s, st_s, handles_s = _served_run(nabo, pool, reqs, sat_arrivals, 0, slots, True)
+    same_s = all(np.array_equal(h.indices, ri) and np.array_equal(h.dists2, rd)
+                 for h, (ri, rd) in zip(handles_s, ref))
+    serial_span_s = float(np.sum(service))
+    out["saturated"] = {"served": {"queries_per_s": round(total_q / span_s, 1),
+                                   "makespan_ms": round(span_s * 1e3, 2),
+                                   "p50_ms": pct(lat_s, 50), "p99_ms": pct(lat_s, 99),
+                                   "stats": st_s},
+                        "serial_plain": {"queries_per_s": round(total_q / serial_span_s, 1),
+                                         "makespan_ms": round(serial_span_s * 1e3, 2)},
+                        "speedup": round(serial_span_s / span_s, 2),
+                        "results_equal_plain": bool(same_s)}
     host_cache = 2 * sum(tree_bytes.values())
     lat_h, span_h, st_h, handles_h = _served_run(nabo, pool, reqs, arrivals, budget, slots, False,
                                                  host_cache)
