@@ -9,8 +9,8 @@
 //            near child; popped after the near subtree, when the reference
 //            tests `rd <= maxRadius2 && rd*maxError2 < head` (:415-417 /
 //            :433-435) against the k-best as it is then;
-//   restore  {cut dim | flag, previous off[cd]}: replaces the pending entry
-//            while the far subtree is visited and puts off[cd] back after
+//   restore  {cut dim | flag, previous off[cd]^2}: replaces the pending entry
+//            while the far subtree is visited and puts off[cd]^2 back after
 //            it (:419-424 / :437-442).
 // The far child's rd is formed from the same operands, in the same order,
 // as the reference, so the visit order, the pruning decisions, ties and
@@ -302,11 +302,13 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
       continue;
     }
     const int64_t qi = a.order ? a.order[i] : i;
-    Vec<DIM> q, off;
+    // off2[d] = off[d] * off[d] (rounded as the reference rounds it): the
+    // reference only ever uses off[cd] squared, and -(a * a) is (-a) * a.
+    Vec<DIM> q, off2;
     const float* qp = a.query + qi * a.qstride;
     for (int d = 0; d < dim; ++d) {
       q.v[d] = qp[d];
-      off.v[d] = 0.f;
+      off2.v[d] = 0.f;
     }
     float hd[K];
     int32_t hi[K];
@@ -332,8 +334,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
     auto split = [&](uint32_t node, uint2 w, bool& go_right) -> uint32_t {
       const uint32_t cd = w.x & a.dim_mask;
       const float new_off = q.get(cd) - __uint_as_float(w.y);
-      const float old_off = off.get(cd);
-      const float rd_far = rd + (-old_off * old_off + new_off * new_off);
+      const float rd_far = rd + (-off2.get(cd) + new_off * new_off);
       const uint32_t right = w.x >> a.dim_bits;
       go_right = new_off > 0;
       if (adm.node(rd_far, hd[K - 1])) {
@@ -380,7 +381,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
           if (!descend && top != base_off) {
             top -= sstride;
             if (e[j].x & kRestore) {
-              off.set(e[j].x & ~kRestore, __uint_as_float(e[j].y));
+              off2.set(e[j].x & ~kRestore, __uint_as_float(e[j].y));
             } else {
               const float rdf = __uint_as_float(e[j].y);
               if (adm.node(rdf, hd[K - 1])) {
@@ -402,9 +403,9 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
         rt = make_uint2(lk.x, lk.y);
         const uint32_t pcd = lk.z;
         const float new_off = q.get(pcd) - __uint_as_float(lk.w);
-        slot(top) = make_uint2(kRestore | pcd, __float_as_uint(off.get(pcd)));
+        slot(top) = make_uint2(kRestore | pcd, __float_as_uint(off2.get(pcd)));
         top += sstride;
-        off.set(pcd, new_off);
+        off2.set(pcd, new_off * new_off);
       }
     }
 
