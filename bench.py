@@ -108,9 +108,13 @@ def bench_brute_force(log_cpu: bool):
     dq = torch.from_numpy(query).cuda()
     out = {"workload": "dirichlet_200k_x_50k_d33_k8"}
     exact = None
-    for prec, extra in (("fp32", 0), ("bf16", 8), ("fp16", 8), ("bf16", 32), ("fp16", 32)):
-        nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, dim, 0,
-                                          {"bfPrecision": prec, "bfExtra": extra})
+    stl = None
+    for prec, extra in (("fp32", 0), ("fp32_stl", 0), ("bf16", 8), ("fp16", 8), ("bf16", 32),
+                        ("fp16", 32)):
+        # fp32_stl: BRUTE_FORCE, the reference's own IndexHeapSTL order
+        nns = nabo.NearestNeighbourSearch(
+            cloud, nabo.SearchType.BRUTE_FORCE if prec == "fp32_stl" else nabo.SearchType.BRUTE_FORCE_HIP,
+            dim, 0, {"bfPrecision": "fp32" if prec == "fp32_stl" else prec, "bfExtra": extra})
         di = torch.empty((m, k), dtype=torch.int32, device="cuda")
         dd = torch.empty((m, k), dtype=torch.float32, device="cuda")
         nns.knn_device(dq, k, di, dd)
@@ -126,9 +130,12 @@ def bench_brute_force(log_cpu: bool):
         kms, launches = nabo.kernel_timing_collect()
         nabo.kernel_timing(False)
         gi, gd = di.cpu().numpy(), dd.cpu().numpy()
-        key = prec if prec == "fp32" else f"{prec}_m{extra}"
+        key = prec if prec.startswith("fp32") else f"{prec}_m{extra}"
         entry = {"ms": round(ms, 3), "queries_per_s": round(m / (ms / 1e3), 1)}
-        if prec == "fp32":
+        if prec == "fp32_stl":
+            stl = (gi, gd)
+            entry["slowdown_vs_fp32"] = round(ms / out["fp32"]["ms"], 2)
+        elif prec == "fp32":
             exact = (gi, gd)
             flops = 3.0 * dim * pairs
             entry["roofline"] = {"bound": "valu_fp32", "achieved": round(flops / (ms / 1e3) / 1e12, 2),
@@ -161,6 +168,11 @@ def bench_brute_force(log_cpu: bool):
                                "kind": "port", "sample": f"oracle brute force, {nq} queries"}
         out["fp32_parity_sample"] = bool(np.array_equal(exact[1][:nq].view(np.uint32),
                                                         od.view(np.uint32)))
+        # the STL-heap form returns the reference's own order: bit-identical rows
+        si, sd = o.bf_knn(cloud, query[:nq], k, flags=0, threads=threads)
+        out["fp32_stl_parity_sample"] = bool(
+            np.array_equal(stl[0][:nq], si) and
+            np.array_equal(stl[1][:nq].view(np.uint32), sd.view(np.uint32)))
     return out
 
 

@@ -7,9 +7,13 @@
 // appended after the reference's: KDTREE_HIP and BRUTE_FORCE_HIP. Header
 // only; link with libnabo_hip.so. Every search runs on the GPU through the
 // C ABI of nabo_hip.h:
-//   KDTREE_HIP, KDTREE_LINEAR_HEAP, KDTREE_TREE_HEAP -> HIP kd-tree
-//     (KDTREE_LINEAR_HEAP results: ascending, ties in visit order)
-//   BRUTE_FORCE_HIP, BRUTE_FORCE                      -> HIP brute force
+//   KDTREE_HIP, KDTREE_LINEAR_HEAP -> HIP kd-tree, IndexHeapBruteForceVector
+//     results (ascending, a new entry after equal distances)
+//   KDTREE_TREE_HEAP               -> HIP kd-tree, IndexHeapSTL results
+//   BRUTE_FORCE                    -> HIP brute force, IndexHeapSTL results
+//     (the reference's own heap: same order and ties)
+//   BRUTE_FORCE_HIP                -> HIP brute force, ascending results,
+//     ties by index (chunked over the cloud, faster)
 //   KDTREE_CL_* / BRUTE_FORCE_CL                      -> error, as the
 //     reference without OpenCL (nabo/nabo.cpp:148-151)
 // Scalar type float only (NNSearchF); NNSearchD searches throw.
@@ -230,7 +234,7 @@ struct HipSearch : public NearestNeighbourSearch<T, CloudType> {
   nabo_hip_index* handle = nullptr;
 
   HipSearch(const CloudType& cloud, const Index dim, const unsigned creationOptionFlags,
-            int hipType, const Parameters& params)
+            int hipType, int heap, const Parameters& params)
       : Base(cloud, dim, creationOptionFlags) {
     static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value, "");
     if (!std::is_same<T, float>::value)
@@ -242,6 +246,7 @@ struct HipSearch : public NearestNeighbourSearch<T, CloudType> {
     p.bf_precision = params.get<int>("bfPrecision", NABO_HIP_BF_FP32);
     p.bf_extra = int(params.get<unsigned>("bfExtra", 32u));
     p.device = params.get<int>("device", -1);
+    p.heap = params.get<int>("heap", heap);
     const float* data = reinterpret_cast<const float*>(&cloud.coeff(0, 0));
     check(nabo_hip_create(data, int64_t(cloud.cols()), int32_t(cloud.rows()),
                           int64_t(cloud.outerStride()), int32_t(dim), hipType,
@@ -293,15 +298,17 @@ NearestNeighbourSearch<T, CloudType>* NearestNeighbourSearch<T, CloudType>::crea
   switch (preferedType) {
     case BRUTE_FORCE:
     case BRUTE_FORCE_HIP:
-      return new hip_detail::HipSearch<T, CloudType>(cloud, dim, creationOptionFlags,
-                                                     NABO_HIP_SEARCH_BRUTE_FORCE,
-                                                     additionalParameters);
+      return new hip_detail::HipSearch<T, CloudType>(
+          cloud, dim, creationOptionFlags, NABO_HIP_SEARCH_BRUTE_FORCE,
+          preferedType == BRUTE_FORCE ? NABO_HIP_HEAP_STL : NABO_HIP_HEAP_LINEAR,
+          additionalParameters);
     case KDTREE_LINEAR_HEAP:
     case KDTREE_TREE_HEAP:
     case KDTREE_HIP:
-      return new hip_detail::HipSearch<T, CloudType>(cloud, dim, creationOptionFlags,
-                                                     NABO_HIP_SEARCH_KDTREE,
-                                                     additionalParameters);
+      return new hip_detail::HipSearch<T, CloudType>(
+          cloud, dim, creationOptionFlags, NABO_HIP_SEARCH_KDTREE,
+          preferedType == KDTREE_TREE_HEAP ? NABO_HIP_HEAP_STL : NABO_HIP_HEAP_LINEAR,
+          additionalParameters);
     case KDTREE_CL_PT_IN_NODES:
     case KDTREE_CL_PT_IN_LEAVES:
     case BRUTE_FORCE_CL:

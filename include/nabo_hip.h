@@ -52,6 +52,16 @@ enum {
   NABO_HIP_BF_FP16 = 2  /* MFMA fp16 shortlist of k+m, exact FP32 rescoring */
 };
 
+/* k-best containers (nabo_hip_params.heap): which of the reference's index
+ * heaps the results follow. LINEAR is IndexHeapBruteForceVector
+ * (nabo/index_heap.h:263-369; KDTREE_LINEAR_HEAP): ascending output, a new
+ * entry goes after equal distances. STL is IndexHeapSTL
+ * (nabo/index_heap.h:51-165; KDTREE_TREE_HEAP and BRUTE_FORCE): binary-heap
+ * order unless SORT_RESULTS (then std::sort_heap order), the same entry
+ * dropped among equal distances as std::pop_heap drops. Both keep the same
+ * distances. */
+enum { NABO_HIP_HEAP_LINEAR = 0, NABO_HIP_HEAP_STL = 1 };
+
 typedef struct nabo_hip_index nabo_hip_index;
 
 /* Creation parameters. The reference's Parameters map carries only
@@ -63,7 +73,8 @@ typedef struct nabo_hip_params {
   int32_t bf_precision;  /* NABO_HIP_BF_* */
   int32_t bf_extra;      /* m: shortlist size is k + m for bf16/fp16 */
   int32_t device;        /* HIP device ordinal, -1 = current */
-  int32_t reserved[11];
+  int32_t heap;          /* NABO_HIP_HEAP_*; STL brute force needs bf_precision FP32 */
+  int32_t reserved[10];
 } nabo_hip_params;
 
 /* Message of the last failure on this thread ("" if none). */

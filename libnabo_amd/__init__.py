@@ -55,6 +55,7 @@ class SearchOptionFlags(enum.IntFlag):
 
 STATUS = {0: "ok", 1: "argument", 2: "hip", 3: "memory", 4: "unsupported"}
 BF_PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2}
+HEAP = {"linear": 0, "stl": 1}  # NABO_HIP_HEAP_*
 
 
 class NaboError(RuntimeError):
@@ -73,7 +74,8 @@ class Params(C.Structure):
         ("bf_precision", C.c_int32),
         ("bf_extra", C.c_int32),
         ("device", C.c_int32),
-        ("reserved", C.c_int32 * 11),
+        ("heap", C.c_int32),
+        ("reserved", C.c_int32 * 10),
     ]
 
 
@@ -228,8 +230,12 @@ def _ptr(a):
 
 
 def make_params(bucket_size: int = 8, query_sort: bool = True, bf_precision: str = "fp32",
-                bf_extra: int = 32, device: int = -1, **unknown) -> Params:
-    """Parameters map (nabo/nabo.h:231-254): only "bucketSize" is a reference key."""
+                bf_extra: int = 32, device: int = -1, heap="linear", **unknown) -> Params:
+    """Parameters map (nabo/nabo.h:231-254): only "bucketSize" is a reference key.
+
+    ``heap``: "linear" (IndexHeapBruteForceVector results) or "stl"
+    (IndexHeapSTL results: heap order unless SORT_RESULTS).
+    """
     p = Params()
     load_library().nabo_hip_params_default(C.byref(p))
     p.bucket_size = int(bucket_size)
@@ -237,11 +243,13 @@ def make_params(bucket_size: int = 8, query_sort: bool = True, bf_precision: str
     p.bf_precision = BF_PRECISION[bf_precision]
     p.bf_extra = int(bf_extra)
     p.device = int(device)
+    p.heap = HEAP[heap] if isinstance(heap, str) else int(heap)
     return p
 
 
 _PARAM_KEYS = {"bucketSize": "bucket_size", "querySort": "query_sort",
-               "bfPrecision": "bf_precision", "bfExtra": "bf_extra", "device": "device"}
+               "bfPrecision": "bf_precision", "bfExtra": "bf_extra", "device": "device",
+               "heap": "heap"}
 
 
 class _Serialized:
@@ -279,6 +287,10 @@ class NearestNeighbourSearch:
             if key not in _PARAM_KEYS:
                 raise NaboError(1, f"unknown parameter {key!r}")
             kw[_PARAM_KEYS[key]] = val
+        # The reference's types keep their k-best containers: KDTREE_TREE_HEAP
+        # and BRUTE_FORCE use IndexHeapSTL (nabo.cpp:143, brute_force_cpu.cpp:79).
+        if int(searchType) in (SearchType.KDTREE_TREE_HEAP, SearchType.BRUTE_FORCE):
+            kw.setdefault("heap", "stl")
         p = make_params(**kw)
         if int(searchType) in (SearchType.KDTREE_LINEAR_HEAP, SearchType.KDTREE_TREE_HEAP):
             searchType = SearchType.KDTREE_HIP

@@ -12,6 +12,12 @@
 // contiguous chunk of the points, staged through LDS in tiles; each lane
 // keeps a sorted k-best of its chunk; a merge kernel then combines the chunks
 // of a query in chunk order, which preserves the lowest-index tie rule.
+//
+// IndexHeapSTL form (the reference's own heap, brute_force_cpu.cpp:79): one
+// chunk, so each lane admits the points of the whole cloud in index order
+// exactly as the reference does, into a binary heap in LDS (StlHeap);
+// results are then bit-identical to the reference's, including the order
+// and which of equal distances are kept.
 
 #include <cfloat>
 
@@ -23,6 +29,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTile = 64;
+constexpr int kBfUnroll = 4;  // STL form: points whose distances are summed together
+static_assert(kTile % kBfUnroll == 0, "a tile holds whole groups of points");
 
 template <int K>
 __device__ __forceinline__ void bf_insert(float (&hd)[K], int32_t (&hi)[K], float value,
@@ -37,6 +45,22 @@ __device__ __forceinline__ void bf_insert(float (&hd)[K], int32_t (&hi)[K], floa
   const bool place0 = hd[0] > value;
   hd[0] = place0 ? value : hd[0];
   hi[0] = place0 ? index : hi[0];
+}
+
+// Sequential sum of squared differences over DIMP coordinates, where those
+// from dim on are zero in both the point and the query: each adds +0 to a
+// sum that is +0 or more, which leaves it unchanged, so the result is the
+// sum over the first dim coordinates, bit for bit, without a branch per
+// coordinate.
+template <int DIMP>
+__device__ __forceinline__ float padded_dist(const float* pp, const float (&q)[DIMP]) {
+  float dist = 0.f;
+#pragma unroll
+  for (int d = 0; d < DIMP; ++d) {
+    const float diff = pp[d] - q[d];
+    dist = dist + diff * diff;
+  }
+  return dist;
 }
 
 struct BfParams {
@@ -94,15 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_bf_chunk(BfParams a) {
     __syncthreads();
     if (valid) {
       for (int pt = 0; pt < tn; ++pt) {
-        const float* pp = tile + pt * DIMP;
-        float dist = 0.f;
-#pragma unroll
-        for (int d = 0; d < DIMP; ++d) {
-          if (d < dim) {
-            const float diff = pp[d] - q[d];
-            dist = dist + diff * diff;
-          }
-        }
+        const float dist = padded_dist<DIMP>(tile + pt * DIMP, q);
         if (dist <= r2 && dist < hd[K - 1] && (a.allow_self || dist > FLT_EPSILON))
           bf_insert<K>(hd, hi, dist, int32_t(t0 + pt));
       }
@@ -145,6 +161,101 @@ __global__ __launch_bounds__(kBlock) void k_bf_merge(BfParams a, int32_t* __rest
       out_d2[qi * a.k + j - (K - a.k)] = hd[j];
     }
   }
+}
+
+constexpr int kStlBlock = 64;  // one wave per workgroup: m / 64 waves spread over the CUs
+
+template <int DIMP>
+__global__ __launch_bounds__(kStlBlock) void k_bf_stl(BfParams a, int sort_results,
+                                                      int32_t* __restrict__ out_idx,
+                                                      float* __restrict__ out_d2) {
+  // A tile is kTile points, DIMP floats each, zero from dim on. It is read
+  // from the packed cloud (dim floats per point) with 16-byte loads.
+  __shared__ float4 tile4[kTile * DIMP / 4];
+  float* tile = reinterpret_cast<float*>(tile4);
+  extern __shared__ uint2 heaps[];  // k slots x kStlBlock lanes
+  const int64_t qi = int64_t(blockIdx.x) * kStlBlock + threadIdx.x;
+  const int dim = a.dim;
+  const bool valid = qi < a.m;
+
+  float q[DIMP];
+#pragma unroll
+  for (int d = 0; d < DIMP; ++d) q[d] = (valid && d < dim) ? a.query[qi * a.qstride + d] : 0.f;
+  float r2 = a.max_r2;
+  if (valid && a.radii) {
+    const float r = a.radii[qi];
+    r2 = r * r;
+  }
+  StlHeap<LdsHeapStore> heap;
+  if (valid) heap.reset(LdsHeapStore{heaps + threadIdx.x}, a.k);
+  for (int e = threadIdx.x; e < kTile * DIMP / 4; e += kStlBlock)
+    tile4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+  for (int64_t t0 = 0; t0 < a.n; t0 += kTile) {
+    const int tn = int(min<int64_t>(kTile, a.n - t0));
+    const int floats = tn * dim;
+    const float* src = a.cloud + t0 * dim;  // 16-byte aligned: t0 is a multiple of kTile
+    constexpr int kVec = kTile * DIMP / 4 / kStlBlock;  // float4 loads per lane, at most
+    float4 v[kVec];
+#pragma unroll
+    for (int j = 0; j < kVec; ++j) {
+      const int e = threadIdx.x + j * kStlBlock;
+      if (4 * e + 4 <= floats) {
+        v[j] = reinterpret_cast<const float4*>(src)[e];
+      } else {
+        v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (4 * e < floats) v[j].x = src[4 * e];
+        if (4 * e + 1 < floats) v[j].y = src[4 * e + 1];
+        if (4 * e + 2 < floats) v[j].z = src[4 * e + 2];
+      }
+    }
+    __syncthreads();  // the previous tile is no longer read
+#pragma unroll
+    for (int j = 0; j < kVec; ++j) {
+      const int e = threadIdx.x + j * kStlBlock;
+      if (4 * e < floats) {
+        int pt = (4 * e) / dim, d = 4 * e - pt * dim;
+        const float c[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (4 * e + i < floats) tile[pt * DIMP + d] = c[i];
+          if (++d == dim) {
+            d = 0;
+            ++pt;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      // kBfUnroll independent sums at a time (one wave per SIMD here, so the
+      // wave's own instructions must overlap); rows past tn are read but not
+      // admitted. Admissions stay in index order.
+      for (int pt = 0; pt < tn; pt += kBfUnroll) {
+        float dist[kBfUnroll];
+#pragma unroll
+        for (int u = 0; u < kBfUnroll; ++u) dist[u] = padded_dist<DIMP>(tile + (pt + u) * DIMP, q);
+#pragma unroll
+        for (int u = 0; u < kBfUnroll; ++u)
+          if (pt + u < tn && dist[u] <= r2 && dist[u] < heap.head &&
+              (a.allow_self || dist[u] > FLT_EPSILON))
+            heap.replace_head(int32_t(t0 + pt + u), dist[u]);
+      }
+    }
+  }
+  if (!valid) return;
+  if (sort_results) heap.sort();
+  heap.write(out_idx + qi * a.k, out_d2 + qi * a.k);
+}
+
+template <int DIMP>
+void launch_bf_stl(const BfParams& p, int sort_results, hipStream_t s, int32_t* oi, float* od) {
+  if (p.cstride != p.dim) throw Error(NABO_HIP_ERR_ARGUMENT, "brute-force cloud is not packed");
+  const size_t heap_lds = size_t(p.k) * kStlBlock * sizeof(uint2);
+  static_assert(kTile * 128 * sizeof(float) + 64 * kStlBlock * sizeof(uint2) <= 65536,
+                "tile and heaps fit the 64 KiB of LDS a workgroup may hold");
+  k_bf_stl<DIMP><<<unsigned(ceil_div(p.m, kStlBlock)), kStlBlock, heap_lds, s>>>(p, sort_results,
+                                                                                 oi, od);
 }
 
 template <int DIMP, int K>
@@ -524,6 +635,9 @@ void bf_prepare(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s) 
   const int prec = ix.params.bf_precision;
   if (prec != NABO_HIP_BF_FP32 && prec != NABO_HIP_BF_BF16 && prec != NABO_HIP_BF_FP16)
     throw Error(NABO_HIP_ERR_ARGUMENT, "unknown brute-force precision");
+  if (ix.params.heap == NABO_HIP_HEAP_STL && prec != NABO_HIP_BF_FP32)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                "the STL-heap brute force (the reference's order) is exact FP32 only");
   if (prec != NABO_HIP_BF_FP32) {
     if (ix.dim > kLpMaxDim)
       throw Error(NABO_HIP_ERR_UNSUPPORTED, "the bf16/fp16 shortlist supports dim <= 62");
@@ -620,6 +734,24 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
       shortlist_l<true>(sp, p, s, a.out_idx, a.out_d2);
     else
       shortlist_l<false>(sp, p, s, a.out_idx, a.out_d2);
+    NH_LAUNCH_CHECK();
+    return;
+  }
+  if (ix.params.heap == NABO_HIP_HEAP_STL) {
+    const int sort = (a.flags & NABO_HIP_SORT_RESULTS) ? 1 : 0;
+    KernelTimer timer(s);
+    if (ix.dim <= 4)
+      launch_bf_stl<4>(p, sort, s, a.out_idx, a.out_d2);
+    else if (ix.dim <= 8)
+      launch_bf_stl<8>(p, sort, s, a.out_idx, a.out_d2);
+    else if (ix.dim <= 16)
+      launch_bf_stl<16>(p, sort, s, a.out_idx, a.out_d2);
+    else if (ix.dim <= 36)
+      launch_bf_stl<36>(p, sort, s, a.out_idx, a.out_d2);
+    else if (ix.dim <= 64)
+      launch_bf_stl<64>(p, sort, s, a.out_idx, a.out_d2);
+    else
+      launch_bf_stl<128>(p, sort, s, a.out_idx, a.out_d2);
     NH_LAUNCH_CHECK();
     return;
   }

@@ -108,6 +108,8 @@ void check_create(int64_t n, int32_t rows, int64_t stride, int32_t dim, int32_t 
     arg_error("Unknown search type");
   if (n >= (int64_t(1) << 30))
     throw Error(NABO_HIP_ERR_UNSUPPORTED, "clouds of 2^30 points or more are not supported");
+  if (p.heap != NABO_HIP_HEAP_LINEAR && p.heap != NABO_HIP_HEAP_STL)
+    arg_error("Unknown heap type " + std::to_string(p.heap));
   if (type == NABO_HIP_SEARCH_KDTREE) {
     const uint32_t bucket = p.bucket_size;
     if (bucket < 2)
@@ -736,6 +738,8 @@ int nabo_hip_deserialize(const void* buf, uint64_t bytes, int32_t device, nabo_h
       arg_error("serialized kd-tree arrays do not match its header");
     if (!kd && h.len[kArCloud] != sizeof(float) * uint64_t(h.n) * uint64_t(h.dim))
       arg_error("serialized brute-force arrays do not match its header");
+    if (h.params.heap != NABO_HIP_HEAP_LINEAR && h.params.heap != NABO_HIP_HEAP_STL)
+      arg_error("serialized index has an unknown heap type");
     int dev = device;
     if (dev < 0) NH_CHECK(hipGetDevice(&dev));
     DeviceScope scope(dev);

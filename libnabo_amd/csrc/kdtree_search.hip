@@ -29,6 +29,7 @@
 
 #include <atomic>
 #include <cstring>
+#include <type_traits>
 
 #include <rocprim/device/device_scan.hpp>
 
@@ -115,6 +116,8 @@ struct SearchParams {
   uint2* stack;
   int64_t stack_lanes;
   int allow_self;
+  int sort_results;  // IndexHeapSTL only: sort_heap before the output
+  int heap_lds;      // IndexHeapSTL in LDS: first uint2 of the heaps in LDS
 };
 
 // Sorted insertion of (value, index): entries greater than value move up by
@@ -136,6 +139,27 @@ __device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], fl
   }
   hi[0] = moved[0];
   hd[0] = fminf(hd[0], value);
+}
+
+// The k-best of a query: for K > 0, IndexHeapBruteForceVector as K sorted
+// entries in VGPRs (hd, hi); for K = 0 or -1, IndexHeapSTL as a heap in LDS
+// (K = 0) or in the query's output row (K = -1, when LDS is short). Both
+// admit the same distances (each drops one largest entry for the new one,
+// so their multisets of values, and their heads, are always equal): the
+// descent and its pruning are the same; only the entry kept among equal
+// distances and the output order differ.
+template <int K>
+constexpr int kArr = K > 0 ? K : 1;
+template <int K>
+using HeapOf = StlHeap<typename std::conditional<K == 0, LdsHeapStore, RowHeapStore>::type>;
+
+template <int K>
+__device__ __forceinline__ void best_insert(float (&hd)[kArr<K>], int32_t (&hi)[kArr<K>],
+                                            HeapOf<K>& heap, float value, int32_t index) {
+  if constexpr (K <= 0)
+    heap.replace_head(index, value);
+  else
+    heap_insert<K>(hd, hi, value, index);
 }
 
 // Admission tests of a bucket entry (kdtree_cpu.cpp:392-394) and of a far
@@ -197,8 +221,9 @@ __device__ __forceinline__ float point_dist(const SearchParams& a, const Vec<DIM
 // them in bucket order (kdtree_cpu.cpp:380-398).
 template <int DIM, int K, bool EXACT>
 __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>& q, uint32_t count,
-                                          int64_t start, const Admit<EXACT>& adm, float (&hd)[K],
-                                          int32_t (&hi)[K]) {
+                                          int64_t start, const Admit<EXACT>& adm,
+                                          float (&hd)[kArr<K>], int32_t (&hi)[kArr<K>],
+                                          HeapOf<K>& heap) {
   if constexpr (DIM > 0 && DIM <= 3) {
     const float4* pts4 = reinterpret_cast<const float4*>(a.pts) + start;
     for (uint32_t j0 = 0; j0 < count; j0 += kLeafBatch) {
@@ -217,8 +242,9 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
             const float diff = q.v[d] - c[d];
             dist = dist + diff * diff;
           }
-          if (adm.point(dist, hd[K - 1]) && (a.allow_self || dist > FLT_EPSILON))
-            heap_insert<K>(hd, hi, dist, __float_as_int(c[DIM]));
+          if (adm.point(dist, K <= 0 ? heap.head : hd[kArr<K> - 1]) &&
+              (a.allow_self || dist > FLT_EPSILON))
+            best_insert<K>(hd, hi, heap, dist, __float_as_int(c[DIM]));
         }
       }
     }
@@ -226,8 +252,9 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
     for (uint32_t j = 0; j < count; ++j) {
       int32_t index;
       const float dist = point_dist<DIM>(a, q, start + j, &index);
-      if (adm.point(dist, hd[K - 1]) && (a.allow_self || dist > FLT_EPSILON))
-        heap_insert<K>(hd, hi, dist, index);
+      if (adm.point(dist, K <= 0 ? heap.head : hd[kArr<K> - 1]) &&
+          (a.allow_self || dist > FLT_EPSILON))
+        best_insert<K>(hd, hi, heap, dist, index);
     }
   }
 }
@@ -310,12 +337,19 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
       q.v[d] = qp[d];
       off2.v[d] = 0.f;
     }
-    float hd[K];
-    int32_t hi[K];
+    float hd[kArr<K>];
+    int32_t hi[kArr<K>];
+    HeapOf<K> heap;
+    if constexpr (K == 0) {
+      heap.reset(LdsHeapStore{lds_stack + a.heap_lds + threadIdx.x}, a.k);
+    } else if constexpr (K < 0) {
+      heap.reset(RowHeapStore{a.out_idx + qi * a.k, a.out_d2 + qi * a.k}, a.k);
+    } else {
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-      hd[j] = j < K - a.k ? -INFINITY : INFINITY;
-      hi[j] = -1;
+      for (int j = 0; j < K; ++j) {
+        hd[j] = j < K - a.k ? -INFINITY : INFINITY;
+        hi[j] = -1;
+      }
     }
     float r2 = a.max_r2;
     if (a.radii) {
@@ -337,7 +371,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
       const float rd_far = rd + (-off2.get(cd) + new_off * new_off);
       const uint32_t right = w.x >> a.dim_bits;
       go_right = new_off > 0;
-      if (adm.node(rd_far, hd[K - 1])) {
+      if (adm.node(rd_far, K <= 0 ? heap.head : hd[kArr<K> - 1])) {
         slot(top) = make_uint2(go_right ? node + 1 : right, __float_as_uint(rd_far));
         top += sstride;
       }
@@ -362,7 +396,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
         rt = right_word(a.links, n);
       }
       const uint32_t count = nd.x >> a.dim_bits;
-      scan_leaf<DIM, K, EXACT>(a, q, count, int64_t(nd.y), adm, hd, hi);
+      scan_leaf<DIM, K, EXACT>(a, q, count, int64_t(nd.y), adm, hd, hi, heap);
       touched += count;
 
       // Pop: kPopBatch entries per round trip to the stack. Restore entries
@@ -384,7 +418,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
               off2.set(e[j].x & ~kRestore, __uint_as_float(e[j].y));
             } else {
               const float rdf = __uint_as_float(e[j].y);
-              if (adm.node(rdf, hd[K - 1])) {
+              if (adm.node(rdf, K <= 0 ? heap.head : hd[kArr<K> - 1])) {
                 descend = true;
                 n = e[j].x;
                 rd = rdf;
@@ -412,7 +446,10 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
     if (lane == 0) claimed = atomicAdd(&a.work[owner], 1u);
     int32_t* oi = a.out_idx + qi * a.k;
     float* od = a.out_d2 + qi * a.k;
-    if (vec_out) {
+    if constexpr (K <= 0) {
+      if (a.sort_results) heap.sort();
+      heap.write(oi, od);
+    } else if (vec_out) {
 #pragma unroll
       for (int j = 0; j < K / 4; ++j) {
         reinterpret_cast<int4*>(oi)[j] =
@@ -622,11 +659,13 @@ __global__ __launch_bounds__(kBlock) void k_sort_fine(const uint32_t* __restrict
 
 // Persistent grid: as many workgroups as the device holds at once (from
 // the occupancy of this instantiation), a multiple of the XCD count.
-// The occupancy is cached per device and LDS size (tree depth).
+// The occupancy is cached per device and LDS size (tree depth, heap size).
+constexpr int kLdsSlotsMax = 65536 / (kSearchBlock * sizeof(uint2));  // 64 KiB per workgroup
+
 template <int DIM, int K, bool LDS, bool EXACT>
 unsigned resident_grid(size_t lds, int cus, int dev) {
-  static std::atomic<int> cache[kMaxCachedDevices][kLdsStackSlots + 2];
-  const size_t slot = std::min<size_t>(lds / (kSearchBlock * sizeof(uint2)), kLdsStackSlots + 1);
+  static std::atomic<int> cache[kMaxCachedDevices][kLdsSlotsMax + 1];
+  const size_t slot = std::min<size_t>(lds / (kSearchBlock * sizeof(uint2)), kLdsSlotsMax);
   int per_cu = dev >= 0 && dev < kMaxCachedDevices ? cache[dev][slot].load() : 0;
   if (per_cu < 1) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kd_search<DIM, K, LDS, EXACT>,
@@ -638,16 +677,28 @@ unsigned resident_grid(size_t lds, int cus, int dev) {
   return unsigned(std::max(kXcds, (cus * per_cu) / kXcds * kXcds));
 }
 
+// LDS of one search workgroup: the stacks (depth slots per lane, when the
+// tree is at most kLdsStackSlots deep), then for K = 0 the STL heaps
+// (k slots per lane).
+template <int K>
+size_t search_lds(int depth, int k) {
+  const size_t stack = depth <= kLdsStackSlots ? size_t(depth) : 0;
+  const size_t heap = K == 0 ? size_t(k) : 0;
+  return (stack + heap) * kSearchBlock * sizeof(uint2);
+}
+
 template <int DIM, int K, bool EXACT>
 void launch_search_as(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& stack,
                       hipStream_t s) {
+  const size_t lds = search_lds<K>(depth, p.k);
   if (depth <= kLdsStackSlots) {
-    const size_t lds = size_t(depth) * kSearchBlock * sizeof(uint2);
+    p.heap_lds = depth * kSearchBlock;
     const unsigned grid = resident_grid<DIM, K, true, EXACT>(lds, cus, dev);
     KernelTimer timer(s);
     k_kd_search<DIM, K, true, EXACT><<<grid, kSearchBlock, lds, s>>>(p);
   } else {
-    const unsigned grid = resident_grid<DIM, K, false, EXACT>(0, cus, dev);
+    p.heap_lds = 0;
+    const unsigned grid = resident_grid<DIM, K, false, EXACT>(lds, cus, dev);
     const int64_t lanes = int64_t(grid) * kSearchBlock;
     if (uint64_t(lanes) * uint64_t(depth + 1) * sizeof(uint2) >= (uint64_t(1) << 32))
       throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
@@ -655,7 +706,7 @@ void launch_search_as(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>
     p.stack = stack.p;
     p.stack_lanes = lanes;
     KernelTimer timer(s);
-    k_kd_search<DIM, K, false, EXACT><<<grid, kSearchBlock, 0, s>>>(p);
+    k_kd_search<DIM, K, false, EXACT><<<grid, kSearchBlock, lds, s>>>(p);
   }
 }
 
@@ -671,10 +722,14 @@ void launch_search(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& s
 }
 
 template <int DIM>
-void dispatch_k(const SearchParams& p, int cus, int dev, int depth, DevBuf<uint2>& stack,
-                hipStream_t s) {
+void dispatch_k(const SearchParams& p, bool stl, int cus, int dev, int depth,
+                DevBuf<uint2>& stack, hipStream_t s) {
   const int k = p.k;
-  if (k <= 1)
+  if (stl && search_lds<0>(depth, k) <= size_t(kLdsSlotsMax) * kSearchBlock * sizeof(uint2))
+    launch_search<DIM, 0>(p, cus, dev, depth, stack, s);
+  else if (stl)
+    launch_search<DIM, -1>(p, cus, dev, depth, stack, s);
+  else if (k <= 1)
     launch_search<DIM, 1>(p, cus, dev, depth, stack, s);
   else if (k <= 2)
     launch_search<DIM, 2>(p, cus, dev, depth, stack, s);
@@ -733,6 +788,8 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   p.out_d2 = a.out_d2;
   p.touched = reinterpret_cast<unsigned long long*>(a.touched);
   p.allow_self = (a.flags & NABO_HIP_ALLOW_SELF_MATCH) ? 1 : 0;
+  p.sort_results = (a.flags & NABO_HIP_SORT_RESULTS) ? 1 : 0;
+  const bool stl = ix.params.heap == NABO_HIP_HEAP_STL;
   if (ix.clouds > 1) {
     if (!a.qcloud) throw Error(NABO_HIP_ERR_ARGUMENT, "batched trees need per-query cloud ids");
     p.roots = ix.d_roots;
@@ -829,11 +886,11 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   const int depth = std::max(1, ix.depth);
   DevBuf<uint2> stack;
   if (ix.dim == 3)
-    dispatch_k<3>(p, cus, dev, depth, stack, s);
+    dispatch_k<3>(p, stl, cus, dev, depth, stack, s);
   else if (ix.dim == 2)
-    dispatch_k<2>(p, cus, dev, depth, stack, s);
+    dispatch_k<2>(p, stl, cus, dev, depth, stack, s);
   else
-    dispatch_k<0>(p, cus, dev, depth, stack, s);
+    dispatch_k<0>(p, stl, cus, dev, depth, stack, s);
   NH_LAUNCH_CHECK();
 }
 

@@ -56,6 +56,20 @@ int main() {
       if (r) EXPECT(d1(r - 1, c) <= d1(r, c));
     }
 
+  // KDTREE_TREE_HEAP: IndexHeapSTL results, a max-heap unless SORT_RESULTS
+  NNSearchF* th = NNSearchF::create(M, 3, NNSearchF::KDTREE_TREE_HEAP);
+  Eigen::MatrixXi ti(5, m), tsi(5, m);
+  Eigen::MatrixXf td(5, m), tsd(5, m);
+  th->knn(q, ti, td, 5, 0, 0);
+  th->knn(q, tsi, tsd, 5, 0, NNSearchF::SORT_RESULTS);
+  for (int c = 0; c < m; ++c)
+    for (int r = 0; r < 5; ++r) {
+      EXPECT(tsd(r, c) == d1(r, c));
+      EXPECT(td(0, c) >= td(r, c));
+      if (r) EXPECT(td((r - 1) / 2, c) >= td(r, c));
+    }
+  delete th;
+
   // self match (examples/usage.cpp:37-57)
   Eigen::MatrixXi si(3, n);
   Eigen::MatrixXf sd(3, n);

@@ -62,3 +62,18 @@ def test_deserialize_rejects_bad_buffers_before_any_device_call():
     assert not h.value
     size = ctypes.c_uint64(0)
     assert lib.nabo_hip_serialize(None, None, 0, ctypes.byref(size)) == 1
+
+
+def test_heap_param_and_unknown_heap_rejected_before_any_device_call():
+    import numpy as np
+
+    assert nabo.make_params().heap == 0 and nabo.make_params(heap="stl").heap == 1
+    lib = nabo.load_library()
+    p = nabo.make_params()
+    p.heap = 5
+    cloud = np.zeros((10, 3), np.float32)
+    h = ctypes.c_void_p()
+    rc = lib.nabo_hip_create(cloud.ctypes.data_as(ctypes.c_void_p), 10, 3, 3, 3,
+                             int(nabo.SearchType.KDTREE_HIP), 0, ctypes.byref(p), ctypes.byref(h))
+    assert rc == 1 and b"Unknown heap type" in lib.nabo_hip_last_error()
+    assert not h.value
