@@ -319,6 +319,7 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kMfmaTile = 32;  // points (rows) and queries (columns) per MFMA
+constexpr int kShortBuf = 8;   // candidates buffered per lane before insertion (long lists)
 
 template <bool F16>
 struct Lp;
@@ -448,6 +449,16 @@ __global__ __launch_bounds__(256) void k_bf_shortlist(ShortlistParams a) {
     ld[j] = j < LCAP - a.l ? -INFINITY : INFINITY;
     li[j] = -1;
   }
+  // Candidate buffer of the long-list path (bv, bi; nb entries).
+  float bv[kShortBuf];
+  int32_t bi[kShortBuf];
+  int nb = 0;
+  auto flush = [&]() {
+#pragma unroll
+    for (int e = 0; e < kShortBuf; ++e)
+      if (e < nb && bv[e] < ld[LCAP - 1]) list_insert<LCAP>(ld, li, bv[e], bi[e]);
+    nb = 0;
+  };
   const T* pbase = static_cast<const T*>(a.points) + (lane & 31) * a.kp + 8 * half;
   for (int64_t t = p0; t < p1; t += kMfmaTile) {
     const T* prow = pbase + t * a.kp;
@@ -478,7 +489,10 @@ __global__ __launch_bounds__(256) void k_bf_shortlist(ShortlistParams a) {
       // Long lists: value r of every lane in row order, r uniform across
       // the wave (a relative register read: the select and its
       // temporaries would not fit beside the lists), r skipped when no lane
-      // has a candidate there.
+      // has a candidate there. Candidates wait in a small unsorted buffer
+      // and are inserted when some lane's buffer is full, all lanes at
+      // once: an O(LCAP) insertion then serves every lane that has one
+      // instead of the few whose candidate came up in that row.
 #pragma unroll 1
       for (int r = 0; r < 16; ++r) {
         const bool mine = (pending >> r) & 1u;
@@ -486,12 +500,19 @@ __global__ __launch_bounds__(256) void k_bf_shortlist(ShortlistParams a) {
           const float v = acc[r];
           if (mine && v < ld[LCAP - 1]) {
             const int32_t row = int32_t((r & 3) + 8 * (r >> 2) + 4 * half);
-            list_insert<LCAP>(ld, li, v, int32_t(t + row));
+#pragma unroll
+            for (int e = 0; e < kShortBuf; ++e) {
+              bv[e] = e == nb ? v : bv[e];
+              bi[e] = e == nb ? int32_t(t + row) : bi[e];
+            }
+            ++nb;
           }
+          if (__any(nb == kShortBuf)) flush();
         }
       }
     }
   }
+  if constexpr (LCAP > 16) flush();
   float* ov = a.part_v + ((int64_t(blockIdx.y) * a.m_pad + q) * 2 + half) * a.l;
   int32_t* oi = a.part_i + ((int64_t(blockIdx.y) * a.m_pad + q) * 2 + half) * a.l;
 #pragma unroll
@@ -719,9 +740,12 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
       k_lp_queries<false><<<gq, kBlock, 0, s>>>(a.m, sp.m_pad, ix.dim, sp.kp, a.query, a.qstride,
                                                  reinterpret_cast<__bf16*>(qlp.p));
     sp.queries = qlp.p;
-    // Chunks of points so that the grid has a few thousand workgroups.
+    // Chunks of points: each chunk fills its own lists from empty, which
+    // costs about l * (1 + ln(chunk / l)) insertions of O(l) each, so longer
+    // lists take fewer, longer chunks (measured on 200k x 50k, dim 33:
+    // l = 16 best at ~1k workgroups, l = 40 at one chunk).
     const int64_t qb = sp.m_pad / 128;
-    int64_t chunks_lp = std::max<int64_t>(1, ceil_div(2048, qb));
+    int64_t chunks_lp = std::max<int64_t>(1, ceil_div(l <= 16 ? 1024 : 256, qb));
     chunks_lp = std::min<int64_t>(chunks_lp, std::max<int64_t>(1, ix.lp_n_pad / 1024));
     sp.chunk = ceil_div(ceil_div(ix.lp_n_pad, chunks_lp), kMfmaTile) * kMfmaTile;
     p.chunks = int(ceil_div(ix.lp_n_pad, sp.chunk));
