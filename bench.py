@@ -52,6 +52,8 @@ def parse():
                     help="also run a 256 MiB device-to-device copy (counter calibration)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline line (no brute-force / batched / served sections)")
+    ap.add_argument("--extras", default="brute_force,batched,served",
+                    help="comma list of the sections to run after the headline")
     return ap.parse_args()
 
 
@@ -556,17 +558,24 @@ def main():
                      "device_cloud": None if build_ms_dev is None else round(build_ms_dev, 2)},
         "parity_vs_oracle": parity,
     }
-    if not args.no_extras:
+    extras = [] if args.no_extras else [e for e in args.extras.split(",") if e]
+    unknown = set(extras) - {"brute_force", "batched", "served"}
+    if unknown:
+        raise SystemExit(f"unknown --extras sections: {sorted(unknown)}")
+    if extras:
         nns.close()
         del d_query, d_idx, d_d2, d_cloud
         torch.cuda.empty_cache()
-        log("brute force (C3)")
+    if "brute_force" in extras:
+        log("brute force: 200k x 33-D points, 50k queries, k=8")
         out["brute_force"] = bench_brute_force(not args.no_cpu)
         torch.cuda.empty_cache()
-        log("batched clouds (C4)")
+    if "batched" in extras:
+        log("batched: 512 uneven clouds, k=8")
         out["batched"] = bench_batched()
         torch.cuda.empty_cache()
-        log("served stream (C5)")
+    if "served" in extras:
+        log("served: 64 resident clouds, Poisson request stream")
         out["served"] = bench_served()
     print(json.dumps(out), flush=True)
 

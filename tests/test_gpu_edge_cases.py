@@ -1,0 +1,96 @@
+"""Edge cases of the kd-tree search against the oracle, bit for bit.
+
+Queries outside the cloud's box (the root starts at rd = 0 with implicit
+bounds, kdtree_cpu.cpp:345-348), non-finite queries (NaN and infinite
+distances are never admitted, :392-394, and never open a far child,
+:415-417), a zero radius (dist <= 0 admits exact duplicates only), a cloud
+of one repeated point, k equal to the cloud size, and empty or one-query
+calls. Every case compares indices, dists2 and the touched count with the
+oracle on the same tree.
+"""
+import numpy as np
+import pytest
+
+from libnabo_amd import datasets
+
+pytestmark = pytest.mark.gpu
+
+SELF, SORT = 1, 2
+
+
+def check(nabo, oracle, cloud, query, k, eps=0.0, flags=0, radius=np.inf, bucket=8, heap="linear"):
+    st = nabo.SearchType.KDTREE_HIP if heap == "linear" else nabo.SearchType.KDTREE_TREE_HEAP
+    nns = nabo.NearestNeighbourSearch(cloud, st, cloud.shape[1],
+                                      nabo.CreationOptionFlags.TOUCH_STATISTICS,
+                                      {"bucketSize": bucket})
+    gi, gd = nns.knn(query, k, eps, flags, radius)
+    oi, od, touched, _ = oracle.kdtree(cloud, bucket_size=bucket).knn(
+        query, k, eps, flags, radius, heap=heap, threads=8, stats=True)
+    assert np.array_equal(gd.view(np.uint32), od.view(np.uint32)), "dists2 differ"
+    assert np.array_equal(gi, oi), "indices differ"
+    assert nns.last_touched == int(touched.sum())
+    return gi, gd
+
+
+def test_queries_outside_the_box(nabo, oracle):
+    cloud, query = datasets.uniform_cube(20_000, 2_000, seed=41)
+    rng = np.random.default_rng(41)
+    far = query + rng.choice([-1.0, 1.0], query.shape).astype(np.float32) * \
+        rng.uniform(0.5, 50.0, query.shape).astype(np.float32)
+    print("case outside the box", flush=True)
+    for heap in ("linear", "stl"):
+        check(nabo, oracle, cloud, far, 8, heap=heap)
+        check(nabo, oracle, cloud, far, 8, radius=2.0, heap=heap)
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf, 1e30])
+def test_non_finite_queries(nabo, oracle, bad):
+    cloud, query = datasets.uniform_cube(8_000, 1_000, seed=42)
+    query = query.copy()
+    query[::3, 0] = bad
+    query[1::7, 2] = bad
+    print(f"case non-finite query {bad}", flush=True)
+    gi, gd = check(nabo, oracle, cloud, query, 6)
+    hit = np.zeros(len(query), bool)
+    hit[::3] = True
+    hit[1::7] = True
+    # every coordinate difference with such a query is NaN or overflows
+    assert (gi[hit] == -1).all() and np.isinf(gd[hit]).all()
+    assert (gi[~hit] >= 0).all()
+    check(nabo, oracle, cloud, query, 6, eps=0.5, heap="stl")
+
+
+def test_zero_radius_admits_exact_duplicates(nabo, oracle):
+    cloud, _ = datasets.uniform_cube(5_000, 1, seed=43)
+    cloud = np.concatenate([cloud, cloud[:500]])  # 500 exact duplicates
+    print("case zero radius", flush=True)
+    gi, gd = check(nabo, oracle, cloud, cloud, 3, flags=SELF, radius=0.0)
+    assert (gd[:, 0] == 0).all()
+    assert ((gi[:500, :2] >= 0).all() and (gi[500:5000, 1] == -1).all())
+    check(nabo, oracle, cloud, cloud, 3, flags=0, radius=0.0)  # self match excluded: nothing
+
+
+def test_one_repeated_point(nabo, oracle):
+    cloud = np.tile(np.array([[1.5, -2.0, 0.25]], np.float32), (3_000, 1))
+    query = np.array([[1.5, -2.0, 0.25], [0.0, 0.0, 0.0], [9.0, 9.0, 9.0]], np.float32)
+    print("case one repeated point", flush=True)
+    check(nabo, oracle, cloud, query, 10, flags=SELF)
+    check(nabo, oracle, cloud, query, 10, flags=SELF | SORT, heap="stl")
+
+
+@pytest.mark.parametrize("n", [9, 64, 128])
+def test_k_equals_cloud_size(nabo, oracle, n):
+    cloud, query = datasets.uniform_cube(n, 200, seed=44)
+    print(f"case k = n = {n}", flush=True)
+    gi, gd = check(nabo, oracle, cloud, query, n, flags=SELF)
+    assert (np.sort(gi, axis=1) == np.arange(n)).all()
+    check(nabo, oracle, cloud, query, n, flags=SELF | SORT, heap="stl")
+
+
+def test_empty_and_single_query(nabo, oracle):
+    cloud, query = datasets.uniform_cube(4_000, 5, seed=45)
+    nns = nabo.NearestNeighbourSearch(cloud)
+    print("case empty / single query", flush=True)
+    gi, gd = nns.knn(np.empty((0, 3), np.float32), 4)
+    assert gi.shape == (0, 4) and gd.shape == (0, 4)
+    check(nabo, oracle, cloud, query[:1], 4)
