@@ -26,6 +26,8 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <mutex>
+#include <vector>
 
 #include "device_common.h"
 #include "internal.h"
@@ -640,6 +642,112 @@ __global__ void k_roots_out(int32_t clouds, const uint32_t* __restrict__ root_ba
 }
 
 inline unsigned grid_for(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, kBlock))); }
+
+// A device array carved from a BuildScratch block.
+template <class T>
+struct Span {
+  T* p = nullptr;
+  size_t count = 0;
+};
+
+// Device scratch of one tree build: a single block carved into 256-byte
+// aligned arrays. Blocks come from a per-device cache to which a build
+// returns its block once its stream has drained, so repeated builds make no
+// device allocation and no free (stream-ordered frees of the ~45 arrays a
+// build needs cost several milliseconds per build). The cache keeps at most
+// kScratchKeep blocks per device, each at most kScratchKeepBytes.
+constexpr int kScratchKeep = 2;
+constexpr size_t kScratchKeepBytes = size_t(1) << 30;
+
+struct CachedBlock {
+  int dev;
+  void* p;
+  size_t bytes;
+};
+std::mutex g_scratch_mu;
+std::vector<CachedBlock> g_scratch;
+
+class BuildScratch {
+ public:
+  explicit BuildScratch(hipStream_t s) : s_(s) {}
+  BuildScratch(const BuildScratch&) = delete;
+  BuildScratch& operator=(const BuildScratch&) = delete;
+
+  template <class T>
+  void want(Span<T>& span, size_t count) {
+    span.count = count;
+    reqs_.push_back({reinterpret_cast<void**>(&span.p), (count * sizeof(T) + 255) / 256 * 256});
+  }
+  // Takes a block that holds every array asked for and points the spans into it.
+  void commit(int dev) {
+    size_t total = 256;
+    for (const Req& r : reqs_) total += r.bytes;
+    {
+      std::lock_guard<std::mutex> lock(g_scratch_mu);
+      int best = -1;
+      for (int i = 0; i < int(g_scratch.size()); ++i)
+        if (g_scratch[i].dev == dev && g_scratch[i].bytes >= total &&
+            (best < 0 || g_scratch[i].bytes < g_scratch[best].bytes))
+          best = i;
+      if (best >= 0) {
+        block_ = g_scratch[best].p;
+        bytes_ = g_scratch[best].bytes;
+        g_scratch.erase(g_scratch.begin() + best);
+      }
+    }
+    if (!block_) {
+      const hipError_t e = hipMalloc(&block_, total);
+      if (e != hipSuccess) {
+        block_ = nullptr;
+        throw Error(NABO_HIP_ERR_MEMORY, "device allocation of " + std::to_string(total) +
+                                             " bytes failed: " + hipGetErrorString(e));
+      }
+      bytes_ = total;
+    }
+    dev_ = dev;
+    char* cur = static_cast<char*>(block_);
+    for (const Req& r : reqs_) {
+      *r.slot = cur;
+      cur += r.bytes;
+    }
+  }
+  ~BuildScratch() {
+    if (!block_) return;
+    // The block goes back only once no kernel of this build can touch it.
+    if (hipStreamSynchronize(s_) != hipSuccess || bytes_ > kScratchKeepBytes) {
+      (void)hipFree(block_);
+      return;
+    }
+    std::vector<void*> drop;
+    {
+      std::lock_guard<std::mutex> lock(g_scratch_mu);
+      g_scratch.push_back({dev_, block_, bytes_});
+      for (;;) {
+        int held = 0, smallest = -1;
+        for (int i = 0; i < int(g_scratch.size()); ++i) {
+          if (g_scratch[i].dev != dev_) continue;
+          ++held;
+          if (smallest < 0 || g_scratch[i].bytes < g_scratch[smallest].bytes) smallest = i;
+        }
+        if (held <= kScratchKeep) break;
+        drop.push_back(g_scratch[smallest].p);
+        g_scratch.erase(g_scratch.begin() + smallest);
+      }
+    }
+    for (void* p : drop) (void)hipFree(p);
+  }
+
+ private:
+  struct Req {
+    void** slot;
+    size_t bytes;
+  };
+  hipStream_t s_;
+  int dev_ = -1;
+  void* block_ = nullptr;
+  size_t bytes_ = 0;
+  std::vector<Req> reqs_;
+};
 inline unsigned tiles_for(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, kTile))); }
 
 }  // namespace
@@ -658,33 +766,52 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
   float* pts = ix.d_pts;
 
   const int64_t max_nodes = 2 * n;
-  // Level buffers (double-buffered), per-segment stats, BFS node records.
-  DevBuf<int32_t> seg_start[2], seg_count[2];
-  DevBuf<float> seg_lo[2], seg_hi[2];
+  // Level buffers (double-buffered), per-segment stats, BFS node records,
+  // per-point arrays: one scratch block for the whole build.
+  BuildScratch scratch(s);
+  Span<int32_t> seg_start[2], seg_count[2];
+  Span<float> seg_lo[2], seg_hi[2];
   const int64_t seg_cap = std::max<int64_t>(n, clouds);
   for (int b = 0; b < 2; ++b) {
-    seg_start[b].alloc(size_t(seg_cap), s);
-    seg_count[b].alloc(size_t(seg_cap), s);
-    seg_lo[b].alloc(size_t(seg_cap) * dim, s);
-    seg_hi[b].alloc(size_t(seg_cap) * dim, s);
+    scratch.want(seg_start[b], size_t(seg_cap));
+    scratch.want(seg_count[b], size_t(seg_cap));
+    scratch.want(seg_lo[b], size_t(seg_cap) * dim);
+    scratch.want(seg_hi[b], size_t(seg_cap) * dim);
   }
-  DevBuf<uint32_t> st_cd(seg_cap, s), st_minkey(seg_cap, s), st_maxkey(seg_cap, s),
-      st_lt(seg_cap, s), st_eq(seg_cap, s), st_split(seg_cap, s), st_childbase(seg_cap, s);
-  DevBuf<float> st_ideal(seg_cap, s), st_cut(seg_cap, s);
-  DevBuf<int32_t> st_left(seg_cap, s);
-  DevBuf<uint32_t> b_cd(max_nodes, s), b_aux(max_nodes, s), b_count(max_nodes, s),
-      b_size(max_nodes, s), b_pre(max_nodes, s);
-  DevBuf<int32_t> b_left(max_nodes, s), b_right(max_nodes, s);
+  Span<uint32_t> st_cd, st_minkey, st_maxkey, st_lt, st_eq, st_split, st_childbase;
+  Span<float> st_ideal, st_cut;
+  Span<int32_t> st_left;
+  for (Span<uint32_t>* a : {&st_cd, &st_minkey, &st_maxkey, &st_lt, &st_eq, &st_split,
+                            &st_childbase})
+    scratch.want(*a, size_t(seg_cap));
+  scratch.want(st_ideal, size_t(seg_cap));
+  scratch.want(st_cut, size_t(seg_cap));
+  scratch.want(st_left, size_t(seg_cap));
+  Span<uint32_t> b_cd, b_aux, b_count, b_size, b_pre;
+  Span<int32_t> b_left, b_right;
+  for (Span<uint32_t>* a : {&b_cd, &b_aux, &b_count, &b_size, &b_pre})
+    scratch.want(*a, size_t(max_nodes));
+  scratch.want(b_left, size_t(max_nodes));
+  scratch.want(b_right, size_t(max_nodes));
 
-  DevBuf<int32_t> perm(n, s), perm2(n, s), pseg(n, s), slot_l(n, s), slot_r(n, s), cid(n, s);
-  DevBuf<float> xs(n, s), xs2(n, s);
-  DevBuf<uint64_t> flags(n, s), excl(n, s);
-  DevBuf<uint32_t> keys(size_t(2) * clouds * dim, s), root_base(size_t(clouds), s);
-  DevBuf<float> bounds(size_t(2) * clouds * dim, s);
-  DevBuf<int32_t> d_total(1, s);
-  DevBuf<int64_t> d_offsets(offsets.size(), s);
-  NH_CHECK(hipMemcpyAsync(d_offsets.p, offsets.data(), sizeof(int64_t) * offsets.size(),
-                          hipMemcpyHostToDevice, s));
+  Span<int32_t> perm, perm2, pseg, slot_l, slot_r, cid;
+  for (Span<int32_t>* a : {&perm, &perm2, &pseg, &slot_l, &slot_r, &cid})
+    scratch.want(*a, size_t(n));
+  Span<float> xs, xs2;
+  scratch.want(xs, size_t(n));
+  scratch.want(xs2, size_t(n));
+  Span<uint64_t> flags, excl;
+  scratch.want(flags, size_t(n));
+  scratch.want(excl, size_t(n));
+  Span<uint32_t> keys, root_base;
+  scratch.want(keys, size_t(2) * clouds * dim);
+  scratch.want(root_base, size_t(clouds));
+  Span<float> bounds;
+  scratch.want(bounds, size_t(2) * clouds * dim);
+  Span<int32_t> d_total;
+  scratch.want(d_total, 1);
+  Span<int64_t> d_offsets;
+  scratch.want(d_offsets, offsets.size());
 
   // Scan scratch, sized for the largest scan (n uint64).
   size_t scan_bytes64 = 0, scan_bytes32 = 0;
@@ -692,11 +819,20 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
                                    rocprim::plus<uint64_t>(), s));
   NH_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes32, st_split.p, st_childbase.p, uint32_t(0),
                                    size_t(seg_cap), rocprim::plus<uint32_t>(), s));
-  DevBuf<uint8_t> scan_tmp(std::max(scan_bytes64, scan_bytes32) + 16, s);
+  Span<uint8_t> scan_tmp;
+  scratch.want(scan_tmp, std::max(scan_bytes64, scan_bytes32) + 16);
   // Tile partials of the reductions: 2 per tile, dim groups for the bounds.
   const int64_t nparts = 2 * int64_t(tiles_for(n));
-  DevBuf<int32_t> part_seg(size_t(nparts) * dim, s);
-  DevBuf<uint32_t> part_a(size_t(nparts) * dim, s), part_b(size_t(nparts) * dim, s);
+  Span<int32_t> part_seg;
+  Span<uint32_t> part_a, part_b;
+  scratch.want(part_seg, size_t(nparts) * dim);
+  scratch.want(part_a, size_t(nparts) * dim);
+  scratch.want(part_b, size_t(nparts) * dim);
+  int dev = 0;
+  NH_CHECK(hipGetDevice(&dev));
+  scratch.commit(dev);
+  NH_CHECK(hipMemcpyAsync(d_offsets.p, offsets.data(), sizeof(int64_t) * offsets.size(),
+                          hipMemcpyHostToDevice, s));
   const TileParts parts{part_seg.p, part_a.p, part_b.p};
   const unsigned gpart = unsigned(ceil_div(nparts, kTile));
 
