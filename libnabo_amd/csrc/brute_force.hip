@@ -779,8 +779,12 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     NH_LAUNCH_CHECK();
     return;
   }
-  DevBuf<float> pd(size_t(p.chunks) * a.m * a.k, s);
-  DevBuf<int32_t> pi(size_t(p.chunks) * a.m * a.k, s);
+  Scratch scratch(s);
+  Span<float> pd;
+  Span<int32_t> pi;
+  scratch.want(pd, size_t(p.chunks) * a.m * a.k);
+  scratch.want(pi, size_t(p.chunks) * a.m * a.k);
+  scratch.commit(ix.device);
   p.part_d = pd.p;
   p.part_i = pi.p;
   KernelTimer timer(s);

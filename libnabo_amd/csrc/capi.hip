@@ -287,6 +287,91 @@ void run_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
 
 }  // namespace
 
+namespace {
+std::mutex g_scratch_mu;
+std::vector<Scratch::Block> g_scratch;
+}  // namespace
+
+void Scratch::commit(int dev) {
+  size_t total = 256;
+  for (const Req& r : reqs_) total += r.bytes;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    int best = -1;
+    for (int i = 0; i < int(g_scratch.size()); ++i) {
+      const Block& b = g_scratch[i];
+      if (b.dev != dev || b.bytes < total) continue;
+      if (best < 0) {
+        best = i;
+        continue;
+      }
+      const Block& c = g_scratch[best];
+      // same stream first (no wait), then the smallest
+      const bool b_same = b.last == s_, c_same = c.last == s_;
+      if (b_same != c_same ? b_same : b.bytes < c.bytes) best = i;
+    }
+    if (best >= 0) {
+      block_ = g_scratch[best];
+      g_scratch.erase(g_scratch.begin() + best);
+    }
+  }
+  if (block_.p) {
+    if (block_.last != s_) NH_CHECK(hipStreamWaitEvent(s_, block_.done, 0));
+  } else {
+    const size_t bytes = (total + (size_t(2) << 20) - 1) / (size_t(2) << 20) * (size_t(2) << 20);
+    const hipError_t e = hipMallocAsync(&block_.p, bytes, s_);
+    if (e != hipSuccess) {
+      block_.p = nullptr;
+      throw Error(NABO_HIP_ERR_MEMORY, "device allocation of " + std::to_string(bytes) +
+                                           " bytes failed: " + hipGetErrorString(e));
+    }
+    block_.dev = dev;
+    block_.bytes = bytes;
+    if (hipEventCreateWithFlags(&block_.done, hipEventDisableTiming) != hipSuccess) {
+      (void)hipFreeAsync(block_.p, s_);
+      block_.p = nullptr;
+      throw Error(NABO_HIP_ERR_HIP, "hipEventCreateWithFlags failed");
+    }
+  }
+  char* cur = static_cast<char*>(block_.p);
+  for (const Req& r : reqs_) {
+    *r.slot = cur;
+    cur += r.bytes;
+  }
+}
+
+Scratch::~Scratch() {
+  if (!block_.p) return;
+  auto drop = [this](const Block& b) {
+    // stream-ordered after the block's last use, on this call's stream
+    if (b.last != s_) (void)hipStreamWaitEvent(s_, b.done, 0);
+    (void)hipFreeAsync(b.p, s_);
+    (void)hipEventDestroy(b.done);
+  };
+  block_.last = s_;
+  if (hipEventRecord(block_.done, s_) != hipSuccess || block_.bytes > kScratchKeepBytes) {
+    drop(block_);
+    return;
+  }
+  std::vector<Block> dropped;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    g_scratch.push_back(block_);
+    for (;;) {
+      int held = 0, smallest = -1;
+      for (int i = 0; i < int(g_scratch.size()); ++i) {
+        if (g_scratch[i].dev != block_.dev) continue;
+        ++held;
+        if (smallest < 0 || g_scratch[i].bytes < g_scratch[smallest].bytes) smallest = i;
+      }
+      if (held <= kScratchKeep) break;
+      dropped.push_back(g_scratch[smallest]);
+      g_scratch.erase(g_scratch.begin() + smallest);
+    }
+  }
+  for (const Block& b : dropped) drop(b);
+}
+
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 void throw_hip(hipError_t e, const char* what, const char* file, int line) {
@@ -501,9 +586,16 @@ int nabo_hip_knn(const nabo_hip_index* index, const float* query, int32_t qrows,
     DeviceScope scope(ix.device);
     hipStream_t s = thread_stream(ix.device);
     const int64_t qfloats = qstride * (m - 1) + qrows;
-    DevBuf<float> dq(size_t(qfloats), s), dr(max_radii ? size_t(m) : 0, s), dd(size_t(m) * k, s);
-    DevBuf<int32_t> di(size_t(m) * k, s);
-    DevBuf<uint64_t> dt(1, s);
+    Scratch scratch(s);
+    Span<float> dq, dr, dd;
+    Span<int32_t> di;
+    Span<uint64_t> dt;
+    scratch.want(dq, size_t(qfloats));
+    scratch.want(dr, max_radii ? size_t(m) : 0);
+    scratch.want(dd, size_t(m) * k);
+    scratch.want(di, size_t(m) * k);
+    scratch.want(dt, 1);
+    scratch.commit(ix.device);
     NH_CHECK(hipMemcpyAsync(dq.p, query, sizeof(float) * size_t(qfloats), hipMemcpyHostToDevice, s));
     if (max_radii)
       NH_CHECK(hipMemcpyAsync(dr.p, max_radii, sizeof(float) * size_t(m), hipMemcpyHostToDevice, s));

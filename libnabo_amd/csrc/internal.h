@@ -60,6 +60,58 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
+// A device array carved from a Scratch block.
+template <class T>
+struct Span {
+  T* p = nullptr;
+  size_t count = 0;
+};
+
+// Device scratch of one call (a build, a knn call): one block carved into
+// 256-byte aligned arrays. Blocks come from a per-device pool; a call gives
+// its block back with an event recorded on its stream, and the next call
+// that takes it on another stream first makes its stream wait for that
+// event, so reuse is stream-ordered and nothing waits on the host. Calls
+// thus make no device allocation once the pool holds blocks of their size
+// (stream-ordered frees of a call's dozens of arrays cost up to
+// milliseconds each). The pool keeps at most kScratchKeep blocks per device,
+// each at most kScratchKeepBytes; larger ones are freed after the call.
+constexpr int kScratchKeep = 4;
+constexpr size_t kScratchKeepBytes = size_t(1) << 30;
+
+class Scratch {
+ public:
+  explicit Scratch(hipStream_t s) : s_(s) {}
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+  ~Scratch();
+
+  template <class T>
+  void want(Span<T>& span, size_t count) {
+    span.count = count;
+    reqs_.push_back({reinterpret_cast<void**>(&span.p), (count * sizeof(T) + 255) / 256 * 256});
+  }
+  // Takes a block holding every array asked for and points the spans into it.
+  void commit(int dev);
+
+  struct Block {
+    int dev;
+    void* p;
+    size_t bytes;
+    hipStream_t last;  // stream of the last call that used it
+    hipEvent_t done;   // recorded after that call's work
+  };
+
+ private:
+  struct Req {
+    void** slot;
+    size_t bytes;
+  };
+  hipStream_t s_;
+  Block block_{-1, nullptr, 0, nullptr, nullptr};
+  std::vector<Req> reqs_;
+};
+
 // A built search structure resident in HBM.
 struct Index {
   int search_type = NABO_HIP_SEARCH_KDTREE;

@@ -643,111 +643,6 @@ __global__ void k_roots_out(int32_t clouds, const uint32_t* __restrict__ root_ba
 
 inline unsigned grid_for(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, kBlock))); }
 
-// A device array carved from a BuildScratch block.
-template <class T>
-struct Span {
-  T* p = nullptr;
-  size_t count = 0;
-};
-
-// Device scratch of one tree build: a single block carved into 256-byte
-// aligned arrays. Blocks come from a per-device cache to which a build
-// returns its block once its stream has drained, so repeated builds make no
-// device allocation and no free (stream-ordered frees of the ~45 arrays a
-// build needs cost several milliseconds per build). The cache keeps at most
-// kScratchKeep blocks per device, each at most kScratchKeepBytes.
-constexpr int kScratchKeep = 2;
-constexpr size_t kScratchKeepBytes = size_t(1) << 30;
-
-struct CachedBlock {
-  int dev;
-  void* p;
-  size_t bytes;
-};
-std::mutex g_scratch_mu;
-std::vector<CachedBlock> g_scratch;
-
-class BuildScratch {
- public:
-  explicit BuildScratch(hipStream_t s) : s_(s) {}
-  BuildScratch(const BuildScratch&) = delete;
-  BuildScratch& operator=(const BuildScratch&) = delete;
-
-  template <class T>
-  void want(Span<T>& span, size_t count) {
-    span.count = count;
-    reqs_.push_back({reinterpret_cast<void**>(&span.p), (count * sizeof(T) + 255) / 256 * 256});
-  }
-  // Takes a block that holds every array asked for and points the spans into it.
-  void commit(int dev) {
-    size_t total = 256;
-    for (const Req& r : reqs_) total += r.bytes;
-    {
-      std::lock_guard<std::mutex> lock(g_scratch_mu);
-      int best = -1;
-      for (int i = 0; i < int(g_scratch.size()); ++i)
-        if (g_scratch[i].dev == dev && g_scratch[i].bytes >= total &&
-            (best < 0 || g_scratch[i].bytes < g_scratch[best].bytes))
-          best = i;
-      if (best >= 0) {
-        block_ = g_scratch[best].p;
-        bytes_ = g_scratch[best].bytes;
-        g_scratch.erase(g_scratch.begin() + best);
-      }
-    }
-    if (!block_) {
-      const hipError_t e = hipMalloc(&block_, total);
-      if (e != hipSuccess) {
-        block_ = nullptr;
-        throw Error(NABO_HIP_ERR_MEMORY, "device allocation of " + std::to_string(total) +
-                                             " bytes failed: " + hipGetErrorString(e));
-      }
-      bytes_ = total;
-    }
-    dev_ = dev;
-    char* cur = static_cast<char*>(block_);
-    for (const Req& r : reqs_) {
-      *r.slot = cur;
-      cur += r.bytes;
-    }
-  }
-  ~BuildScratch() {
-    if (!block_) return;
-    // The block goes back only once no kernel of this build can touch it.
-    if (hipStreamSynchronize(s_) != hipSuccess || bytes_ > kScratchKeepBytes) {
-      (void)hipFree(block_);
-      return;
-    }
-    std::vector<void*> drop;
-    {
-      std::lock_guard<std::mutex> lock(g_scratch_mu);
-      g_scratch.push_back({dev_, block_, bytes_});
-      for (;;) {
-        int held = 0, smallest = -1;
-        for (int i = 0; i < int(g_scratch.size()); ++i) {
-          if (g_scratch[i].dev != dev_) continue;
-          ++held;
-          if (smallest < 0 || g_scratch[i].bytes < g_scratch[smallest].bytes) smallest = i;
-        }
-        if (held <= kScratchKeep) break;
-        drop.push_back(g_scratch[smallest].p);
-        g_scratch.erase(g_scratch.begin() + smallest);
-      }
-    }
-    for (void* p : drop) (void)hipFree(p);
-  }
-
- private:
-  struct Req {
-    void** slot;
-    size_t bytes;
-  };
-  hipStream_t s_;
-  int dev_ = -1;
-  void* block_ = nullptr;
-  size_t bytes_ = 0;
-  std::vector<Req> reqs_;
-};
 inline unsigned tiles_for(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, kTile))); }
 
 }  // namespace
@@ -768,7 +663,7 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
   const int64_t max_nodes = 2 * n;
   // Level buffers (double-buffered), per-segment stats, BFS node records,
   // per-point arrays: one scratch block for the whole build.
-  BuildScratch scratch(s);
+  Scratch scratch(s);
   Span<int32_t> seg_start[2], seg_count[2];
   Span<float> seg_lo[2], seg_hi[2];
   const int64_t seg_cap = std::max<int64_t>(n, clouds);

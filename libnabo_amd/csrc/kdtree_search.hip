@@ -803,22 +803,54 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   // start of the leaf the greedy descent reaches, cut to 16 bits (bucket
   // starts are global, so this also groups the queries of batched trees by
   // cloud).
-  DevBuf<uint32_t> counters(kXcds, s), keys, counts, starts, fkeys;
-  DevBuf<int32_t> fidx, order;
-  DevBuf<uint8_t> scan_tmp;
+  // Scratch of the call: chunk counters, then the sort's arrays, sized
+  // before anything is launched.
+  const bool sort = ix.params.query_sort && a.m > 1 && ix.node_count > 1;
+  const bool morton = ix.clouds == 1 && (ix.dim == 2 || ix.dim == 3);
+  int key_bits = 0, axis_bits = 0;
+  uint32_t locate_shift = 0;
+  if (sort && morton) {
+    const int max_bits = ix.dim == 3 ? 7 : 10;
+    axis_bits = std::max(1, std::min(max_bits, (int(bits_needed(uint64_t(a.m))) + 2) / ix.dim));
+    key_bits = axis_bits * ix.dim;
+  } else if (sort) {
+    const uint32_t full_bits = std::max(1u, bits_needed(uint64_t(ix.n)));
+    locate_shift = full_bits > 16 ? full_bits - 16 : 0;
+    key_bits = int(full_bits - locate_shift);
+  }
+  const int coarse_bits = std::min(kCoarseBits, key_bits);
+  const int fine_bits = std::min(kFineBitsMax, key_bits - coarse_bits);
+  const int coarse_bins = 1 << coarse_bits;
+  const int shift_fine = key_bits - coarse_bits;  // key bits below the coarse ones
+  const int64_t tiles = ceil_div(a.m, kSortTile);
+  const int64_t ncounts = int64_t(coarse_bins) * tiles;
+
+  Scratch scratch(s);
+  Span<uint32_t> counters, keys, counts, starts, fkeys;
+  Span<int32_t> fidx, order;
+  Span<uint8_t> scan_tmp;
+  scratch.want(counters, kXcds);
+  size_t scan_bytes = 0;
+  if (sort) {
+    scratch.want(keys, size_t(a.m));
+    scratch.want(counts, size_t(ncounts));
+    scratch.want(starts, size_t(ncounts));
+    scratch.want(fkeys, size_t(a.m));
+    scratch.want(fidx, size_t(a.m));
+    scratch.want(order, size_t(a.m));
+    NH_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes, counts.p, starts.p, uint32_t(0),
+                                     size_t(ncounts), rocprim::plus<uint32_t>(), s));
+    scratch.want(scan_tmp, scan_bytes + 16);
+  }
+  int dev = 0;
+  NH_CHECK(hipGetDevice(&dev));
+  scratch.commit(dev);
+
   k_zero_counters<<<1, kBlock, 0, s>>>(counters.p, kXcds);
   p.work = counters.p;
-  const bool sort = ix.params.query_sort && a.m > 1 && ix.node_count > 1;
   if (sort) {
-    const bool morton = ix.clouds == 1 && (ix.dim == 2 || ix.dim == 3);
     const unsigned g = unsigned(ceil_div(a.m, kBlock));
-    keys.alloc(size_t(a.m), s);
-    int key_bits = 0;
     if (morton) {
-      const int max_bits = ix.dim == 3 ? 7 : 10;
-      const int axis_bits =
-          std::max(1, std::min(max_bits, (int(bits_needed(uint64_t(a.m))) + 2) / ix.dim));
-      key_bits = axis_bits * ix.dim;
       float lo[3] = {0.f, 0.f, 0.f}, sc[3] = {0.f, 0.f, 0.f};
       const float top = float((1u << axis_bits) - 1u);
       for (int d = 0; d < ix.dim; ++d) {
@@ -833,43 +865,29 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
         k_morton<2><<<g, kBlock, 0, s>>>(a.query, a.qstride, a.m, axis_bits, lo[0], lo[1], lo[2],
                                          sc[0], sc[1], sc[2], keys.p);
     } else {
-      const uint32_t full_bits = std::max(1u, bits_needed(uint64_t(ix.n)));
-      const uint32_t shift = full_bits > 16 ? full_bits - 16 : 0;
-      key_bits = int(full_bits - shift);
       if (ix.dim == 3)
         k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
-                                         a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p);
+                                         a.query, a.qstride, a.m, locate_shift, p.roots, p.qcloud,
+                                         keys.p);
       else if (ix.dim == 2)
         k_locate<2><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
-                                         a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p);
+                                         a.query, a.qstride, a.m, locate_shift, p.roots, p.qcloud,
+                                         keys.p);
       else
         k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
-                                         a.query, a.qstride, a.m, shift, p.roots, p.qcloud, keys.p);
+                                         a.query, a.qstride, a.m, locate_shift, p.roots, p.qcloud,
+                                         keys.p);
     }
     NH_LAUNCH_CHECK();
-    const int coarse_bits = std::min(kCoarseBits, key_bits);
-    const int fine_bits = std::min(kFineBitsMax, key_bits - coarse_bits);
-    const int coarse_bins = 1 << coarse_bits;
-    const int shift_fine = key_bits - coarse_bits;  // key bits below the coarse ones
-    const int64_t tiles = ceil_div(a.m, kSortTile);
-    const int64_t ncounts = int64_t(coarse_bins) * tiles;
     if (shift_fine > fine_bits) {
       // Drop key bits below the fine ones (only when the key is wider than
       // coarse + fine bits).
       k_shift_keys<<<g, kBlock, 0, s>>>(keys.p, a.m, uint32_t(shift_fine - fine_bits));
     }
-    counts.alloc(size_t(ncounts), s);
-    starts.alloc(size_t(ncounts), s);
-    fkeys.alloc(size_t(a.m), s);
-    fidx.alloc(size_t(a.m), s);
-    order.alloc(size_t(a.m), s);
     k_sort_hist<<<unsigned(tiles), kBlock, 0, s>>>(keys.p, a.m, fine_bits, coarse_bins, tiles,
                                                     counts.p);
     NH_LAUNCH_CHECK();
-    size_t bytes = 0;
-    NH_CHECK(rocprim::exclusive_scan(nullptr, bytes, counts.p, starts.p, uint32_t(0),
-                                     size_t(ncounts), rocprim::plus<uint32_t>(), s));
-    scan_tmp.alloc(bytes + 16, s);
+    size_t bytes = scan_bytes;
     NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, counts.p, starts.p, uint32_t(0),
                                      size_t(ncounts), rocprim::plus<uint32_t>(), s));
     k_sort_scatter<<<unsigned(tiles), kBlock, 0, s>>>(keys.p, a.m, fine_bits, coarse_bins, tiles,
@@ -880,8 +898,7 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     p.order = order.p;
   }
 
-  int dev = 0, cus = 256;
-  NH_CHECK(hipGetDevice(&dev));
+  int cus = 256;
   NH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int depth = std::max(1, ix.depth);
   DevBuf<uint2> stack;
