@@ -12,6 +12,11 @@
  *
  * Every function returns NABO_HIP_OK or an error code; the message of the
  * last error on the calling thread is nabo_hip_last_error().
+ *
+ * Device memory: besides each index's own arrays, builds and knn calls take
+ * their temporaries from a per-device pool of scratch blocks that the
+ * library keeps between calls (at most eight blocks of at most 1 GiB per
+ * device, until the process ends).
  */
 #ifndef NABO_HIP_H
 #define NABO_HIP_H

@@ -266,11 +266,17 @@ __global__ void k_query_clouds(int64_t m, int32_t clouds, const int64_t* __restr
 // their scratch stream-ordered) made illegal memory accesses on the GPU,
 // intermittently, after other captured graphs had run in the process; the
 // cause was not found.
-void run_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
+// knn calls allocate from the scratch pool and query its events, which a
+// stream capture forbids: they refuse capture before doing anything.
+void refuse_capture(hipStream_t s) {
   hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
   NH_CHECK(hipStreamIsCapturing(s, &capture));
   if (capture != hipStreamCaptureStatusNone)
     throw Error(NABO_HIP_ERR_UNSUPPORTED, "knn calls cannot be captured in a HIP graph");
+}
+
+void run_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
+  refuse_capture(s);
   if (a.touched) {
     k_set_u64<<<1, 1, 0, s>>>(a.touched, 0);
     NH_LAUNCH_CHECK();
@@ -297,16 +303,20 @@ void Scratch::commit(int dev) {
   for (const Req& r : reqs_) total += r.bytes;
   {
     std::lock_guard<std::mutex> lock(g_scratch_mu);
+    // A block last used on this stream is reused at once (stream order); a
+    // block of another stream only once its work is done, so that calls on
+    // different streams never wait for each other through the pool.
     int best = -1;
     for (int i = 0; i < int(g_scratch.size()); ++i) {
       const Block& b = g_scratch[i];
       if (b.dev != dev || b.bytes < total) continue;
+      if (b.last != s_ && hipEventQuery(b.done) != hipSuccess) continue;
       if (best < 0) {
         best = i;
         continue;
       }
       const Block& c = g_scratch[best];
-      // same stream first (no wait), then the smallest
+      // same stream first, then the smallest
       const bool b_same = b.last == s_, c_same = c.last == s_;
       if (b_same != c_same ? b_same : b.bytes < c.bytes) best = i;
     }
@@ -318,7 +328,9 @@ void Scratch::commit(int dev) {
   if (block_.p) {
     if (block_.last != s_) NH_CHECK(hipStreamWaitEvent(s_, block_.done, 0));
   } else {
-    const size_t bytes = (total + (size_t(2) << 20) - 1) / (size_t(2) << 20) * (size_t(2) << 20);
+    // Power-of-two sizes from 2 MiB: blocks then fit calls of similar sizes.
+    size_t bytes = size_t(2) << 20;
+    while (bytes < total) bytes <<= 1;
     const hipError_t e = hipMallocAsync(&block_.p, bytes, s_);
     if (e != hipSuccess) {
       block_.p = nullptr;
@@ -486,7 +498,10 @@ int nabo_hip_create(const float* cloud, int64_t n, int32_t rows, int64_t stride,
     prepare_device(dev);
     hipStream_t s = thread_stream(dev);
     const int64_t floats = stride * (n - 1) + rows;
-    DevBuf<float> d_cloud(size_t(floats), s);
+    Scratch scratch(s);
+    Span<float> d_cloud;
+    scratch.want(d_cloud, size_t(floats));
+    scratch.commit(dev);
     NH_CHECK(hipMemcpyAsync(d_cloud.p, cloud, sizeof(float) * size_t(floats),
                             hipMemcpyHostToDevice, s));
     auto* h = new nabo_hip_index();
@@ -681,9 +696,12 @@ int nabo_hip_batch_knn(const nabo_hip_index* index, const float* d_query, int32_
       arg_error("query, offsets or output pointer is null");
     DeviceScope scope(ix.device);
     hipStream_t s = static_cast<hipStream_t>(hip_stream);
-    DevBuf<int32_t> qcloud;
+    refuse_capture(s);
+    Scratch scratch(s);
+    Span<int32_t> qcloud;
     if (ix.clouds > 1 && m > 0) {
-      qcloud.alloc(size_t(m), s);
+      scratch.want(qcloud, size_t(m));
+      scratch.commit(ix.device);
       k_query_clouds<<<unsigned(ceil_div(m, 256)), 256, 0, s>>>(m, ix.clouds, d_query_offsets,
                                                                  qcloud.p);
       NH_LAUNCH_CHECK();

@@ -69,14 +69,16 @@ struct Span {
 
 // Device scratch of one call (a build, a knn call): one block carved into
 // 256-byte aligned arrays. Blocks come from a per-device pool; a call gives
-// its block back with an event recorded on its stream, and the next call
-// that takes it on another stream first makes its stream wait for that
-// event, so reuse is stream-ordered and nothing waits on the host. Calls
-// thus make no device allocation once the pool holds blocks of their size
-// (stream-ordered frees of a call's dozens of arrays cost up to
+// its block back with an event recorded on its stream. A later call on the
+// same stream reuses it at once (stream order); a call on another stream
+// takes it only once that event has completed (and still orders itself
+// after it on the device), else it allocates: calls on different streams
+// never wait for each other through the pool, and nothing waits on the
+// host. Calls thus make no device allocation once the pool holds blocks of
+// their size (stream-ordered frees of a call's dozens of arrays cost up to
 // milliseconds each). The pool keeps at most kScratchKeep blocks per device,
 // each at most kScratchKeepBytes; larger ones are freed after the call.
-constexpr int kScratchKeep = 4;
+constexpr int kScratchKeep = 8;
 constexpr size_t kScratchKeepBytes = size_t(1) << 30;
 
 class Scratch {

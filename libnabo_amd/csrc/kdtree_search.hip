@@ -534,32 +534,23 @@ __global__ __launch_bounds__(kBlock) void k_locate(const uint2* __restrict__ nod
 // Morton code of a 2-D / 3-D query in the cloud's bounding box with
 // axis_bits bits per axis (coordinates outside the box clamp to it, NaN to
 // its low corner).
+struct MortonBox {
+  int axis_bits;
+  float lo[3], sc[3];
+};
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ query,
-                                                   int64_t qstride, int64_t m, int axis_bits,
-                                                   float lo0, float lo1, float lo2, float sc0,
-                                                   float sc1, float sc2,
-                                                   uint32_t* __restrict__ keys) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const float top = float((1u << axis_bits) - 1u);
-  const float* qp = query + i * qstride;
-  const float lo[3] = {lo0, lo1, lo2}, sc[3] = {sc0, sc1, sc2};
+__device__ __forceinline__ uint32_t morton_code(const float* qp, const MortonBox& box) {
+  const float top = float((1u << box.axis_bits) - 1u);
   uint32_t code = 0;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    const uint32_t g = uint32_t(fminf(fmaxf((qp[d] - lo[d]) * sc[d], 0.f), top));
-    for (int b = 0; b < axis_bits; ++b) code |= ((g >> b) & 1u) << (b * DIM + d);
+    const uint32_t g = uint32_t(fminf(fmaxf((qp[d] - box.lo[d]) * box.sc[d], 0.f), top));
+    for (int b = 0; b < box.axis_bits; ++b) code |= ((g >> b) & 1u) << (b * DIM + d);
   }
-  keys[i] = code;
+  return code;
 }
 
-__global__ void k_shift_keys(uint32_t* keys, int64_t m, uint32_t shift) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < m) keys[i] >>= shift;
-}
-
-// Zeroes the search's chunk counters.
+// Zeroes the search's chunk counters (when the queries are not sorted).
 __global__ void k_zero_counters(uint32_t* counters, int64_t n) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e < n) counters[e] = 0;
@@ -571,7 +562,9 @@ __global__ void k_zero_counters(uint32_t* counters, int64_t n) {
 // depend on the order in which queries are searched.
 //   1. k_sort_hist: per tile of kSortTile queries, the count of each coarse
 //      bucket (the key's top coarse_bits), written bucket-major
-//      counts[bucket * tiles + tile];
+//      counts[bucket * tiles + tile]; for Morton keys it also computes and
+//      stores the keys, and its first workgroup zeroes the search's chunk
+//      counters (one launch instead of three);
 //   2. exclusive scan of counts: where each (bucket, tile) pair starts;
 //   3. k_sort_scatter: each tile places its queries in their coarse buckets;
 //   4. k_sort_fine: one workgroup per coarse bucket places its queries by
@@ -580,16 +573,31 @@ constexpr int kSortTile = 4096;
 constexpr int kCoarseBits = 12;
 constexpr int kFineBitsMax = 9;
 
-__global__ __launch_bounds__(kBlock) void k_sort_hist(const uint32_t* __restrict__ keys,
-                                                      int64_t m, int fine_bits, int coarse_bins,
-                                                      int64_t tiles, uint32_t* __restrict__ counts) {
+// MDIM = 2 / 3: keys are the queries' Morton codes, computed here and
+// stored; MDIM = 0: keys were computed before.
+template <int MDIM>
+__global__ __launch_bounds__(kBlock) void k_sort_hist(uint32_t* __restrict__ keys,
+                                                      const float* __restrict__ query,
+                                                      int64_t qstride, MortonBox box, int64_t m,
+                                                      int fine_bits, int coarse_bins,
+                                                      int64_t tiles, uint32_t* __restrict__ counts,
+                                                      uint32_t* __restrict__ counters) {
   __shared__ uint32_t h[1 << kCoarseBits];
   const int64_t tile = blockIdx.x;
+  if (tile == 0 && threadIdx.x < kXcds) counters[threadIdx.x] = 0;
   for (int b = threadIdx.x; b < coarse_bins; b += kBlock) h[b] = 0;
   __syncthreads();
   const int64_t end = min(m, (tile + 1) * kSortTile);
-  for (int64_t q = tile * kSortTile + threadIdx.x; q < end; q += kBlock)
-    atomicAdd(&h[keys[q] >> fine_bits], 1u);
+  for (int64_t q = tile * kSortTile + threadIdx.x; q < end; q += kBlock) {
+    uint32_t key;
+    if constexpr (MDIM > 0) {
+      key = morton_code<MDIM>(query + q * qstride, box);
+      keys[q] = key;
+    } else {
+      key = keys[q];
+    }
+    atomicAdd(&h[key >> fine_bits], 1u);
+  }
   __syncthreads();
   for (int b = threadIdx.x; b < coarse_bins; b += kBlock) counts[int64_t(b) * tiles + tile] = h[b];
 }
@@ -818,10 +826,12 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     locate_shift = full_bits > 16 ? full_bits - 16 : 0;
     key_bits = int(full_bits - locate_shift);
   }
+  // Keys have at most kCoarseBits + kFineBitsMax bits (Morton: 7 x 3 or
+  // 10 x 2; leaf keys: 16), so the fine pass orders the rest exactly.
+  static_assert(kCoarseBits + kFineBitsMax >= 21, "sort keys fit the two passes");
   const int coarse_bits = std::min(kCoarseBits, key_bits);
-  const int fine_bits = std::min(kFineBitsMax, key_bits - coarse_bits);
+  const int fine_bits = key_bits - coarse_bits;
   const int coarse_bins = 1 << coarse_bits;
-  const int shift_fine = key_bits - coarse_bits;  // key bits below the coarse ones
   const int64_t tiles = ceil_div(a.m, kSortTile);
   const int64_t ncounts = int64_t(coarse_bins) * tiles;
 
@@ -846,24 +856,26 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   NH_CHECK(hipGetDevice(&dev));
   scratch.commit(dev);
 
-  k_zero_counters<<<1, kBlock, 0, s>>>(counters.p, kXcds);
   p.work = counters.p;
+  if (!sort) k_zero_counters<<<1, kBlock, 0, s>>>(counters.p, kXcds);
   if (sort) {
     const unsigned g = unsigned(ceil_div(a.m, kBlock));
+    MortonBox box{axis_bits, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
     if (morton) {
-      float lo[3] = {0.f, 0.f, 0.f}, sc[3] = {0.f, 0.f, 0.f};
       const float top = float((1u << axis_bits) - 1u);
       for (int d = 0; d < ix.dim; ++d) {
-        lo[d] = ix.min_bound[size_t(d)];
-        const float ext = ix.max_bound[size_t(d)] - lo[d];
-        sc[d] = ext > 0.f ? top / ext : 0.f;
+        box.lo[d] = ix.min_bound[size_t(d)];
+        const float ext = ix.max_bound[size_t(d)] - box.lo[d];
+        box.sc[d] = ext > 0.f ? top / ext : 0.f;
       }
       if (ix.dim == 3)
-        k_morton<3><<<g, kBlock, 0, s>>>(a.query, a.qstride, a.m, axis_bits, lo[0], lo[1], lo[2],
-                                         sc[0], sc[1], sc[2], keys.p);
+        k_sort_hist<3><<<unsigned(tiles), kBlock, 0, s>>>(keys.p, a.query, a.qstride, box, a.m,
+                                                           fine_bits, coarse_bins, tiles, counts.p,
+                                                           counters.p);
       else
-        k_morton<2><<<g, kBlock, 0, s>>>(a.query, a.qstride, a.m, axis_bits, lo[0], lo[1], lo[2],
-                                         sc[0], sc[1], sc[2], keys.p);
+        k_sort_hist<2><<<unsigned(tiles), kBlock, 0, s>>>(keys.p, a.query, a.qstride, box, a.m,
+                                                           fine_bits, coarse_bins, tiles, counts.p,
+                                                           counters.p);
     } else {
       if (ix.dim == 3)
         k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
@@ -877,15 +889,10 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
         k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
                                          a.query, a.qstride, a.m, locate_shift, p.roots, p.qcloud,
                                          keys.p);
+      k_sort_hist<0><<<unsigned(tiles), kBlock, 0, s>>>(keys.p, a.query, a.qstride, box, a.m,
+                                                         fine_bits, coarse_bins, tiles, counts.p,
+                                                         counters.p);
     }
-    NH_LAUNCH_CHECK();
-    if (shift_fine > fine_bits) {
-      // Drop key bits below the fine ones (only when the key is wider than
-      // coarse + fine bits).
-      k_shift_keys<<<g, kBlock, 0, s>>>(keys.p, a.m, uint32_t(shift_fine - fine_bits));
-    }
-    k_sort_hist<<<unsigned(tiles), kBlock, 0, s>>>(keys.p, a.m, fine_bits, coarse_bins, tiles,
-                                                    counts.p);
     NH_LAUNCH_CHECK();
     size_t bytes = scan_bytes;
     NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, counts.p, starts.p, uint32_t(0),
