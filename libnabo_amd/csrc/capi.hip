@@ -384,6 +384,148 @@ Scratch::~Scratch() {
   for (const Block& b : dropped) drop(b);
 }
 
+namespace {
+
+// Pinned host staging for the host-buffer entry points. A copy between
+// pageable user memory and the device goes through a pinned buffer in
+// chunks: the DMA of one chunk overlaps the host copy of the next (a
+// pageable hipMemcpy moved ~10 GB/s here, pinned DMA ~55 GB/s and a host
+// memcpy ~30 GB/s). Buffers are pooled (at most kStageKeep, each at most
+// kStageKeepBytes kept); every call that uses one synchronizes its stream
+// before giving it back.
+constexpr size_t kStageChunk = size_t(8) << 20;
+constexpr int kStageEvents = 32;  // chunks per direction and call, at most
+constexpr int kStageKeep = 2;
+constexpr size_t kStageKeepBytes = size_t(256) << 20;
+
+struct StageBuf {
+  char* p = nullptr;
+  size_t bytes = 0;
+  hipEvent_t ev[kStageEvents] = {};
+};
+std::mutex g_stage_mu;
+std::vector<StageBuf> g_stage;
+
+class HostStage {
+ public:
+  HostStage(size_t bytes, hipStream_t s) : s_(s) {
+    {
+      std::lock_guard<std::mutex> lock(g_stage_mu);
+      int best = -1;
+      for (int i = 0; i < int(g_stage.size()); ++i)
+        if (g_stage[i].bytes >= bytes && (best < 0 || g_stage[i].bytes < g_stage[best].bytes))
+          best = i;
+      if (best >= 0) {
+        b_ = g_stage[best];
+        g_stage.erase(g_stage.begin() + best);
+      }
+    }
+    if (!b_.p) {
+      size_t sz = kStageChunk;
+      while (sz < bytes) sz <<= 1;
+      if (hipHostMalloc(reinterpret_cast<void**>(&b_.p), sz, hipHostMallocDefault) != hipSuccess) {
+        b_.p = nullptr;
+        throw Error(NABO_HIP_ERR_MEMORY, "pinned host allocation of " + std::to_string(sz) +
+                                             " bytes failed");
+      }
+      b_.bytes = sz;
+      for (hipEvent_t& e : b_.ev) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+          release_events();
+          (void)hipHostFree(b_.p);
+          b_.p = nullptr;
+          throw Error(NABO_HIP_ERR_HIP, "hipEventCreateWithFlags failed");
+        }
+      }
+    }
+  }
+  HostStage(const HostStage&) = delete;
+  HostStage& operator=(const HostStage&) = delete;
+  ~HostStage() {
+    if (!b_.p) return;
+    if (hipStreamSynchronize(s_) != hipSuccess || b_.bytes > kStageKeepBytes) {
+      free_buf(b_);
+      return;
+    }
+    std::vector<StageBuf> drop;
+    {
+      std::lock_guard<std::mutex> lock(g_stage_mu);
+      g_stage.push_back(b_);
+      while (int(g_stage.size()) > kStageKeep) {
+        int smallest = 0;
+        for (int i = 1; i < int(g_stage.size()); ++i)
+          if (g_stage[i].bytes < g_stage[smallest].bytes) smallest = i;
+        drop.push_back(g_stage[smallest]);
+        g_stage.erase(g_stage.begin() + smallest);
+      }
+    }
+    for (StageBuf& d : drop) free_buf(d);
+  }
+
+  // Host to device: chunk i is copied into the stage, then its DMA is
+  // enqueued; the next chunk's host copy overlaps it.
+  void to_device(void* dst, const void* src, size_t bytes, size_t stage_off) {
+    for (size_t off = 0; off < bytes; off += kStageChunk) {
+      const size_t len = std::min(kStageChunk, bytes - off);
+      std::memcpy(b_.p + stage_off + off, static_cast<const char*>(src) + off, len);
+      NH_CHECK(hipMemcpyAsync(static_cast<char*>(dst) + off, b_.p + stage_off + off, len,
+                              hipMemcpyHostToDevice, s_));
+    }
+  }
+  // Device to host, for several arrays: all DMAs are enqueued (one event
+  // per chunk), then each chunk is copied out as soon as it has landed.
+  struct Out {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  };
+  void to_host(const Out* outs, int count) {
+    size_t chunk = kStageChunk;
+    for (;;) {
+      size_t pieces = 0;
+      for (int i = 0; i < count; ++i) pieces += (outs[i].bytes + chunk - 1) / chunk;
+      if (pieces <= size_t(kStageEvents)) break;
+      chunk <<= 1;
+    }
+    struct Piece {
+      char* dst;
+      size_t stage_off, len;
+    };
+    Piece pieces[kStageEvents];
+    int n = 0;
+    size_t stage_off = 0;
+    for (int i = 0; i < count; ++i) {
+      for (size_t off = 0; off < outs[i].bytes; off += chunk) {
+        const size_t len = std::min(chunk, outs[i].bytes - off);
+        NH_CHECK(hipMemcpyAsync(b_.p + stage_off, static_cast<const char*>(outs[i].src) + off, len,
+                                hipMemcpyDeviceToHost, s_));
+        NH_CHECK(hipEventRecord(b_.ev[n], s_));
+        pieces[n++] = {static_cast<char*>(outs[i].dst) + off, stage_off, len};
+        stage_off += len;
+      }
+    }
+    for (int j = 0; j < n; ++j) {
+      NH_CHECK(hipEventSynchronize(b_.ev[j]));
+      std::memcpy(pieces[j].dst, b_.p + pieces[j].stage_off, pieces[j].len);
+    }
+  }
+
+ private:
+  void release_events() {
+    for (hipEvent_t& e : b_.ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  static void free_buf(StageBuf& b) {
+    for (hipEvent_t& e : b.ev)
+      if (e) (void)hipEventDestroy(e);
+    (void)hipHostFree(b.p);
+  }
+  hipStream_t s_;
+  StageBuf b_;
+};
+
+}  // namespace
+
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 void throw_hip(hipError_t e, const char* what, const char* file, int line) {
@@ -502,8 +644,10 @@ int nabo_hip_create(const float* cloud, int64_t n, int32_t rows, int64_t stride,
     Span<float> d_cloud;
     scratch.want(d_cloud, size_t(floats));
     scratch.commit(dev);
-    NH_CHECK(hipMemcpyAsync(d_cloud.p, cloud, sizeof(float) * size_t(floats),
-                            hipMemcpyHostToDevice, s));
+    {
+      HostStage stage(sizeof(float) * size_t(floats), s);
+      stage.to_device(d_cloud.p, cloud, sizeof(float) * size_t(floats), 0);
+    }
     auto* h = new nabo_hip_index();
     h->ix.device = dev;
     try {
@@ -611,9 +755,12 @@ int nabo_hip_knn(const nabo_hip_index* index, const float* query, int32_t qrows,
     scratch.want(di, size_t(m) * k);
     scratch.want(dt, 1);
     scratch.commit(ix.device);
-    NH_CHECK(hipMemcpyAsync(dq.p, query, sizeof(float) * size_t(qfloats), hipMemcpyHostToDevice, s));
-    if (max_radii)
-      NH_CHECK(hipMemcpyAsync(dr.p, max_radii, sizeof(float) * size_t(m), hipMemcpyHostToDevice, s));
+    const size_t q_bytes = sizeof(float) * size_t(qfloats);
+    const size_t r_bytes = max_radii ? sizeof(float) * size_t(m) : 0;
+    const size_t o_bytes = sizeof(float) * size_t(m) * size_t(k);
+    HostStage stage(std::max(q_bytes + r_bytes, 2 * o_bytes + sizeof(uint64_t)), s);
+    stage.to_device(dq.p, query, q_bytes, 0);
+    if (max_radii) stage.to_device(dr.p, max_radii, r_bytes, q_bytes);
     KnnArgs a;
     a.query = dq.p;
     a.qrows = qrows;
@@ -628,10 +775,10 @@ int nabo_hip_knn(const nabo_hip_index* index, const float* query, int32_t qrows,
     a.out_d2 = dd.p;
     a.touched = dt.p;
     run_knn(ix, a, s);
-    NH_CHECK(hipMemcpyAsync(indices, di.p, sizeof(int32_t) * size_t(m) * k, hipMemcpyDeviceToHost, s));
-    NH_CHECK(hipMemcpyAsync(dists2, dd.p, sizeof(float) * size_t(m) * k, hipMemcpyDeviceToHost, s));
     uint64_t t = 0;
-    NH_CHECK(hipMemcpyAsync(&t, dt.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    const HostStage::Out outs[3] = {{indices, di.p, o_bytes}, {dists2, dd.p, o_bytes},
+                                    {&t, dt.p, sizeof(uint64_t)}};
+    stage.to_host(outs, 3);
     NH_CHECK(hipStreamSynchronize(s));
     if (touched) {
       if (!(ix.creation_flags & NABO_HIP_TOUCH_STATISTICS))
