@@ -443,6 +443,12 @@ def main():
 
     log("building tree on the GPU")
     params = {"bucketSize": args.bucket, "querySort": 0 if args.no_query_sort else 1}
+    # The process's first build also loads code objects and fills the
+    # scratch pools; the second is the steady-state figure.
+    t0 = time.perf_counter()
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0, params)
+    build_ms_first = (time.perf_counter() - t0) * 1e3
+    nns.close()
     t0 = time.perf_counter()
     nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0, params)
     build_ms_host = (time.perf_counter() - t0) * 1e3
@@ -554,7 +560,8 @@ def main():
                    "tree_nodes": info.node_count, "tree_depth": info.depth},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "build_ms": {"host_cloud": round(build_ms_host, 2),
+        "build_ms": {"first_in_process": round(build_ms_first, 2),
+                     "host_cloud": round(build_ms_host, 2),
                      "device_cloud": None if build_ms_dev is None else round(build_ms_dev, 2)},
         "parity_vs_oracle": parity,
     }

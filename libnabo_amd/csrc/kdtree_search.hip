@@ -21,6 +21,8 @@
 // The k-best array is K = next supported size >= k entries in VGPRs; the
 // first K - k entries hold -inf and never move, so the head (largest kept
 // distance) is always entry K - 1 and no register is indexed dynamically.
+// KDTREE_TREE_HEAP instantiates the same kernel with K = 0 (IndexHeapSTL in
+// LDS) or K = -1 (in the output rows, when LDS is short).
 //
 // Queries are first put in a spatial order (Morton order in the cloud's box
 // for one 2-D / 3-D tree, else the order of the leaf their greedy descent
