@@ -93,6 +93,54 @@ def cpu_baseline(cloud, query, k, eps, radius, bucket, budget_s):
     }, (oi, od, touched, visits)
 
 
+def bench_lidar_variants(nns, cloud, query, d_query, radius, bucket, use_oracle: bool):
+    """The other k / epsilon pairs of the LiDAR workload (k in {1, 16},
+    epsilon in {0, 0.1}): time per knn call, search kernel time, parity with
+    the oracle on all queries, and the kernel's HBM roofline fraction by the
+    same algorithmic-byte count as the headline."""
+    import torch
+
+    import libnabo_amd as nabo
+    import oracle as orc
+
+    m = len(query)
+    tree = orc.default().kdtree(cloud, bucket_size=bucket) if use_oracle else None
+    out = {}
+    for k, eps in ((1, 0.0), (1, 0.1), (16, 0.1)):
+        di = torch.empty((m, k), dtype=torch.int32, device="cuda")
+        dd = torch.empty((m, k), dtype=torch.float32, device="cuda")
+        for _ in range(2):
+            nns.knn_device(d_query, k, di, dd, eps, 0, radius)
+        torch.cuda.synchronize()
+        nabo.kernel_timing(True)
+        nabo.kernel_timing_collect()
+        reps = 10
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            nns.knn_device(d_query, k, di, dd, eps, 0, radius)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        kms, launches = nabo.kernel_timing_collect()
+        nabo.kernel_timing(False)
+        kernel_ms = kms / max(1, launches)
+        entry = {"ms_per_call": round(ms, 4), "queries_per_s": round(m / (ms / 1e3), 1),
+                 "kernel_ms": round(kernel_ms, 4)}
+        if tree is not None:
+            oi, od, touched, visits = tree.knn(query, k, eps, 0, radius, stats=True,
+                                               threads=os.cpu_count() or 1)
+            gi, gd = di.cpu().numpy(), dd.cpu().numpy()
+            entry["parity_vs_oracle"] = bool(np.array_equal(gi, oi) and
+                                             np.array_equal(gd.view(np.uint32), od.view(np.uint32)))
+            algo = m * (12 + 8 * k) + 16 * int(touched.sum()) + 8 * int(visits.sum())
+            achieved = algo / (kernel_ms / 1e3) / 1e9
+            entry["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(achieved / HBM_PEAK_GBS, 4)}
+        out[f"k{k}_eps{eps}"] = entry
+        log(f"lidar k={k} eps={eps}: {entry}")
+    return out
+
+
 def bench_brute_force(log_cpu: bool):
     """Brute force, 200k x 33-D Dirichlet descriptors, 50k queries, k=8: exact
     fp32 path and bf16 / fp16 MFMA shortlists (m = 8, 32) with exact
@@ -565,6 +613,10 @@ def main():
                      "device_cloud": None if build_ms_dev is None else round(build_ms_dev, 2)},
         "parity_vs_oracle": parity,
     }
+    if not args.no_extras:
+        log("lidar: the other k / epsilon pairs")
+        out["lidar_variants"] = bench_lidar_variants(nns, cloud, query, d_query, args.radius,
+                                                     args.bucket, not args.no_cpu)
     extras = [] if args.no_extras else [e for e in args.extras.split(",") if e]
     unknown = set(extras) - {"brute_force", "batched", "served"}
     if unknown:
