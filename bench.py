@@ -29,9 +29,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per search launch on the default workload (lidar 1M x 1M, k=16,
 # bucket 8), from two rocprofv3 counter passes scaled by a 256 MiB copy:
-# FETCH_SIZE 54,797 KiB x 2.000 + WRITE_SIZE 130,641 KiB x 1.000
+# FETCH_SIZE 55,908 KiB x 2.000 + WRITE_SIZE 130,507 KiB x 1.000
 # (profiles/kd_search_traffic_lidar1M_k16.md).
-MEASURED_TRAFFIC_BYTES = (54_797 * 2.000 + 130_641 * 1.000) * 1024
+MEASURED_TRAFFIC_BYTES = (55_908 * 2.000 + 130_507 * 1.000) * 1024
 METRIC = "exact kNN queries/s, 1M 3D pts k=16; batched+served vs plain GPU; p99 ms"
 
 
