@@ -326,7 +326,9 @@ void Scratch::commit(int dev) {
     }
   }
   if (block_.p) {
-    if (block_.last != s_) NH_CHECK(hipStreamWaitEvent(s_, block_.done, 0));
+    // Also on the same stream: a destroyed stream's handle can come back
+    // for a new stream that does not follow the old one's work.
+    NH_CHECK(hipStreamWaitEvent(s_, block_.done, 0));
   } else {
     // Power-of-two sizes from 2 MiB: blocks then fit calls of similar sizes.
     size_t bytes = size_t(2) << 20;
@@ -356,7 +358,7 @@ Scratch::~Scratch() {
   if (!block_.p) return;
   auto drop = [this](const Block& b) {
     // stream-ordered after the block's last use, on this call's stream
-    if (b.last != s_) (void)hipStreamWaitEvent(s_, b.done, 0);
+    (void)hipStreamWaitEvent(s_, b.done, 0);
     (void)hipFreeAsync(b.p, s_);
     (void)hipEventDestroy(b.done);
   };
