@@ -63,6 +63,72 @@ __device__ __forceinline__ float padded_dist(const float* pp, const float (&q)[D
   return dist;
 }
 
+// Tiles of kTile points in LDS, DIMP floats per point, zero from dim on
+// (the zeros are written once; later tiles only overwrite the first dim
+// floats of each row). A tile is read from the packed cloud (dim floats per
+// point, src 16-byte aligned) with 16-byte loads issued together (the first
+// round before the barrier, so they are in flight while other waves finish
+// the previous tile); the barrier keeps the previous tile intact until
+// every thread is done with it.
+template <int DIMP, int BLOCK>
+__device__ __forceinline__ void zero_tile(float4* tile4) {
+  for (int e = threadIdx.x; e < kTile * DIMP / 4; e += BLOCK) tile4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+template <int DIMP, int BLOCK>
+__device__ __forceinline__ void load_round(float4 (&v)[(kTile * DIMP / 4 + BLOCK - 1) / BLOCK <= 12
+                                                          ? (kTile * DIMP / 4 + BLOCK - 1) / BLOCK
+                                                          : 8],
+                                           const float* src, int floats, int r0) {
+  constexpr int kRound = sizeof(v) / sizeof(v[0]);
+#pragma unroll
+  for (int j = 0; j < kRound; ++j) {
+    const int e = threadIdx.x + (r0 + j) * BLOCK;
+    if (4 * e + 4 <= floats) {
+      v[j] = reinterpret_cast<const float4*>(src)[e];
+    } else {
+      v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (4 * e < floats) v[j].x = src[4 * e];
+      if (4 * e + 1 < floats) v[j].y = src[4 * e + 1];
+      if (4 * e + 2 < floats) v[j].z = src[4 * e + 2];
+    }
+  }
+}
+template <int DIMP, int BLOCK, int N>
+__device__ __forceinline__ void scatter_round(float* tile, const float4 (&v)[N], int floats,
+                                              int dim, int r0) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const int e = threadIdx.x + (r0 + j) * BLOCK;
+    if (4 * e < floats) {
+      int pt = (4 * e) / dim, d = 4 * e - pt * dim;
+      const float c[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (4 * e + i < floats) tile[pt * DIMP + d] = c[i];
+        if (++d == dim) {
+          d = 0;
+          ++pt;
+        }
+      }
+    }
+  }
+}
+template <int DIMP, int BLOCK>
+__device__ __forceinline__ void load_tile(float4* tile4, const float* src, int floats, int dim) {
+  constexpr int kVec = (kTile * DIMP / 4 + BLOCK - 1) / BLOCK;  // float4 loads per thread
+  constexpr int kRound = kVec <= 12 ? kVec : 8;                 // issued together
+  float* tile = reinterpret_cast<float*>(tile4);
+  float4 v[kRound];
+  load_round<DIMP, BLOCK>(v, src, floats, 0);  // in flight while others finish the last tile
+  __syncthreads();
+  scatter_round<DIMP, BLOCK>(tile, v, floats, dim, 0);
+#pragma unroll
+  for (int r0 = kRound; r0 < kVec; r0 += kRound) {
+    load_round<DIMP, BLOCK>(v, src, floats, r0);
+    scatter_round<DIMP, BLOCK>(tile, v, floats, dim, r0);
+  }
+}
+
 struct BfParams {
   const float* cloud;
   int64_t cstride;
@@ -84,7 +150,8 @@ struct BfParams {
 // DIMP: compile-time bound on dim (query held in VGPRs).
 template <int DIMP, int K>
 __global__ __launch_bounds__(kBlock) void k_bf_chunk(BfParams a) {
-  __shared__ float tile[kTile * DIMP];
+  __shared__ float4 tile4[kTile * DIMP / 4];
+  const float* tile = reinterpret_cast<const float*>(tile4);
   const int64_t qi = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   const int chunk_id = blockIdx.y;
   const int64_t p0 = int64_t(chunk_id) * a.chunk;
@@ -108,13 +175,12 @@ __global__ __launch_bounds__(kBlock) void k_bf_chunk(BfParams a) {
     hi[j] = -1;
   }
 
+  zero_tile<DIMP, kBlock>(tile4);
   for (int64_t t0 = p0; t0 < p1; t0 += kTile) {
     const int tn = int(min<int64_t>(kTile, p1 - t0));
-    __syncthreads();
-    for (int e = threadIdx.x; e < kTile * DIMP; e += kBlock) {
-      const int pt = e / DIMP, d = e % DIMP;
-      tile[e] = (pt < tn && d < dim) ? a.cloud[(t0 + pt) * a.cstride + d] : 0.f;
-    }
+    // p0 is a multiple of kTile (the host rounds the chunk size), so the
+    // tile's first float is 16-byte aligned in the packed cloud
+    load_tile<DIMP, kBlock>(tile4, a.cloud + t0 * dim, tn * dim, dim);
     __syncthreads();
     if (valid) {
       for (int pt = 0; pt < tn; ++pt) {
@@ -188,55 +254,24 @@ __global__ __launch_bounds__(kStlBlock) void k_bf_stl(BfParams a, int sort_resul
   }
   StlHeap<LdsHeapStore> heap;
   if (valid) heap.reset(LdsHeapStore{heaps + threadIdx.x}, a.k);
-  for (int e = threadIdx.x; e < kTile * DIMP / 4; e += kStlBlock)
-    tile4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+  zero_tile<DIMP, kStlBlock>(tile4);
 
   for (int64_t t0 = 0; t0 < a.n; t0 += kTile) {
     const int tn = int(min<int64_t>(kTile, a.n - t0));
-    const int floats = tn * dim;
-    const float* src = a.cloud + t0 * dim;  // 16-byte aligned: t0 is a multiple of kTile
-    constexpr int kVec = kTile * DIMP / 4 / kStlBlock;  // float4 loads per lane, at most
-    float4 v[kVec];
-#pragma unroll
-    for (int j = 0; j < kVec; ++j) {
-      const int e = threadIdx.x + j * kStlBlock;
-      if (4 * e + 4 <= floats) {
-        v[j] = reinterpret_cast<const float4*>(src)[e];
-      } else {
-        v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (4 * e < floats) v[j].x = src[4 * e];
-        if (4 * e + 1 < floats) v[j].y = src[4 * e + 1];
-        if (4 * e + 2 < floats) v[j].z = src[4 * e + 2];
-      }
-    }
-    __syncthreads();  // the previous tile is no longer read
-#pragma unroll
-    for (int j = 0; j < kVec; ++j) {
-      const int e = threadIdx.x + j * kStlBlock;
-      if (4 * e < floats) {
-        int pt = (4 * e) / dim, d = 4 * e - pt * dim;
-        const float c[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          if (4 * e + i < floats) tile[pt * DIMP + d] = c[i];
-          if (++d == dim) {
-            d = 0;
-            ++pt;
-          }
-        }
-      }
-    }
+    // t0 is a multiple of kTile: 16-byte aligned in the packed cloud
+    load_tile<DIMP, kStlBlock>(tile4, a.cloud + t0 * dim, tn * dim, dim);
     __syncthreads();
     if (valid) {
       // kBfUnroll independent sums at a time (one wave per SIMD here, so the
       // wave's own instructions must overlap); rows past tn are read but not
       // admitted. Admissions stay in index order.
-      for (int pt = 0; pt < tn; pt += kBfUnroll) {
-        float dist[kBfUnroll];
+      constexpr int kUnroll = DIMP <= 36 ? kBfUnroll : (DIMP <= 64 ? 2 : 1);  // within VGPRs
+      for (int pt = 0; pt < tn; pt += kUnroll) {
+        float dist[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kBfUnroll; ++u) dist[u] = padded_dist<DIMP>(tile + (pt + u) * DIMP, q);
+        for (int u = 0; u < kUnroll; ++u) dist[u] = padded_dist<DIMP>(tile + (pt + u) * DIMP, q);
 #pragma unroll
-        for (int u = 0; u < kBfUnroll; ++u)
+        for (int u = 0; u < kUnroll; ++u)
           if (pt + u < tn && dist[u] <= r2 && dist[u] < heap.head &&
               (a.allow_self || dist[u] > FLT_EPSILON))
             heap.replace_head(int32_t(t0 + pt + u), dist[u]);
@@ -714,11 +749,12 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   p.max_r2 = a.max_radius * a.max_radius;
   p.radii = a.radii;
   p.allow_self = (a.flags & NABO_HIP_ALLOW_SELF_MATCH) ? 1 : 0;
+  if (p.cstride != p.dim) throw Error(NABO_HIP_ERR_ARGUMENT, "brute-force cloud is not packed");
   const int64_t qblocks = ceil_div(a.m, kBlock);
   int64_t chunks = std::max<int64_t>(1, ceil_div(2048, qblocks));
   chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, ceil_div(ix.n, 1024)));
   chunks = std::min<int64_t>(chunks, 65535);
-  p.chunk = ceil_div(ix.n, chunks);
+  p.chunk = ceil_div(ceil_div(ix.n, chunks), kTile) * kTile;  // tiles start 16-byte aligned
   p.chunks = int(ceil_div(ix.n, p.chunk));
   if (ix.params.bf_precision != NABO_HIP_BF_FP32) {
     const int l = a.k + ix.params.bf_extra;
