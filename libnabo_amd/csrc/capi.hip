@@ -569,6 +569,7 @@ Index::~Index() {
   if (d_cloud_lp) (void)hipFree(d_cloud_lp);
   if (d_norms) (void)hipFree(d_norms);
   if (d_roots) (void)hipFree(d_roots);
+  if (d_boxes) (void)hipFree(d_boxes);
 }
 
 }  // namespace nabo_hip
@@ -1043,6 +1044,7 @@ int nabo_hip_deserialize(const void* buf, uint64_t bytes, int32_t device, nabo_h
       if (h.len[kArMin]) std::memcpy(ix.min_bound.data(), in + h.off[kArMin], h.len[kArMin]);
       if (h.len[kArMax]) std::memcpy(ix.max_bound.data(), in + h.off[kArMax], h.len[kArMax]);
       NH_CHECK(hipStreamSynchronize(s));
+      if (kd) kd_make_boxes(ix, s);
     } catch (...) {
       delete hnd;
       throw;

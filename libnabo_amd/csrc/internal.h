@@ -153,6 +153,8 @@ struct Index {
   int32_t clouds = 1;
   std::vector<int64_t> cloud_offsets;
   int32_t* d_roots = nullptr;
+  float* d_boxes = nullptr;  // batched 2-D / 3-D trees: per cloud {min bound[3],
+                             // 1 / extent[3] (0 if none)}, for the query order
 
   std::vector<float> min_bound, max_bound;  // clouds x dim
   int64_t device_bytes = 0;
@@ -180,6 +182,8 @@ struct KnnArgs {
 // offsets: clouds + 1 point offsets into d_cloud ({0, n} for one cloud).
 void kd_build(Index& ix, const float* d_cloud, int64_t stride,
               const std::vector<int64_t>& offsets, hipStream_t s);
+// Fills d_boxes from the per-cloud bounds (batched 2-D / 3-D trees).
+void kd_make_boxes(Index& ix, hipStream_t s);
 // Searches a built kd-tree (kdtree_search.hip).
 void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s);
 // Prepares and searches a brute-force index (brute_force.hip).

@@ -912,6 +912,23 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
   }
   ix.node_count = count;
   ix.depth = levels - 1;
+  kd_make_boxes(ix, s);
+}
+
+void kd_make_boxes(Index& ix, hipStream_t s) {
+  if (ix.clouds <= 1 || (ix.dim != 2 && ix.dim != 3) || ix.d_boxes) return;
+  std::vector<float> hb(size_t(ix.clouds) * 6, 0.f);
+  for (int32_t c = 0; c < ix.clouds; ++c)
+    for (int d = 0; d < ix.dim; ++d) {
+      const float lo = ix.min_bound[size_t(c) * ix.dim + d];
+      const float ext = ix.max_bound[size_t(c) * ix.dim + d] - lo;
+      hb[size_t(c) * 6 + d] = lo;
+      hb[size_t(c) * 6 + 3 + d] = ext > 0.f ? 1.f / ext : 0.f;
+    }
+  NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_boxes), sizeof(float) * hb.size()));
+  NH_CHECK(hipMemcpyAsync(ix.d_boxes, hb.data(), sizeof(float) * hb.size(),
+                          hipMemcpyHostToDevice, s));
+  NH_CHECK(hipStreamSynchronize(s));
 }
 
 }  // namespace nabo_hip

@@ -605,12 +605,16 @@ constexpr int kHistBlock = 1024;  // k_sort_hist: 4 queries per thread and tile
 constexpr int kCoarseBits = 12;
 constexpr int kFineBitsMax = 9;
 
-// MDIM = 2 / 3: keys are the queries' Hilbert indices, computed here and
-// stored; MDIM = 0: keys were computed before.
+// MDIM = 2 / 3: keys are the queries' Hilbert indices (batched trees: with
+// the cloud id above, boxes per cloud), computed here and stored; MDIM = 0:
+// keys were computed before.
 template <int MDIM>
 __global__ __launch_bounds__(kHistBlock) void k_sort_hist(uint32_t* __restrict__ keys,
                                                       const float* __restrict__ query,
-                                                      int64_t qstride, CurveBox box, int64_t m,
+                                                      int64_t qstride, CurveBox box,
+                                                      const int32_t* __restrict__ qcloud,
+                                                      const float* __restrict__ boxes,
+                                                      uint32_t curve_bits, int64_t m,
                                                       int fine_bits, int coarse_bins,
                                                       int64_t tiles, uint32_t* __restrict__ counts,
                                                       uint32_t* __restrict__ counters) {
@@ -623,7 +627,20 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_hist(uint32_t* __restrict__
   for (int64_t q = tile * kSortTile + threadIdx.x; q < end; q += kHistBlock) {
     uint32_t key;
     if constexpr (MDIM > 0) {
-      key = hilbert_code<MDIM>(query + q * qstride, box);
+      if (qcloud) {
+        // batched trees: the cloud id above the index in that cloud's box
+        const int32_t c = qcloud[q];
+        const float top = float((1u << box.axis_bits) - 1u);
+        CurveBox cb{box.axis_bits, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int d = 0; d < MDIM; ++d) {
+          cb.lo[d] = boxes[int64_t(c) * 6 + d];
+          cb.sc[d] = boxes[int64_t(c) * 6 + 3 + d] * top;
+        }
+        key = (uint32_t(c) << curve_bits) | hilbert_code<MDIM>(query + q * qstride, cb);
+      } else {
+        key = hilbert_code<MDIM>(query + q * qstride, box);
+      }
       keys[q] = key;
     } else {
       key = keys[q];
@@ -836,23 +853,24 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     p.qcloud = a.qcloud;
   }
 
-  // Query order: keys, then the two-level bucket sort. One 2-D / 3-D tree:
-  // key = Hilbert index of the query in the cloud's box, with up to a few
-  // cells per query (at most 2^21 cells); lanes of a wave then walk
-  // neighbouring paths. Batched trees and other dimensions: key = bucket
-  // start of the leaf the greedy descent reaches, cut to 16 bits (bucket
-  // starts are global, so this also groups the queries of batched trees by
-  // cloud).
+  // Query order: keys, then the two-level bucket sort. 2-D / 3-D trees:
+  // key = Hilbert index of the query in its cloud's box, with up to a few
+  // cells per query (at most 2^21 cells); batched trees put the cloud id
+  // above it (the index then gets the bits left of 21). Lanes of a wave then
+  // walk neighbouring paths. Other dimensions: key = bucket start of the
+  // leaf the greedy descent reaches, cut to 16 bits (bucket starts are
+  // global, so this also groups the queries of batched trees by cloud).
   // Scratch of the call: chunk counters, then the sort's arrays, sized
   // before anything is launched.
   const bool sort = ix.params.query_sort && a.m > 1 && ix.node_count > 1;
-  const bool curve = ix.clouds == 1 && (ix.dim == 2 || ix.dim == 3);
+  const int cloud_bits = ix.clouds > 1 ? int(bits_needed(uint64_t(ix.clouds - 1))) : 0;
+  const int max_bits = std::min(ix.dim == 3 ? 7 : 10, (21 - cloud_bits) / std::max(ix.dim, 1));
+  const bool curve = (ix.dim == 2 || ix.dim == 3) && max_bits >= 2;
   int key_bits = 0, axis_bits = 0;
   uint32_t locate_shift = 0;
   if (sort && curve) {
-    const int max_bits = ix.dim == 3 ? 7 : 10;
     axis_bits = std::max(1, std::min(max_bits, (int(bits_needed(uint64_t(a.m))) + 2) / ix.dim));
-    key_bits = axis_bits * ix.dim;
+    key_bits = cloud_bits + axis_bits * ix.dim;
   } else if (sort) {
     const uint32_t full_bits = std::max(1u, bits_needed(uint64_t(ix.n)));
     locate_shift = full_bits > 16 ? full_bits - 16 : 0;
@@ -894,20 +912,28 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     const unsigned g = unsigned(ceil_div(a.m, kBlock));
     CurveBox box{axis_bits, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
     if (curve) {
+      // Cell scale of the cloud's box (one tree); batched trees read theirs
+      // from d_boxes (a cloud of at most bucketSize points keeps the
+      // reference's initial bounds, max < min: scale 0).
       const float top = float((1u << axis_bits) - 1u);
-      for (int d = 0; d < ix.dim; ++d) {
-        box.lo[d] = ix.min_bound[size_t(d)];
-        const float ext = ix.max_bound[size_t(d)] - box.lo[d];
-        box.sc[d] = ext > 0.f ? top / ext : 0.f;
+      if (ix.clouds == 1) {
+        for (int d = 0; d < ix.dim; ++d) {
+          box.lo[d] = ix.min_bound[size_t(d)];
+          const float ext = ix.max_bound[size_t(d)] - box.lo[d];
+          box.sc[d] = ext > 0.f ? top / ext : 0.f;
+        }
       }
+      if (ix.clouds > 1 && !ix.d_boxes) throw Error(NABO_HIP_ERR_HIP, "batched tree without boxes");
+      const float* d_boxes = ix.clouds > 1 ? ix.d_boxes : nullptr;
+      const uint32_t curve_bits = uint32_t(axis_bits * ix.dim);
       if (ix.dim == 3)
-        k_sort_hist<3><<<unsigned(tiles), kHistBlock, 0, s>>>(keys.p, a.query, a.qstride, box, a.m,
-                                                           fine_bits, coarse_bins, tiles, counts.p,
-                                                           counters.p);
+        k_sort_hist<3><<<unsigned(tiles), kHistBlock, 0, s>>>(
+            keys.p, a.query, a.qstride, box, p.qcloud, d_boxes, curve_bits, a.m, fine_bits,
+            coarse_bins, tiles, counts.p, counters.p);
       else
-        k_sort_hist<2><<<unsigned(tiles), kHistBlock, 0, s>>>(keys.p, a.query, a.qstride, box, a.m,
-                                                           fine_bits, coarse_bins, tiles, counts.p,
-                                                           counters.p);
+        k_sort_hist<2><<<unsigned(tiles), kHistBlock, 0, s>>>(
+            keys.p, a.query, a.qstride, box, p.qcloud, d_boxes, curve_bits, a.m, fine_bits,
+            coarse_bins, tiles, counts.p, counters.p);
     } else {
       if (ix.dim == 3)
         k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
@@ -921,9 +947,9 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
         k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
                                          a.query, a.qstride, a.m, locate_shift, p.roots, p.qcloud,
                                          keys.p);
-      k_sort_hist<0><<<unsigned(tiles), kHistBlock, 0, s>>>(keys.p, a.query, a.qstride, box, a.m,
-                                                         fine_bits, coarse_bins, tiles, counts.p,
-                                                         counters.p);
+      k_sort_hist<0><<<unsigned(tiles), kHistBlock, 0, s>>>(
+          keys.p, a.query, a.qstride, box, nullptr, nullptr, 0, a.m, fine_bits, coarse_bins,
+          tiles, counts.p, counters.p);
     }
     NH_LAUNCH_CHECK();
     size_t bytes = scan_bytes;
