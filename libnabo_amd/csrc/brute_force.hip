@@ -145,10 +145,20 @@ struct BfParams {
   int chunks;
   float* part_d;   // chunks x m x k
   int32_t* part_i;
+  // IndexHeapSTL in one pass: per (chunk, query) admission logs, slot-major
+  // log[(chunk * log_cap + j) * m + query], their lengths log_n[chunk * m +
+  // query] (may exceed log_cap: overflow), and the queries to redo by a full
+  // scan (redo[query] != 0).
+  uint2* log;
+  uint32_t* log_n;
+  int log_cap;
+  int32_t* redo;
 };
 
-// DIMP: compile-time bound on dim (query held in VGPRs).
-template <int DIMP, int K>
+// DIMP: compile-time bound on dim (query held in VGPRs). LOG: also append
+// every admission of the chunk's own k-best (index, dist) to the query's
+// log, instead of writing the chunk's k-best.
+template <int DIMP, int K, bool LOG>
 __global__ __launch_bounds__(kBlock) void k_bf_chunk(BfParams a) {
   __shared__ float4 tile4[kTile * DIMP / 4];
   const float* tile = reinterpret_cast<const float*>(tile4);
@@ -175,6 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_bf_chunk(BfParams a) {
     hi[j] = -1;
   }
 
+  uint32_t nlog = 0;
   zero_tile<DIMP, kBlock>(tile4);
   for (int64_t t0 = p0; t0 < p1; t0 += kTile) {
     const int tn = int(min<int64_t>(kTile, p1 - t0));
@@ -185,12 +196,23 @@ __global__ __launch_bounds__(kBlock) void k_bf_chunk(BfParams a) {
     if (valid) {
       for (int pt = 0; pt < tn; ++pt) {
         const float dist = padded_dist<DIMP>(tile + pt * DIMP, q);
-        if (dist <= r2 && dist < hd[K - 1] && (a.allow_self || dist > FLT_EPSILON))
+        if (dist <= r2 && dist < hd[K - 1] && (a.allow_self || dist > FLT_EPSILON)) {
+          if constexpr (LOG) {
+            if (nlog < uint32_t(a.log_cap))
+              a.log[(int64_t(chunk_id) * a.log_cap + nlog) * a.m + qi] =
+                  make_uint2(uint32_t(t0 + pt), __float_as_uint(dist));
+            ++nlog;
+          }
           bf_insert<K>(hd, hi, dist, int32_t(t0 + pt));
+        }
       }
     }
   }
   if (!valid) return;
+  if constexpr (LOG) {
+    a.log_n[int64_t(chunk_id) * a.m + qi] = nlog;
+    return;
+  }
   const int64_t o = (int64_t(chunk_id) * a.m + qi) * a.k;
 #pragma unroll
   for (int j = 0; j < K; ++j) {
@@ -230,6 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_bf_merge(BfParams a, int32_t* __rest
 }
 
 constexpr int kStlBlock = 64;  // one wave per workgroup: m / 64 waves spread over the CUs
+constexpr size_t kStlLogBytes = size_t(256) << 20;  // admission logs per batch of queries
 
 template <int DIMP>
 __global__ __launch_bounds__(kStlBlock) void k_bf_stl(BfParams a, int sort_results,
@@ -242,7 +265,9 @@ __global__ __launch_bounds__(kStlBlock) void k_bf_stl(BfParams a, int sort_resul
   extern __shared__ uint2 heaps[];  // k slots x kStlBlock lanes
   const int64_t qi = int64_t(blockIdx.x) * kStlBlock + threadIdx.x;
   const int dim = a.dim;
-  const bool valid = qi < a.m;
+  // with redo set, only the queries it marks (the rest are done)
+  const bool valid = qi < a.m && (!a.redo || a.redo[qi] != 0);
+  if (!__syncthreads_or(valid)) return;
 
   float q[DIMP];
 #pragma unroll
@@ -283,6 +308,37 @@ __global__ __launch_bounds__(kStlBlock) void k_bf_stl(BfParams a, int sort_resul
   heap.write(out_idx + qi * a.k, out_d2 + qi * a.k);
 }
 
+// Replays a query's chunk logs in chunk order through an IndexHeapSTL with
+// the global admission test. A chunk's own k-best starts empty, so its head
+// is never below the global head at the same point: every point the
+// reference admits is in its chunk's log, in order, and the replay admits
+// exactly those. A query with an overflowed log is left to a full scan.
+__global__ __launch_bounds__(kStlBlock) void k_bf_replay(BfParams a, int sort_results,
+                                                         int32_t* __restrict__ out_idx,
+                                                         float* __restrict__ out_d2) {
+  extern __shared__ uint2 heaps[];  // k slots x kStlBlock lanes
+  const int64_t qi = int64_t(blockIdx.x) * kStlBlock + threadIdx.x;
+  if (qi >= a.m) return;
+  StlHeap<LdsHeapStore> heap;
+  heap.reset(LdsHeapStore{heaps + threadIdx.x}, a.k);
+  for (int c = 0; c < a.chunks; ++c) {
+    const uint32_t n = a.log_n[int64_t(c) * a.m + qi];
+    if (n > uint32_t(a.log_cap)) {
+      a.redo[qi] = 1;
+      return;
+    }
+    const uint2* lg = a.log + int64_t(c) * a.log_cap * a.m + qi;
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint2 e = lg[int64_t(j) * a.m];
+      const float d = __uint_as_float(e.y);
+      if (d < heap.head) heap.replace_head(int32_t(e.x), d);
+    }
+  }
+  a.redo[qi] = 0;
+  if (sort_results) heap.sort();
+  heap.write(out_idx + qi * a.k, out_d2 + qi * a.k);
+}
+
 template <int DIMP>
 void launch_bf_stl(const BfParams& p, int sort_results, hipStream_t s, int32_t* oi, float* od) {
   if (p.cstride != p.dim) throw Error(NABO_HIP_ERR_ARGUMENT, "brute-force cloud is not packed");
@@ -293,27 +349,40 @@ void launch_bf_stl(const BfParams& p, int sort_results, hipStream_t s, int32_t* 
                                                                                  oi, od);
 }
 
-template <int DIMP, int K>
+// LOG: the chunk pass of the IndexHeapSTL form (logs only, no merge).
+template <int DIMP, int K, bool LOG>
 void launch_bf(const BfParams& p, hipStream_t s, int32_t* oi, float* od) {
   dim3 grid(unsigned(ceil_div(p.m, kBlock)), unsigned(p.chunks));
-  k_bf_chunk<DIMP, K><<<grid, kBlock, 0, s>>>(p);
-  k_bf_merge<K><<<unsigned(ceil_div(p.m, kBlock)), kBlock, 0, s>>>(p, oi, od);
+  k_bf_chunk<DIMP, K, LOG><<<grid, kBlock, 0, s>>>(p);
+  if constexpr (!LOG) k_bf_merge<K><<<unsigned(ceil_div(p.m, kBlock)), kBlock, 0, s>>>(p, oi, od);
 }
 
-template <int DIMP>
+template <int DIMP, bool LOG = false>
 void bf_dispatch_k(const BfParams& p, hipStream_t s, int32_t* oi, float* od) {
   if (p.k <= 1)
-    launch_bf<DIMP, 1>(p, s, oi, od);
+    launch_bf<DIMP, 1, LOG>(p, s, oi, od);
   else if (p.k <= 4)
-    launch_bf<DIMP, 4>(p, s, oi, od);
+    launch_bf<DIMP, 4, LOG>(p, s, oi, od);
   else if (p.k <= 8)
-    launch_bf<DIMP, 8>(p, s, oi, od);
+    launch_bf<DIMP, 8, LOG>(p, s, oi, od);
   else if (p.k <= 16)
-    launch_bf<DIMP, 16>(p, s, oi, od);
+    launch_bf<DIMP, 16, LOG>(p, s, oi, od);
   else if (p.k <= 32)
-    launch_bf<DIMP, 32>(p, s, oi, od);
+    launch_bf<DIMP, 32, LOG>(p, s, oi, od);
   else
-    launch_bf<DIMP, 64>(p, s, oi, od);
+    launch_bf<DIMP, 64, LOG>(p, s, oi, od);
+}
+
+// IndexHeapSTL brute force: chunk pass with logs, replay, then a full scan
+// of the queries whose logs overflowed.
+template <int DIMP>
+void launch_bf_stl_logged(const BfParams& p, int sort_results, hipStream_t s, int32_t* oi,
+                          float* od) {
+  bf_dispatch_k<DIMP, true>(p, s, oi, od);
+  const size_t heap_lds = size_t(p.k) * kStlBlock * sizeof(uint2);
+  k_bf_replay<<<unsigned(ceil_div(p.m, kStlBlock)), kStlBlock, heap_lds, s>>>(p, sort_results,
+                                                                                oi, od);
+  launch_bf_stl<DIMP>(p, sort_results, s, oi, od);
 }
 
 __global__ void k_copy_cloud(int64_t n, int dim, const float* __restrict__ src, int64_t stride,
@@ -799,20 +868,50 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   }
   if (ix.params.heap == NABO_HIP_HEAP_STL) {
     const int sort = (a.flags & NABO_HIP_SORT_RESULTS) ? 1 : 0;
+    // Chunk logs of up to `cap` admissions per (chunk, query); a chunk's
+    // own k-best admits about k (1 + ln(chunk / k)) points, so a log
+    // overflows only for adversarial orders (those queries are redone by a
+    // full scan). Queries go in batches that keep the logs within
+    // kStlLogBytes.
+    const int cap = std::min(1024, std::max(64, 16 * a.k));
+    const size_t per_query =
+        size_t(p.chunks) * (size_t(cap) * sizeof(uint2) + sizeof(uint32_t)) + sizeof(int32_t);
+    int64_t mb = std::max<int64_t>(kBlock, int64_t(kStlLogBytes / per_query) / kBlock * kBlock);
+    mb = std::min(mb, a.m);
+    Scratch scratch(s);
+    Span<uint2> log;
+    Span<uint32_t> log_n;
+    Span<int32_t> redo;
+    scratch.want(log, size_t(p.chunks) * size_t(cap) * size_t(mb));
+    scratch.want(log_n, size_t(p.chunks) * size_t(mb));
+    scratch.want(redo, size_t(mb));
+    scratch.commit(ix.device);
     KernelTimer timer(s);
-    if (ix.dim <= 4)
-      launch_bf_stl<4>(p, sort, s, a.out_idx, a.out_d2);
-    else if (ix.dim <= 8)
-      launch_bf_stl<8>(p, sort, s, a.out_idx, a.out_d2);
-    else if (ix.dim <= 16)
-      launch_bf_stl<16>(p, sort, s, a.out_idx, a.out_d2);
-    else if (ix.dim <= 36)
-      launch_bf_stl<36>(p, sort, s, a.out_idx, a.out_d2);
-    else if (ix.dim <= 64)
-      launch_bf_stl<64>(p, sort, s, a.out_idx, a.out_d2);
-    else
-      launch_bf_stl<128>(p, sort, s, a.out_idx, a.out_d2);
-    NH_LAUNCH_CHECK();
+    for (int64_t q0 = 0; q0 < a.m; q0 += mb) {
+      BfParams b = p;
+      b.query = a.query + q0 * a.qstride;
+      b.m = std::min(mb, a.m - q0);
+      b.radii = a.radii ? a.radii + q0 : nullptr;
+      b.log = log.p;
+      b.log_n = log_n.p;
+      b.log_cap = cap;
+      b.redo = redo.p;
+      int32_t* oi = a.out_idx + q0 * a.k;
+      float* od = a.out_d2 + q0 * a.k;
+      if (ix.dim <= 4)
+        launch_bf_stl_logged<4>(b, sort, s, oi, od);
+      else if (ix.dim <= 8)
+        launch_bf_stl_logged<8>(b, sort, s, oi, od);
+      else if (ix.dim <= 16)
+        launch_bf_stl_logged<16>(b, sort, s, oi, od);
+      else if (ix.dim <= 36)
+        launch_bf_stl_logged<36>(b, sort, s, oi, od);
+      else if (ix.dim <= 64)
+        launch_bf_stl_logged<64>(b, sort, s, oi, od);
+      else
+        launch_bf_stl_logged<128>(b, sort, s, oi, od);
+      NH_LAUNCH_CHECK();
+    }
     return;
   }
   Scratch scratch(s);

@@ -111,3 +111,23 @@ def test_stl_heap_refuses_shortlist(nabo):
         nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, 3, 0,
                                     {"heap": "stl", "bfPrecision": "bf16"})
     assert e.value.code == 4
+
+
+def test_brute_force_stl_log_overflow(nabo, oracle):
+    # Points met in order of decreasing distance from the first queries:
+    # every point enters a chunk's k-best, which overflows its admission log;
+    # those queries are redone by a full scan, the others replayed.
+    rng = np.random.default_rng(34)
+    n = 5_000
+    line = np.zeros((n, 3), np.float32)
+    line[:, 0] = 1000.0 - 0.1 * np.arange(n, dtype=np.float32)
+    cloud = np.concatenate([line, (rng.random((3_000, 3)) * 50).astype(np.float32)])
+    query = np.concatenate([
+        np.array([[-1.0, 0.0, 0.0], [-2.0, 0.01, 0.0]], np.float32),
+        (rng.random((300, 3)) * 50).astype(np.float32)])
+    print("case bf stl log overflow", flush=True)
+    bf = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE)
+    for k, flags in ((1, 0), (4, SORT), (8, 0)):
+        gi, gd = bf.knn(query, k, 0.0, flags)
+        oi, od = oracle.bf_knn(cloud, query, k, flags=flags, threads=8)
+        _exact(gi, gd, oi, od)
