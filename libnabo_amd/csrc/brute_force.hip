@@ -13,11 +13,13 @@
 // keeps a sorted k-best of its chunk; a merge kernel then combines the chunks
 // of a query in chunk order, which preserves the lowest-index tie rule.
 //
-// IndexHeapSTL form (the reference's own heap, brute_force_cpu.cpp:79): one
-// chunk, so each lane admits the points of the whole cloud in index order
-// exactly as the reference does, into a binary heap in LDS (StlHeap);
-// results are then bit-identical to the reference's, including the order
-// and which of equal distances are kept.
+// IndexHeapSTL form (the reference's own heap, brute_force_cpu.cpp:79): the
+// same chunk pass also logs each chunk's admissions; one lane per query then
+// replays its logs in chunk order through a binary heap in LDS (StlHeap)
+// with the reference's admission test, and queries whose logs overflowed
+// are redone by one lane scanning the whole cloud in index order. Results
+// are bit-identical to the reference's, including the order and which of
+// equal distances are kept.
 
 #include <cfloat>
 
