@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_bf_merge(BfParams a, int32_t* __rest
 }
 
 constexpr int kStlBlock = 64;  // one wave per workgroup: m / 64 waves spread over the CUs
-constexpr size_t kStlLogBytes = size_t(256) << 20;  // admission logs per batch of queries
+constexpr size_t kStlLogBytes = size_t(768) << 20;  // admission logs per batch of queries
 
 template <int DIMP>
 __global__ __launch_bounds__(kStlBlock) void k_bf_stl(BfParams a, int sort_results,
