@@ -13,7 +13,7 @@
 //   BRUTE_FORCE                    -> HIP brute force, IndexHeapSTL results
 //     (the reference's own heap: same order and ties)
 //   BRUTE_FORCE_HIP                -> HIP brute force, ascending results,
-//     ties by index (chunked over the cloud, faster)
+//     ties by index
 //   KDTREE_CL_* / BRUTE_FORCE_CL                      -> error, as the
 //     reference without OpenCL (nabo/nabo.cpp:148-151)
 // Scalar type float only (NNSearchF); NNSearchD searches throw.
