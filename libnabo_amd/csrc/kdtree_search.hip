@@ -677,7 +677,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_fine(const uint32_t* __restrict
                                                       int64_t tiles, int64_t m, int fine_bits,
                                                       int coarse_bins, int32_t* __restrict__ order) {
   __shared__ uint32_t h[1 << kFineBitsMax];
-  __shared__ uint32_t part[kBlock];
+  __shared__ uint32_t wave_sum[kBlock / kWave];
   const int bin = blockIdx.x;
   const uint32_t begin = starts[int64_t(bin) * tiles];
   const uint32_t end = bin + 1 < coarse_bins ? starts[int64_t(bin + 1) * tiles] : uint32_t(m);
@@ -695,15 +695,19 @@ __global__ __launch_bounds__(kBlock) void k_sort_fine(const uint32_t* __restrict
   const int b0 = threadIdx.x * per;
   uint32_t sum = 0;
   for (int b = b0; b < min(bins, b0 + per); ++b) sum += h[b];
-  part[threadIdx.x] = sum;
-  __syncthreads();
-  for (int d = 1; d < kBlock; d <<= 1) {
-    const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
+  // inclusive scan of the partial sums: within each wave by shuffles, then
+  // across the block's waves through LDS (one barrier)
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  uint32_t incl = sum;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t up = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += up;
   }
-  uint32_t run = begin + part[threadIdx.x] - sum;
+  if (lane == kWave - 1) wave_sum[wave] = incl;
+  __syncthreads();
+  for (int w = 0; w < wave; ++w) incl += wave_sum[w];
+  uint32_t run = begin + incl - sum;
   for (int b = b0; b < min(bins, b0 + per); ++b) {
     const uint32_t c = h[b];
     h[b] = run;
