@@ -95,9 +95,10 @@ def cpu_baseline(cloud, query, k, eps, radius, bucket, budget_s):
 
 def bench_lidar_variants(nns, cloud, query, d_query, radius, bucket, use_oracle: bool):
     """The other k / epsilon pairs of the LiDAR workload (k in {1, 16},
-    epsilon in {0, 0.1}): time per knn call, search kernel time, parity with
-    the oracle on all queries, and the kernel's HBM roofline fraction by the
-    same algorithmic-byte count as the headline."""
+    epsilon in {0, 0.1}), and k = 16 with KDTREE_TREE_HEAP (IndexHeapSTL):
+    time per knn call, search kernel time, parity with the oracle on all
+    queries, and the kernel's HBM roofline fraction by the same
+    algorithmic-byte count as the headline."""
     import torch
 
     import libnabo_amd as nabo
@@ -106,18 +107,22 @@ def bench_lidar_variants(nns, cloud, query, d_query, radius, bucket, use_oracle:
     m = len(query)
     tree = orc.default().kdtree(cloud, bucket_size=bucket) if use_oracle else None
     out = {}
-    for k, eps in ((1, 0.0), (1, 0.1), (16, 0.1)):
+    stl = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_TREE_HEAP, 3, 0,
+                                      {"bucketSize": bucket})
+    for k, eps, heap in ((1, 0.0, "linear"), (1, 0.1, "linear"), (16, 0.1, "linear"),
+                         (16, 0.0, "stl")):
+        index = nns if heap == "linear" else stl
         di = torch.empty((m, k), dtype=torch.int32, device="cuda")
         dd = torch.empty((m, k), dtype=torch.float32, device="cuda")
         for _ in range(2):
-            nns.knn_device(d_query, k, di, dd, eps, 0, radius)
+            index.knn_device(d_query, k, di, dd, eps, 0, radius)
         torch.cuda.synchronize()
         nabo.kernel_timing(True)
         nabo.kernel_timing_collect()
         reps = 10
         t0 = time.perf_counter()
         for _ in range(reps):
-            nns.knn_device(d_query, k, di, dd, eps, 0, radius)
+            index.knn_device(d_query, k, di, dd, eps, 0, radius)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / reps
         kms, launches = nabo.kernel_timing_collect()
@@ -126,7 +131,7 @@ def bench_lidar_variants(nns, cloud, query, d_query, radius, bucket, use_oracle:
         entry = {"ms_per_call": round(ms, 4), "queries_per_s": round(m / (ms / 1e3), 1),
                  "kernel_ms": round(kernel_ms, 4)}
         if tree is not None:
-            oi, od, touched, visits = tree.knn(query, k, eps, 0, radius, stats=True,
+            oi, od, touched, visits = tree.knn(query, k, eps, 0, radius, heap=heap, stats=True,
                                                threads=os.cpu_count() or 1)
             gi, gd = di.cpu().numpy(), dd.cpu().numpy()
             entry["parity_vs_oracle"] = bool(np.array_equal(gi, oi) and
@@ -136,8 +141,10 @@ def bench_lidar_variants(nns, cloud, query, d_query, radius, bucket, use_oracle:
             entry["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1),
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": round(achieved / HBM_PEAK_GBS, 4)}
-        out[f"k{k}_eps{eps}"] = entry
-        log(f"lidar k={k} eps={eps}: {entry}")
+        name = f"k{k}_eps{eps}" + ("_tree_heap" if heap == "stl" else "")
+        out[name] = entry
+        log(f"lidar {name}: {entry}")
+    stl.close()
     return out
 
 

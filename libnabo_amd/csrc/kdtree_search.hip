@@ -119,7 +119,10 @@ struct SearchParams {
   int64_t stack_lanes;
   int allow_self;
   int sort_results;  // IndexHeapSTL only: sort_heap before the output
-  int heap_lds;      // IndexHeapSTL in LDS: first uint2 of the heaps in LDS
+  int heap_lds;      // LOG kernels: first uint2 of the IndexHeapSTL heaps in LDS
+  uint2* log;          // LOG kernels: admission logs, [entry][grid lane]
+  uint32_t log_cap;    // entries per lane
+  uint32_t log_lanes;  // lanes of the grid
 };
 
 // Sorted insertion of (value, index): entries greater than value move up by
@@ -144,16 +147,16 @@ __device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], fl
 }
 
 // The k-best of a query: for K > 0, IndexHeapBruteForceVector as K sorted
-// entries in VGPRs (hd, hi); for K = 0 or -1, IndexHeapSTL as a heap in LDS
-// (K = 0) or in the query's output row (K = -1, when LDS is short). Both
-// admit the same distances (each drops one largest entry for the new one,
-// so their multisets of values, and their heads, are always equal): the
-// descent and its pruning are the same; only the entry kept among equal
-// distances and the output order differ.
+// entries in VGPRs (hd, hi); for K = -1, IndexHeapSTL as a heap in the
+// query's output row, updated at each admission (k above 64). Both admit the
+// same distances (each drops one largest entry for the new one, so their
+// multisets of values, and their heads, are always equal): the descent and
+// its pruning are the same; only the entry kept among equal distances and
+// the output order differ.
 template <int K>
 constexpr int kArr = K > 0 ? K : 1;
 template <int K>
-using HeapOf = StlHeap<typename std::conditional<K == 0, LdsHeapStore, RowHeapStore>::type>;
+using HeapOf = StlHeap<RowHeapStore>;
 
 template <int K>
 __device__ __forceinline__ void best_insert(float (&hd)[kArr<K>], int32_t (&hi)[kArr<K>],
@@ -163,6 +166,37 @@ __device__ __forceinline__ void best_insert(float (&hd)[kArr<K>], int32_t (&hi)[
   else
     heap_insert<K>(hd, hi, value, index);
 }
+
+// IndexHeapSTL from a log (LOG kernels, KDTREE_TREE_HEAP): the sorted k-best
+// in VGPRs decides admissions and pruning as for the linear heap, and every
+// admission is also appended to the lane's log in HBM; the log is replayed
+// into a binary heap in LDS when it fills and when the query ends. The
+// replay at the end runs for all lanes of the wave together, where heap
+// moves made at each admission would run for the few lanes that admit.
+// Both heaps take the same admissions in the same order, so the result is
+// the one of IndexHeapSTL updated on the fly.
+struct NoLog {
+  __device__ __forceinline__ void add(float, int32_t) {}
+};
+struct AdmitLog {
+  uint2* base;      // entry j of this lane at base[j * stride]
+  uint32_t stride;  // lanes of the grid
+  uint32_t cap;
+  uint32_t n;
+  StlHeap<LdsHeapStore> heap;
+  __device__ void flush() {
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint2 e = base[size_t(j) * stride];
+      heap.replace_head(int32_t(e.x), __uint_as_float(e.y));
+    }
+    n = 0;
+  }
+  __device__ __forceinline__ void add(float value, int32_t index) {
+    if (n == cap) flush();
+    base[size_t(n) * stride] = make_uint2(uint32_t(index), __float_as_uint(value));
+    ++n;
+  }
+};
 
 // Admission tests of a bucket entry (kdtree_cpu.cpp:392-394) and of a far
 // child (:415-417 / :433-435). With epsilon = 0 (EXACT) the far-child test
@@ -221,11 +255,11 @@ __device__ __forceinline__ float point_dist(const SearchParams& a, const Vec<DIM
 
 // Scans one leaf bucket: loads up to kLeafBatch entries at once, then tests
 // them in bucket order (kdtree_cpu.cpp:380-398).
-template <int DIM, int K, bool EXACT>
+template <int DIM, int K, bool EXACT, class Log>
 __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>& q, uint32_t count,
                                           int64_t start, const Admit<EXACT>& adm,
                                           float (&hd)[kArr<K>], int32_t (&hi)[kArr<K>],
-                                          HeapOf<K>& heap) {
+                                          HeapOf<K>& heap, Log& lg) {
   if constexpr (DIM > 0 && DIM <= 3) {
     const float4* pts4 = reinterpret_cast<const float4*>(a.pts) + start;
     for (uint32_t j0 = 0; j0 < count; j0 += kLeafBatch) {
@@ -245,8 +279,10 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
             dist = dist + diff * diff;
           }
           if (adm.point(dist, K <= 0 ? heap.head : hd[kArr<K> - 1]) &&
-              (a.allow_self || dist > FLT_EPSILON))
+              (a.allow_self || dist > FLT_EPSILON)) {
             best_insert<K>(hd, hi, heap, dist, __float_as_int(c[DIM]));
+            lg.add(dist, __float_as_int(c[DIM]));
+          }
         }
       }
     }
@@ -255,8 +291,10 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
       int32_t index;
       const float dist = point_dist<DIM>(a, q, start + j, &index);
       if (adm.point(dist, K <= 0 ? heap.head : hd[kArr<K> - 1]) &&
-          (a.allow_self || dist > FLT_EPSILON))
+          (a.allow_self || dist > FLT_EPSILON)) {
         best_insert<K>(hd, hi, heap, dist, index);
+        lg.add(dist, index);
+      }
     }
   }
 }
@@ -270,7 +308,7 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
 // another still has work.
 // Up to K = 16 three waves per SIMD fit without spills (at most 168 VGPRs),
 // one more than the LDS stack of a 28-level tree lets a CU hold.
-template <int DIM, int K, bool LDS_STACK, bool EXACT>
+template <int DIM, int K, bool LDS_STACK, bool EXACT, bool LOG = false>
 __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(SearchParams a) {
   // Per-lane stack: in LDS when the tree is shallow enough ([slot][lane] of
   // this workgroup), else in HBM ([slot][lane] over the grid).
@@ -342,9 +380,16 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
     float hd[kArr<K>];
     int32_t hi[kArr<K>];
     HeapOf<K> heap;
-    if constexpr (K == 0) {
-      heap.reset(LdsHeapStore{lds_stack + a.heap_lds + threadIdx.x}, a.k);
-    } else if constexpr (K < 0) {
+    typename std::conditional<LOG, AdmitLog, NoLog>::type lg;
+    if constexpr (LOG) {
+      static_assert(K > 0, "logged IndexHeapSTL runs beside the sorted k-best");
+      lg.base = a.log + lane_id;
+      lg.stride = a.log_lanes;
+      lg.cap = a.log_cap;
+      lg.n = 0;
+      lg.heap.reset(LdsHeapStore{lds_stack + a.heap_lds + threadIdx.x}, a.k);
+    }
+    if constexpr (K < 0) {
       heap.reset(RowHeapStore{a.out_idx + qi * a.k, a.out_d2 + qi * a.k}, a.k);
     } else {
 #pragma unroll
@@ -398,7 +443,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
         rt = right_word(a.links, n);
       }
       const uint32_t count = nd.x >> a.dim_bits;
-      scan_leaf<DIM, K, EXACT>(a, q, count, int64_t(nd.y), adm, hd, hi, heap);
+      scan_leaf<DIM, K, EXACT>(a, q, count, int64_t(nd.y), adm, hd, hi, heap, lg);
       touched += count;
 
       // Pop: kPopBatch entries per round trip to the stack. Restore entries
@@ -448,7 +493,11 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
     if (lane == 0) claimed = atomicAdd(&a.work[owner], 1u);
     int32_t* oi = a.out_idx + qi * a.k;
     float* od = a.out_d2 + qi * a.k;
-    if constexpr (K <= 0) {
+    if constexpr (LOG) {
+      lg.flush();
+      if (a.sort_results) lg.heap.sort();
+      lg.heap.write(oi, od);
+    } else if constexpr (K <= 0) {
       if (a.sort_results) heap.sort();
       heap.write(oi, od);
     } else if (vec_out) {
@@ -723,13 +772,14 @@ __global__ __launch_bounds__(kBlock) void k_sort_fine(const uint32_t* __restrict
 // The occupancy is cached per device and LDS size (tree depth, heap size).
 constexpr int kLdsSlotsMax = 65536 / (kSearchBlock * sizeof(uint2));  // 64 KiB per workgroup
 
-template <int DIM, int K, bool LDS, bool EXACT>
+template <int DIM, int K, bool LDS, bool EXACT, bool LOG>
 unsigned resident_grid(size_t lds, int cus, int dev) {
   static std::atomic<int> cache[kMaxCachedDevices][kLdsSlotsMax + 1];
   const size_t slot = std::min<size_t>(lds / (kSearchBlock * sizeof(uint2)), kLdsSlotsMax);
   int per_cu = dev >= 0 && dev < kMaxCachedDevices ? cache[dev][slot].load() : 0;
   if (per_cu < 1) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kd_search<DIM, K, LDS, EXACT>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
+                                                     k_kd_search<DIM, K, LDS, EXACT, LOG>,
                                                      kSearchBlock, lds) != hipSuccess ||
         per_cu < 1)
       per_cu = 1;
@@ -739,73 +789,99 @@ unsigned resident_grid(size_t lds, int cus, int dev) {
 }
 
 // LDS of one search workgroup: the stacks (depth slots per lane, when the
-// tree is at most kLdsStackSlots deep), then for K = 0 the STL heaps
+// tree is at most kLdsStackSlots deep), then for LOG the IndexHeapSTL heaps
 // (k slots per lane).
-template <int K>
+template <bool LOG>
 size_t search_lds(int depth, int k) {
   const size_t stack = depth <= kLdsStackSlots ? size_t(depth) : 0;
-  const size_t heap = K == 0 ? size_t(k) : 0;
+  const size_t heap = LOG ? size_t(k) : 0;
   return (stack + heap) * kSearchBlock * sizeof(uint2);
 }
 
-template <int DIM, int K, bool EXACT>
+// Admission log entries per lane (LOG kernels); a full log is replayed and
+// emptied, so this bounds memory, not the number of admissions.
+uint32_t search_log_cap(int k) { return uint32_t(std::min(256, std::max(64, 8 * k))); }
+
+template <int DIM, int K, bool EXACT, bool LOG>
 void launch_search_as(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& stack,
                       hipStream_t s) {
-  const size_t lds = search_lds<K>(depth, p.k);
-  if (depth <= kLdsStackSlots) {
+  const size_t lds = search_lds<LOG>(depth, p.k);
+  const bool lds_stack = depth <= kLdsStackSlots;
+  const unsigned grid = lds_stack ? resident_grid<DIM, K, true, EXACT, LOG>(lds, cus, dev)
+                                  : resident_grid<DIM, K, false, EXACT, LOG>(lds, cus, dev);
+  const int64_t lanes = int64_t(grid) * kSearchBlock;
+  Scratch scratch(s);  // the LOG kernels' admission logs
+  Span<uint2> log;
+  if constexpr (LOG) {
+    p.log_cap = search_log_cap(p.k);
+    p.log_lanes = uint32_t(lanes);
+    scratch.want(log, size_t(p.log_cap) * size_t(lanes));
+    scratch.commit(dev);
+    p.log = log.p;
+  }
+  if (lds_stack) {
     p.heap_lds = depth * kSearchBlock;
-    const unsigned grid = resident_grid<DIM, K, true, EXACT>(lds, cus, dev);
     KernelTimer timer(s);
-    k_kd_search<DIM, K, true, EXACT><<<grid, kSearchBlock, lds, s>>>(p);
+    k_kd_search<DIM, K, true, EXACT, LOG><<<grid, kSearchBlock, lds, s>>>(p);
   } else {
     p.heap_lds = 0;
-    const unsigned grid = resident_grid<DIM, K, false, EXACT>(lds, cus, dev);
-    const int64_t lanes = int64_t(grid) * kSearchBlock;
     if (uint64_t(lanes) * uint64_t(depth + 1) * sizeof(uint2) >= (uint64_t(1) << 32))
       throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
     stack.alloc(size_t(lanes) * size_t(depth), s);
     p.stack = stack.p;
     p.stack_lanes = lanes;
     KernelTimer timer(s);
-    k_kd_search<DIM, K, false, EXACT><<<grid, kSearchBlock, lds, s>>>(p);
+    k_kd_search<DIM, K, false, EXACT, LOG><<<grid, kSearchBlock, lds, s>>>(p);
   }
 }
 
-template <int DIM, int K>
+template <int DIM, int K, bool LOG = false>
 void launch_search(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& stack,
                    hipStream_t s) {
   // epsilon = 0 (max_err2 == 1, so the multiply is exact) takes the
   // one-compare admission tests.
   if (p.max_err2 == 1.f)
-    launch_search_as<DIM, K, true>(p, cus, dev, depth, stack, s);
+    launch_search_as<DIM, K, true, LOG>(p, cus, dev, depth, stack, s);
   else
-    launch_search_as<DIM, K, false>(p, cus, dev, depth, stack, s);
+    launch_search_as<DIM, K, false, LOG>(p, cus, dev, depth, stack, s);
+}
+
+// Sorted k-best of K >= k entries (LOG: with the logged IndexHeapSTL).
+template <int DIM, bool LOG>
+void dispatch_sorted(const SearchParams& p, int cus, int dev, int depth, DevBuf<uint2>& stack,
+                     hipStream_t s) {
+  const int k = p.k;
+  if (k <= 1)
+    launch_search<DIM, 1, LOG>(p, cus, dev, depth, stack, s);
+  else if (k <= 2)
+    launch_search<DIM, 2, LOG>(p, cus, dev, depth, stack, s);
+  else if (k <= 4)
+    launch_search<DIM, 4, LOG>(p, cus, dev, depth, stack, s);
+  else if (k <= 8)
+    launch_search<DIM, 8, LOG>(p, cus, dev, depth, stack, s);
+  else if (k <= 16)
+    launch_search<DIM, 16, LOG>(p, cus, dev, depth, stack, s);
+  else if (k <= 32)
+    launch_search<DIM, 32, LOG>(p, cus, dev, depth, stack, s);
+  else if (LOG || k <= 64)  // the logged form runs up to k = 64
+    launch_search<DIM, 64, LOG>(p, cus, dev, depth, stack, s);
+  else if constexpr (!LOG)
+    launch_search<DIM, 128, LOG>(p, cus, dev, depth, stack, s);
 }
 
 template <int DIM>
 void dispatch_k(const SearchParams& p, bool stl, int cus, int dev, int depth,
                 DevBuf<uint2>& stack, hipStream_t s) {
-  const int k = p.k;
-  if (stl && search_lds<0>(depth, k) <= size_t(kLdsSlotsMax) * kSearchBlock * sizeof(uint2))
-    launch_search<DIM, 0>(p, cus, dev, depth, stack, s);
+  // IndexHeapSTL: the logged form up to k = 64 (its heaps beside the stacks
+  // in LDS), else the heap in the output rows.
+  static_assert((kLdsStackSlots + 64) * kSearchBlock * sizeof(uint2) <= 65536,
+                "stacks and 64-entry heaps fit in 64 KiB of LDS");
+  if (stl && p.k <= 64)
+    dispatch_sorted<DIM, true>(p, cus, dev, depth, stack, s);
   else if (stl)
     launch_search<DIM, -1>(p, cus, dev, depth, stack, s);
-  else if (k <= 1)
-    launch_search<DIM, 1>(p, cus, dev, depth, stack, s);
-  else if (k <= 2)
-    launch_search<DIM, 2>(p, cus, dev, depth, stack, s);
-  else if (k <= 4)
-    launch_search<DIM, 4>(p, cus, dev, depth, stack, s);
-  else if (k <= 8)
-    launch_search<DIM, 8>(p, cus, dev, depth, stack, s);
-  else if (k <= 16)
-    launch_search<DIM, 16>(p, cus, dev, depth, stack, s);
-  else if (k <= 32)
-    launch_search<DIM, 32>(p, cus, dev, depth, stack, s);
-  else if (k <= 64)
-    launch_search<DIM, 64>(p, cus, dev, depth, stack, s);
   else
-    launch_search<DIM, 128>(p, cus, dev, depth, stack, s);
+    dispatch_sorted<DIM, false>(p, cus, dev, depth, stack, s);
 }
 
 uint32_t bits_needed(uint64_t v) {
