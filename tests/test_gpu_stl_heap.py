@@ -131,3 +131,29 @@ def test_brute_force_stl_log_overflow(nabo, oracle):
         gi, gd = bf.knn(query, k, 0.0, flags)
         oi, od = oracle.bf_knn(cloud, query, k, flags=flags, threads=8)
         _exact(gi, gd, oi, od)
+
+
+@pytest.mark.parametrize("k,flags", [(1, 0), (5, SORT), (16, 0), (40, SORT), (64, 0)])
+def test_tree_heap_admission_log_refills(nabo, oracle, k, flags):
+    # One bucket (n <= bucketSize) scanned in index order, with the points
+    # met in order of decreasing distance from the first queries: every
+    # point is admitted, so each lane's admission log (at most 256 entries)
+    # fills and is replayed into the heap many times during the query, at
+    # different moments for different lanes of a wave.
+    rng = np.random.default_rng(35)
+    n = 3_000
+    cloud = np.zeros((n, 3), np.float32)
+    cloud[:, 0] = 1000.0 - 0.1 * np.arange(n, dtype=np.float32)
+    cloud[:, 1] = rng.random(n).astype(np.float32) * 1e-3
+    query = np.concatenate([
+        np.array([[-1.0, 0.0, 0.0], [-2.0, 0.01, 0.0], [-50.0, 3.0, 1.0]], np.float32),
+        (rng.random((200, 3)) * [1000.0, 1.0, 1.0]).astype(np.float32)])
+    print(f"case kd stl log refills k={k} flags={flags}", flush=True)
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_TREE_HEAP, 3,
+                                      nabo.CreationOptionFlags.TOUCH_STATISTICS,
+                                      {"bucketSize": 4_000})
+    gi, gd = nns.knn(query, k, 0.0, flags)
+    oi, od, otouch, _ = oracle.kdtree(cloud, bucket_size=4_000).knn(
+        query, k, 0.0, flags, heap="stl", threads=8, stats=True)
+    _exact(gi, gd, oi, od)
+    assert nns.last_touched == int(otouch.sum())
