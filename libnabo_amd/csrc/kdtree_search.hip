@@ -650,9 +650,9 @@ __global__ void k_zero_counters(uint32_t* counters, int64_t n) {
 //   4. k_sort_fine: one workgroup per coarse bucket places its queries by
 //      the key's low fine_bits.
 constexpr int kSortTile = 4096;
-constexpr int kHistBlock = 1024;  // k_sort_hist: 4 queries per thread and tile
-constexpr int kCoarseBits = 12;
-constexpr int kFineBitsMax = 9;
+constexpr int kHistBlock = 1024;  // sort kernels: 4 queries per thread and tile
+constexpr int kCoarseBits = 10;
+constexpr int kFineBitsMax = 11;
 
 // MDIM = 2 / 3: keys are the queries' Hilbert indices (batched trees: with
 // the cloud id above, boxes per cloud), computed here and stored; MDIM = 0:
@@ -700,7 +700,7 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_hist(uint32_t* __restrict__
   for (int b = threadIdx.x; b < coarse_bins; b += kHistBlock) counts[int64_t(b) * tiles + tile] = h[b];
 }
 
-__global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint32_t* __restrict__ keys,
+__global__ __launch_bounds__(kHistBlock) void k_sort_scatter(const uint32_t* __restrict__ keys,
                                                          int64_t m, int fine_bits,
                                                          int coarse_bins, int64_t tiles,
                                                          const uint32_t* __restrict__ starts,
@@ -708,11 +708,12 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint32_t* __restr
                                                          int32_t* __restrict__ fidx) {
   __shared__ uint32_t cur[1 << kCoarseBits];
   const int64_t tile = blockIdx.x;
-  for (int b = threadIdx.x; b < coarse_bins; b += kBlock) cur[b] = starts[int64_t(b) * tiles + tile];
+  for (int b = threadIdx.x; b < coarse_bins; b += kHistBlock)
+    cur[b] = starts[int64_t(b) * tiles + tile];
   __syncthreads();
   const uint32_t fine_mask = (1u << fine_bits) - 1u;
   const int64_t end = min(m, (tile + 1) * kSortTile);
-  for (int64_t q = tile * kSortTile + threadIdx.x; q < end; q += kBlock) {
+  for (int64_t q = tile * kSortTile + threadIdx.x; q < end; q += kHistBlock) {
     const uint32_t key = keys[q];
     const uint32_t pos = atomicAdd(&cur[key >> fine_bits], 1u);
     fkeys[pos] = key & fine_mask;
@@ -720,27 +721,27 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint32_t* __restr
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_sort_fine(const uint32_t* __restrict__ fkeys,
+__global__ __launch_bounds__(kHistBlock) void k_sort_fine(const uint32_t* __restrict__ fkeys,
                                                       const int32_t* __restrict__ fidx,
                                                       const uint32_t* __restrict__ starts,
                                                       int64_t tiles, int64_t m, int fine_bits,
                                                       int coarse_bins, int32_t* __restrict__ order) {
   __shared__ uint32_t h[1 << kFineBitsMax];
-  __shared__ uint32_t wave_sum[kBlock / kWave];
+  __shared__ uint32_t wave_sum[kHistBlock / kWave];
   const int bin = blockIdx.x;
   const uint32_t begin = starts[int64_t(bin) * tiles];
   const uint32_t end = bin + 1 < coarse_bins ? starts[int64_t(bin + 1) * tiles] : uint32_t(m);
   if (end - begin <= 1 || fine_bits == 0) {
-    for (uint32_t e = begin + threadIdx.x; e < end; e += kBlock) order[e] = fidx[e];
+    for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock) order[e] = fidx[e];
     return;
   }
   const int bins = 1 << fine_bits;
-  for (int b = threadIdx.x; b < bins; b += kBlock) h[b] = 0;
+  for (int b = threadIdx.x; b < bins; b += kHistBlock) h[b] = 0;
   __syncthreads();
-  for (uint32_t e = begin + threadIdx.x; e < end; e += kBlock) atomicAdd(&h[fkeys[e]], 1u);
+  for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock) atomicAdd(&h[fkeys[e]], 1u);
   __syncthreads();
   // Exclusive scan of the bins: each thread owns bins [t * per, (t + 1) * per).
-  const int per = (bins + kBlock - 1) / kBlock;
+  const int per = (bins + kHistBlock - 1) / kHistBlock;
   const int b0 = threadIdx.x * per;
   uint32_t sum = 0;
   for (int b = b0; b < min(bins, b0 + per); ++b) sum += h[b];
@@ -763,7 +764,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_fine(const uint32_t* __restrict
     run += c;
   }
   __syncthreads();
-  for (uint32_t e = begin + threadIdx.x; e < end; e += kBlock)
+  for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock)
     order[atomicAdd(&h[fkeys[e]], 1u)] = fidx[e];
 }
 
@@ -1035,9 +1036,9 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     size_t bytes = scan_bytes;
     NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, counts.p, starts.p, uint32_t(0),
                                      size_t(ncounts), rocprim::plus<uint32_t>(), s));
-    k_sort_scatter<<<unsigned(tiles), kBlock, 0, s>>>(keys.p, a.m, fine_bits, coarse_bins, tiles,
+    k_sort_scatter<<<unsigned(tiles), kHistBlock, 0, s>>>(keys.p, a.m, fine_bits, coarse_bins, tiles,
                                                        starts.p, fkeys.p, fidx.p);
-    k_sort_fine<<<unsigned(coarse_bins), kBlock, 0, s>>>(fkeys.p, fidx.p, starts.p, tiles, a.m,
+    k_sort_fine<<<unsigned(coarse_bins), kHistBlock, 0, s>>>(fkeys.p, fidx.p, starts.p, tiles, a.m,
                                                          fine_bits, coarse_bins, order.p);
     NH_LAUNCH_CHECK();
     p.order = order.p;
