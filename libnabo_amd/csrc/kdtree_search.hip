@@ -651,8 +651,8 @@ __global__ void k_zero_counters(uint32_t* counters, int64_t n) {
 //      the key's low fine_bits.
 constexpr int kSortTile = 4096;
 constexpr int kHistBlock = 1024;  // sort kernels: 4 queries per thread and tile
-constexpr int kCoarseBits = 10;
-constexpr int kFineBitsMax = 11;
+constexpr int kCoarseBits = 11;
+constexpr int kFineBitsMax = 10;
 
 // MDIM = 2 / 3: keys are the queries' Hilbert indices (batched trees: with
 // the cloud id above, boxes per cloud), computed here and stored; MDIM = 0:
