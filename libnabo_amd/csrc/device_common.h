@@ -57,6 +57,12 @@ struct LdsHeapStore {  // entry j at base[j * kWave] (slot-major over a wave's l
   __device__ __forceinline__ uint2 get(int j) const { return base[j * kWave]; }
   __device__ __forceinline__ void set(int j, uint2 e) { base[j * kWave] = e; }
 };
+struct StridedHeapStore {  // entry j at base[j * stride] (slot-major over a grid's lanes)
+  uint2* base;
+  uint32_t stride;
+  __device__ __forceinline__ uint2 get(int j) const { return base[size_t(j) * stride]; }
+  __device__ __forceinline__ void set(int j, uint2 e) { base[size_t(j) * stride] = e; }
+};
 struct RowHeapStore {  // entry j in the query's output row: idx[j], val[j]
   int32_t* idx;
   float* val;

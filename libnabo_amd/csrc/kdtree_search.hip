@@ -119,8 +119,8 @@ struct SearchParams {
   int64_t stack_lanes;
   int allow_self;
   int sort_results;  // IndexHeapSTL only: sort_heap before the output
-  int heap_lds;      // LOG kernels: first uint2 of the IndexHeapSTL heaps in LDS
   uint2* log;          // LOG kernels: admission logs, [entry][grid lane]
+  uint2* log_heap;     // LOG kernels: heaps of lanes whose log filled, [entry][grid lane]
   uint32_t log_cap;    // entries per lane
   uint32_t log_lanes;  // lanes of the grid
 };
@@ -169,12 +169,14 @@ __device__ __forceinline__ void best_insert(float (&hd)[kArr<K>], int32_t (&hi)[
 
 // IndexHeapSTL from a log (LOG kernels, KDTREE_TREE_HEAP): the sorted k-best
 // in VGPRs decides admissions and pruning as for the linear heap, and every
-// admission is also appended to the lane's log in HBM; the log is replayed
-// into a binary heap in LDS when it fills and when the query ends. The
-// replay at the end runs for all lanes of the wave together, where heap
-// moves made at each admission would run for the few lanes that admit.
-// Both heaps take the same admissions in the same order, so the result is
-// the one of IndexHeapSTL updated on the fly.
+// admission is also appended to the lane's log in HBM. When the query ends
+// its stack is empty, and the log is replayed into a binary heap held in the
+// stack's LDS slots. The replay runs for all lanes of the wave together,
+// where heap moves made at each admission would run for the few lanes that
+// admit, and the heap takes no LDS beyond the stacks. A lane whose log fills
+// before the query ends moves its heap to HBM (log_heap) and replays the log
+// there, then and at the end. Both heaps take the same admissions in the
+// same order, so the result is the one of IndexHeapSTL updated on the fly.
 struct NoLog {
   __device__ __forceinline__ void add(float, int32_t) {}
 };
@@ -183,8 +185,10 @@ struct AdmitLog {
   uint32_t stride;  // lanes of the grid
   uint32_t cap;
   uint32_t n;
-  StlHeap<LdsHeapStore> heap;
-  __device__ void flush() {
+  bool spilled;                      // the log filled: the heap is in HBM
+  StlHeap<StridedHeapStore> spill;   // that heap (store and k set up front)
+  template <class Heap>
+  __device__ void replay(Heap& heap) {
     for (uint32_t j = 0; j < n; ++j) {
       const uint2 e = base[size_t(j) * stride];
       heap.replace_head(int32_t(e.x), __uint_as_float(e.y));
@@ -192,7 +196,13 @@ struct AdmitLog {
     n = 0;
   }
   __device__ __forceinline__ void add(float value, int32_t index) {
-    if (n == cap) flush();
+    if (n == cap) {
+      if (!spilled) {
+        spilled = true;
+        spill.reset(spill.st, spill.k);
+      }
+      replay(spill);
+    }
     base[size_t(n) * stride] = make_uint2(uint32_t(index), __float_as_uint(value));
     ++n;
   }
@@ -387,7 +397,9 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
       lg.stride = a.log_lanes;
       lg.cap = a.log_cap;
       lg.n = 0;
-      lg.heap.reset(LdsHeapStore{lds_stack + a.heap_lds + threadIdx.x}, a.k);
+      lg.spilled = false;
+      lg.spill.st = StridedHeapStore{a.log_heap + lane_id, a.log_lanes};
+      lg.spill.k = a.k;
     }
     if constexpr (K < 0) {
       heap.reset(RowHeapStore{a.out_idx + qi * a.k, a.out_d2 + qi * a.k}, a.k);
@@ -494,9 +506,18 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
     int32_t* oi = a.out_idx + qi * a.k;
     float* od = a.out_d2 + qi * a.k;
     if constexpr (LOG) {
-      lg.flush();
-      if (a.sort_results) lg.heap.sort();
-      lg.heap.write(oi, od);
+      if (lg.spilled) {
+        lg.replay(lg.spill);
+        if (a.sort_results) lg.spill.sort();
+        lg.spill.write(oi, od);
+      } else {
+        // the stack is empty: its first k LDS slots of this lane hold the heap
+        StlHeap<LdsHeapStore> heap_lds;
+        heap_lds.reset(LdsHeapStore{lds_stack + threadIdx.x}, a.k);
+        lg.replay(heap_lds);
+        if (a.sort_results) heap_lds.sort();
+        heap_lds.write(oi, od);
+      }
     } else if constexpr (K <= 0) {
       if (a.sort_results) heap.sort();
       heap.write(oi, od);
@@ -790,13 +811,13 @@ unsigned resident_grid(size_t lds, int cus, int dev) {
 }
 
 // LDS of one search workgroup: the stacks (depth slots per lane, when the
-// tree is at most kLdsStackSlots deep), then for LOG the IndexHeapSTL heaps
-// (k slots per lane).
+// tree is at most kLdsStackSlots deep); for LOG at least k slots per lane,
+// which hold the IndexHeapSTL heaps once the stacks are empty.
 template <bool LOG>
 size_t search_lds(int depth, int k) {
   const size_t stack = depth <= kLdsStackSlots ? size_t(depth) : 0;
-  const size_t heap = LOG ? size_t(k) : 0;
-  return (stack + heap) * kSearchBlock * sizeof(uint2);
+  const size_t slots = LOG ? std::max(stack, size_t(k)) : stack;
+  return slots * kSearchBlock * sizeof(uint2);
 }
 
 // Admission log entries per lane (LOG kernels); a full log is replayed and
@@ -811,21 +832,21 @@ void launch_search_as(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>
   const unsigned grid = lds_stack ? resident_grid<DIM, K, true, EXACT, LOG>(lds, cus, dev)
                                   : resident_grid<DIM, K, false, EXACT, LOG>(lds, cus, dev);
   const int64_t lanes = int64_t(grid) * kSearchBlock;
-  Scratch scratch(s);  // the LOG kernels' admission logs
-  Span<uint2> log;
+  Scratch scratch(s);  // the LOG kernels' admission logs and spilled heaps
+  Span<uint2> log, log_heap;
   if constexpr (LOG) {
     p.log_cap = search_log_cap(p.k);
     p.log_lanes = uint32_t(lanes);
     scratch.want(log, size_t(p.log_cap) * size_t(lanes));
+    scratch.want(log_heap, size_t(p.k) * size_t(lanes));
     scratch.commit(dev);
     p.log = log.p;
+    p.log_heap = log_heap.p;
   }
   if (lds_stack) {
-    p.heap_lds = depth * kSearchBlock;
     KernelTimer timer(s);
     k_kd_search<DIM, K, true, EXACT, LOG><<<grid, kSearchBlock, lds, s>>>(p);
   } else {
-    p.heap_lds = 0;
     if (uint64_t(lanes) * uint64_t(depth + 1) * sizeof(uint2) >= (uint64_t(1) << 32))
       throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
     stack.alloc(size_t(lanes) * size_t(depth), s);
@@ -873,10 +894,10 @@ void dispatch_sorted(const SearchParams& p, int cus, int dev, int depth, DevBuf<
 template <int DIM>
 void dispatch_k(const SearchParams& p, bool stl, int cus, int dev, int depth,
                 DevBuf<uint2>& stack, hipStream_t s) {
-  // IndexHeapSTL: the logged form up to k = 64 (its heaps beside the stacks
-  // in LDS), else the heap in the output rows.
-  static_assert((kLdsStackSlots + 64) * kSearchBlock * sizeof(uint2) <= 65536,
-                "stacks and 64-entry heaps fit in 64 KiB of LDS");
+  // IndexHeapSTL: the logged form up to k = 64 (its heaps in the stacks'
+  // LDS slots), else the heap in the output rows.
+  static_assert(std::max(kLdsStackSlots, 64) * kSearchBlock * sizeof(uint2) <= 65536,
+                "64-entry heaps fit in 64 KiB of LDS");
   if (stl && p.k <= 64)
     dispatch_sorted<DIM, true>(p, cus, dev, depth, stack, s);
   else if (stl)
