@@ -16,7 +16,10 @@
  * Device memory: besides each index's own arrays, builds and knn calls take
  * their temporaries from a per-device pool of scratch blocks that the
  * library keeps between calls (at most eight blocks of at most 1 GiB per
- * device, until the process ends).
+ * device, until the process ends). A kd-tree knn call with the IndexHeapSTL
+ * heap (NABO_HIP_HEAP_STL) and k <= 64 also takes 8 * (min(256, max(64, 8k))
+ * + k) bytes per resident search lane (about 0.4 GB at k = 64 on MI355X)
+ * for its admission logs.
  */
 #ifndef NABO_HIP_H
 #define NABO_HIP_H
