@@ -18,7 +18,7 @@
  * library keeps between calls (at most eight blocks of at most 1 GiB per
  * device, until the process ends). A kd-tree knn call with the IndexHeapSTL
  * heap (NABO_HIP_HEAP_STL) and k <= 64 also takes 8 * (min(256, max(64, 8k))
- * + k) bytes per resident search lane (about 0.4 GB at k = 64 on MI355X)
+ * + k) bytes per resident search lane (about 0.2 GB for k = 16 to 64 on MI355X)
  * for its admission logs.
  */
 #ifndef NABO_HIP_H
