@@ -1,7 +1,7 @@
 /* nabo_hip.h - C ABI of the MI355X (gfx950) kNN library.
  *
  * Plain C types only: pointers, sizes and status codes. Host C++
- * (include/nabo_amd/nabo.h), Python (libnabo_amd) and any FFI bind to these.
+ * (include/nabo/nabo.h), Python (libnabo_amd) and any FFI bind to these.
  *
  * Layout conventions are libnabo's (nabo/nabo.h:260-272): a cloud or query
  * set is a column-major `rows x count` float matrix, one point per column,
