@@ -887,8 +887,12 @@ void dispatch_sorted(const SearchParams& p, int cus, int dev, int depth, DevBuf<
     launch_search<DIM, 32, LOG>(p, cus, dev, depth, stack, s);
   else if (LOG || k <= 64)  // the logged form runs up to k = 64
     launch_search<DIM, 64, LOG>(p, cus, dev, depth, stack, s);
-  else if constexpr (!LOG)
-    launch_search<DIM, 128, LOG>(p, cus, dev, depth, stack, s);
+  else if constexpr (!LOG) {
+    if (k <= 96)
+      launch_search<DIM, 96, LOG>(p, cus, dev, depth, stack, s);
+    else
+      launch_search<DIM, 128, LOG>(p, cus, dev, depth, stack, s);
+  }
 }
 
 template <int DIM>

@@ -122,7 +122,7 @@ def test_knn_per_query_radii(nabo, oracle, rng):
     assert_bit_exact(gi, gd, oi, od, "radii")
 
 
-@pytest.mark.parametrize("k", [32, 33, 64, 100])
+@pytest.mark.parametrize("k", [32, 33, 64, 65, 96, 100, 128])
 def test_knn_large_k(nabo, oracle, k):
     cloud, query = datasets.uniform_cube(8_000, 1_500, seed=11)
     nns = make(nabo, cloud, 8)
