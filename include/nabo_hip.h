@@ -17,8 +17,8 @@
  * their temporaries from a per-device pool of scratch blocks that the
  * library keeps between calls (at most eight blocks of at most 1 GiB per
  * device, until the process ends). A kd-tree knn call with the IndexHeapSTL
- * heap (NABO_HIP_HEAP_STL) and k <= 64 also takes 8 * (min(256, max(64, 8k))
- * + k) bytes per resident search lane (about 0.2 GB for k = 16 to 64 on MI355X)
+ * heap (NABO_HIP_HEAP_STL) and k <= 96 also takes 8 * (min(256, max(64, 8k))
+ * + k) bytes per resident search lane (about 0.2 GB for k = 16 to 96 on MI355X)
  * for its admission logs.
  */
 #ifndef NABO_HIP_H

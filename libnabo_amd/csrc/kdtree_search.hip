@@ -131,24 +131,38 @@ struct SearchParams {
 // value, else min(hd[j], value): the median of (hd[j-1], hd[j], value).
 // Indices follow with two selects per entry; both compares are taken on
 // the old distances, so the update has no branch.
+// For K = 128 each entry's own select is taken inside the descending loop
+// (entry j still holds its old value there), so that no K-entry temporary
+// is live beside the k-best and the kernel fits its registers.
 template <int K>
 __device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], float value,
                                             int32_t index) {
-  int32_t moved[K];
+  if constexpr (K >= 128) {
 #pragma unroll
-  for (int j = 0; j < K; ++j) moved[j] = hd[j] > value ? index : hi[j];
+    for (int j = K - 1; j > 0; --j) {
+      const int32_t own = hd[j] > value ? index : hi[j];
+      hi[j] = hd[j - 1] > value ? hi[j - 1] : own;
+      hd[j] = __builtin_amdgcn_fmed3f(hd[j - 1], hd[j], value);
+    }
+    hi[0] = hd[0] > value ? index : hi[0];
+    hd[0] = fminf(hd[0], value);
+  } else {
+    int32_t moved[K];
 #pragma unroll
-  for (int j = K - 1; j > 0; --j) {
-    hi[j] = hd[j - 1] > value ? hi[j - 1] : moved[j];
-    hd[j] = __builtin_amdgcn_fmed3f(hd[j - 1], hd[j], value);
+    for (int j = 0; j < K; ++j) moved[j] = hd[j] > value ? index : hi[j];
+#pragma unroll
+    for (int j = K - 1; j > 0; --j) {
+      hi[j] = hd[j - 1] > value ? hi[j - 1] : moved[j];
+      hd[j] = __builtin_amdgcn_fmed3f(hd[j - 1], hd[j], value);
+    }
+    hi[0] = moved[0];
+    hd[0] = fminf(hd[0], value);
   }
-  hi[0] = moved[0];
-  hd[0] = fminf(hd[0], value);
 }
 
 // The k-best of a query: for K > 0, IndexHeapBruteForceVector as K sorted
 // entries in VGPRs (hd, hi); for K = -1, IndexHeapSTL as a heap in the
-// query's output row, updated at each admission (k above 64). Both admit the
+// query's output row, updated at each admission (k above 96). Both admit the
 // same distances (each drops one largest entry for the new one, so their
 // multisets of values, and their heads, are always equal): the descent and
 // its pruning are the same; only the entry kept among equal distances and
@@ -885,24 +899,24 @@ void dispatch_sorted(const SearchParams& p, int cus, int dev, int depth, DevBuf<
     launch_search<DIM, 16, LOG>(p, cus, dev, depth, stack, s);
   else if (k <= 32)
     launch_search<DIM, 32, LOG>(p, cus, dev, depth, stack, s);
-  else if (LOG || k <= 64)  // the logged form runs up to k = 64
+  else if (k <= 64)
     launch_search<DIM, 64, LOG>(p, cus, dev, depth, stack, s);
-  else if constexpr (!LOG) {
-    if (k <= 96)
-      launch_search<DIM, 96, LOG>(p, cus, dev, depth, stack, s);
-    else
-      launch_search<DIM, 128, LOG>(p, cus, dev, depth, stack, s);
-  }
+  else if (k <= 96)
+    launch_search<DIM, 96, LOG>(p, cus, dev, depth, stack, s);
+  else if constexpr (LOG)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED, "the logged IndexHeapSTL search runs up to k = 96");
+  else
+    launch_search<DIM, 128, LOG>(p, cus, dev, depth, stack, s);
 }
 
 template <int DIM>
 void dispatch_k(const SearchParams& p, bool stl, int cus, int dev, int depth,
                 DevBuf<uint2>& stack, hipStream_t s) {
-  // IndexHeapSTL: the logged form up to k = 64 (its heaps in the stacks'
+  // IndexHeapSTL: the logged form up to k = 96 (its heaps in the stacks'
   // LDS slots), else the heap in the output rows.
-  static_assert(std::max(kLdsStackSlots, 64) * kSearchBlock * sizeof(uint2) <= 65536,
-                "64-entry heaps fit in 64 KiB of LDS");
-  if (stl && p.k <= 64)
+  static_assert(std::max(kLdsStackSlots, 96) * kSearchBlock * sizeof(uint2) <= 65536,
+                "96-entry heaps fit in 64 KiB of LDS");
+  if (stl && p.k <= 96)
     dispatch_sorted<DIM, true>(p, cus, dev, depth, stack, s);
   else if (stl)
     launch_search<DIM, -1>(p, cus, dev, depth, stack, s);

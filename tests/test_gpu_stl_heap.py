@@ -46,7 +46,8 @@ def _exact(gi, gd, oi, od):
 @pytest.mark.parametrize("k,eps,flags,radius", [
     (16, 0.0, 0, np.inf), (16, 0.0, SORT, np.inf), (5, 0.1, 0, np.inf), (1, 0.0, 0, np.inf),
     (40, 0.0, SORT, np.inf), (10, 0.0, 0, 0.6), (7, 0.0, SELF, np.inf),
-    (128, 0.0, 0, np.inf),  # heaps too large for LDS: kept in the output rows
+    (80, 0.0, SORT, np.inf),
+    (128, 0.0, 0, np.inf),  # k above 96: the heap is kept in the output rows
 ])
 def test_tree_heap_matches_oracle(nabo, oracle, name, k, eps, flags, radius):
     cloud, query = KD_CASES[name]()
@@ -133,7 +134,7 @@ def test_brute_force_stl_log_overflow(nabo, oracle):
         _exact(gi, gd, oi, od)
 
 
-@pytest.mark.parametrize("k,flags", [(1, 0), (5, SORT), (16, 0), (40, SORT), (64, 0)])
+@pytest.mark.parametrize("k,flags", [(1, 0), (5, SORT), (16, 0), (40, SORT), (64, 0), (96, SORT)])
 def test_tree_heap_admission_log_refills(nabo, oracle, k, flags):
     # One bucket (n <= bucketSize) scanned in index order, with the points
     # met in order of decreasing distance from the first queries: every
