@@ -407,6 +407,8 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(S
     typename std::conditional<LOG, AdmitLog, NoLog>::type lg;
     if constexpr (LOG) {
       static_assert(K > 0, "logged IndexHeapSTL runs beside the sorted k-best");
+      static_assert(kSearchBlock == kWave,
+                    "the heap's LDS slots (kWave apart) are the stack's (kSearchBlock apart)");
       lg.base = a.log + lane_id;
       lg.stride = a.log_lanes;
       lg.cap = a.log_cap;
@@ -685,7 +687,7 @@ __global__ void k_zero_counters(uint32_t* counters, int64_t n) {
 //   4. k_sort_fine: one workgroup per coarse bucket places its queries by
 //      the key's low fine_bits.
 constexpr int kSortTile = 4096;
-constexpr int kHistBlock = 1024;  // sort kernels: 4 queries per thread and tile
+constexpr int kHistBlock = 1024;  // sort kernels' workgroups (hist, scatter: 4 queries per thread)
 constexpr int kCoarseBits = 11;
 constexpr int kFineBitsMax = 10;
 
