@@ -4,8 +4,8 @@ Queries outside the cloud's box (the root starts at rd = 0 with implicit
 bounds, kdtree_cpu.cpp:345-348), non-finite queries (NaN and infinite
 distances are never admitted, :392-394, and never open a far child,
 :415-417), a zero radius (dist <= 0 admits exact duplicates only), a cloud
-of one repeated point, k equal to the cloud size, and empty or one-query
-calls. Every case compares indices, dists2 and the touched count with the
+of one repeated point, k equal to the cloud size, empty or one-query calls,
+and a tree too deep for stacks in LDS. Every case compares indices, dists2 and the touched count with the
 oracle on the same tree.
 """
 import numpy as np
@@ -94,3 +94,23 @@ def test_empty_and_single_query(nabo, oracle):
     gi, gd = nns.knn(np.empty((0, 3), np.float32), 4)
     assert gi.shape == (0, 4) and gd.shape == (0, 4)
     check(nabo, oracle, cloud, query[:1], 4)
+
+
+@pytest.mark.parametrize("k", [1, 8, 40, 96])
+def test_deep_tree_stacks_in_hbm(nabo, oracle, k):
+    # Points at x = 2^-i split off one per level (midpoint cuts of the
+    # widest side): a tree of depth 89, beyond the 32 levels whose stacks fit
+    # in LDS, so the search keeps its stacks in HBM; the IndexHeapSTL form
+    # then holds its heap in LDS slots of its own.
+    rng = np.random.default_rng(46)
+    e = np.zeros((90, 3), np.float32)
+    e[:, 0] = np.float32(2.0) ** (-np.arange(90, dtype=np.float32))
+    e[:, 1] = rng.random(90).astype(np.float32) * np.float32(1e-30)
+    cloud = np.concatenate([e, (2 + rng.random((3_000, 3))).astype(np.float32)])
+    query = np.concatenate([e[::3] * np.float32(1.5), (rng.random((300, 3)) * 3).astype(np.float32)])
+    print(f"case deep tree k={k}", flush=True)
+    for heap in ("linear", "stl"):
+        check(nabo, oracle, cloud, query, k, bucket=2, heap=heap)
+        check(nabo, oracle, cloud, query, k, flags=SORT, radius=1.0, bucket=2, heap=heap)
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0, {"bucketSize": 2})
+    assert nns.info().depth == oracle.kdtree(cloud, bucket_size=2).depth > 32
