@@ -95,3 +95,31 @@ def test_batch_rejects_too_large_k(nabo):
     with pytest.raises(nabo.NaboError, match="minus 1"):
         b.knn(dq, qoff, 4)
     b.knn(dq, qoff, 4, 0.0, 1)
+
+
+@pytest.mark.parametrize("k,flags", [(6, 0), (12, 2)])
+def test_batch_knn_tree_heap(nabo, oracle, k, flags):
+    # heap "stl": every cloud's results in IndexHeapSTL order (KDTREE_TREE_HEAP)
+    import torch
+
+    rng = np.random.default_rng(7)
+    sizes = list(np.round(np.exp(rng.uniform(np.log(20), np.log(20000), size=40))).astype(int))
+    clouds, queries = make_batch(rng, sizes)
+    print(f"case batch knn tree heap k={k} flags={flags}", flush=True)
+    pts = torch.from_numpy(np.concatenate(clouds)).cuda()
+    b = nabo.BatchSearch(pts, offsets_of(clouds), 3, nabo.CreationOptionFlags.TOUCH_STATISTICS,
+                         {"bucketSize": 8, "heap": "stl"})
+    dq = torch.from_numpy(np.concatenate(queries)).cuda()
+    qoff = torch.from_numpy(offsets_of(queries)).cuda()
+    touched = torch.zeros(1, dtype=torch.int64, device="cuda")
+    gi, gd = b.knn(dq, qoff, k, 0.0, flags, 2.0, touched=touched)
+    gi, gd = gi.cpu().numpy(), gd.cpu().numpy()
+    qo = offsets_of(queries)
+    total = 0
+    for c, (cl, q) in enumerate(zip(clouds, queries)):
+        oi, od, t, _ = oracle.kdtree(cl, bucket_size=8).knn(q, k, 0.0, flags, 2.0, heap="stl",
+                                                            stats=True)
+        total += int(t.sum())
+        assert np.array_equal(gi[qo[c]:qo[c + 1]], oi), f"cloud {c} indices"
+        assert np.array_equal(gd[qo[c]:qo[c + 1]].view(np.uint32), od.view(np.uint32)), c
+    assert int(touched.item()) == total
