@@ -24,8 +24,14 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// Every entry point first clears the thread's HIP error state: the launch
+// checks read it with hipGetLastError, which also returns an error that an
+// earlier HIP call of the caller's, or of another library on the thread,
+// left unread (a graph library's teardown, for instance), and that error is
+// not this call's.
 template <typename F>
 int guarded(F&& f) {
+  (void)hipGetLastError();
   try {
     f();
     g_last_error.clear();
@@ -310,7 +316,10 @@ void Scratch::commit(int dev) {
     for (int i = 0; i < int(g_scratch.size()); ++i) {
       const Block& b = g_scratch[i];
       if (b.dev != dev || b.bytes < total) continue;
-      if (b.last != s_ && hipEventQuery(b.done) != hipSuccess) continue;
+      if (b.last != s_ && hipEventQuery(b.done) != hipSuccess) {
+        (void)hipGetLastError();  // not ready: not an error of this call
+        continue;
+      }
       if (best < 0) {
         best = i;
         continue;

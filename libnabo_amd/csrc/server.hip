@@ -205,6 +205,7 @@ struct nabo_hip_server {
 namespace {
 template <typename F>
 int server_guard(F&& f) {
+  (void)hipGetLastError();  // an error left unread by an earlier call is not this call's
   try {
     f();
     nabo_hip::set_last_error("");
@@ -364,6 +365,7 @@ bool nabo_hip_server::retire() {
   for (auto& s : slots) {
     if (!s.busy) continue;
     const hipError_t q = hipEventQuery(s.event);
+    (void)hipGetLastError();  // q carries the result; keep the launch checks clean
     if (q == hipErrorNotReady) continue;
     const double t = now_ms();
     for (Request* r : s.reqs) {

@@ -114,3 +114,20 @@ def test_deep_tree_stacks_in_hbm(nabo, oracle, k):
         check(nabo, oracle, cloud, query, k, flags=SORT, radius=1.0, bucket=2, heap=heap)
     nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0, {"bucketSize": 2})
     assert nns.info().depth == oracle.kdtree(cloud, bucket_size=2).depth > 32
+
+
+def test_unread_hip_error_of_another_call_is_not_reported(nabo, oracle):
+    # A HIP error that an earlier call on the thread left unread (here an
+    # invalid device ordinal, which changes nothing) is not the library's:
+    # builds and searches still succeed.
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime the library is linked against
+    cloud, query = datasets.uniform_cube(3_000, 200, seed=47)
+    print("case unread HIP error", flush=True)
+    assert hip.hipSetDevice(ctypes.c_int(1 << 20)) != 0
+    nns = nabo.NearestNeighbourSearch(cloud)
+    assert hip.hipSetDevice(ctypes.c_int(1 << 20)) != 0
+    gi, gd = nns.knn(query, 4)
+    oi, od = oracle.kdtree(cloud).knn(query, 4, threads=8)
+    assert np.array_equal(gi, oi) and np.array_equal(gd.view(np.uint32), od.view(np.uint32))
