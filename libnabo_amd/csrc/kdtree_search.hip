@@ -131,13 +131,15 @@ struct SearchParams {
 // value, else min(hd[j], value): the median of (hd[j-1], hd[j], value).
 // Indices follow with two selects per entry; both compares are taken on
 // the old distances, so the update has no branch.
-// For K = 128 each entry's own select is taken inside the descending loop
-// (entry j still holds its old value there), so that no K-entry temporary
-// is live beside the k-best and the kernel fits its registers.
+// For K = 32, 64 and 128 each entry's own select is taken inside the
+// descending loop (entry j still holds its old value there), so that no
+// K-entry temporary is live beside the k-best: K = 128 then fits its
+// registers without spills, K = 64 two waves per SIMD, K = 32 three (for
+// K = 96 the first form measured faster).
 template <int K>
 __device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], float value,
                                             int32_t index) {
-  if constexpr (K >= 128) {
+  if constexpr (K == 32 || K == 64 || K >= 128) {
 #pragma unroll
     for (int j = K - 1; j > 0; --j) {
       const int32_t own = hd[j] > value ? index : hi[j];
@@ -330,10 +332,12 @@ __device__ __forceinline__ void scan_leaf(const SearchParams& a, const Vec<DIM>&
 // take chunks from it through an atomic counter; a wave whose XCD's eighth
 // is exhausted takes chunks from the other eighths, so no XCD idles while
 // another still has work.
-// Up to K = 16 three waves per SIMD fit without spills (at most 168 VGPRs),
-// one more than the LDS stack of a 28-level tree lets a CU hold.
+// Up to K = 32 three waves per SIMD fit without spills (at most 168 VGPRs),
+// K = 64 two (256 VGPRs, a few bytes of spill for DIM 3); for K = 16 the
+// third wave is one more than the LDS stack of a 28-level tree lets a CU
+// hold.
 template <int DIM, int K, bool LDS_STACK, bool EXACT, bool LOG = false>
-__global__ __launch_bounds__(kSearchBlock, (K <= 16 ? 3 : 1)) void k_kd_search(SearchParams a) {
+__global__ __launch_bounds__(kSearchBlock, (K <= 32 ? 3 : K <= 64 ? 2 : 1)) void k_kd_search(SearchParams a) {
   // Per-lane stack: in LDS when the tree is shallow enough ([slot][lane] of
   // this workgroup), else in HBM ([slot][lane] over the grid).
   extern __shared__ uint2 lds_stack[];
