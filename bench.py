@@ -107,7 +107,7 @@ def bench_lidar_variants(nns, cloud, query, d_query, radius, bucket, use_oracle:
     m = len(query)
     tree = orc.default().kdtree(cloud, bucket_size=bucket) if use_oracle else None
     out = {}
-    stl = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_TREE_HEAP, 3, 0,
+    stl = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_TREE_HEAP, 3,
                                       {"bucketSize": bucket})
     for k, eps, heap in ((1, 0.0, "linear"), (1, 0.1, "linear"), (16, 0.1, "linear"),
                          (16, 0.0, "stl")):
@@ -171,7 +171,7 @@ def bench_brute_force(log_cpu: bool):
         # fp32_stl: BRUTE_FORCE, the reference's own IndexHeapSTL order
         nns = nabo.NearestNeighbourSearch(
             cloud, nabo.SearchType.BRUTE_FORCE if prec == "fp32_stl" else nabo.SearchType.BRUTE_FORCE_HIP,
-            dim, 0, {"bfPrecision": "fp32" if prec == "fp32_stl" else prec, "bfExtra": extra})
+            dim, {"bfPrecision": "fp32" if prec == "fp32_stl" else prec, "bfExtra": extra})
         di = torch.empty((m, k), dtype=torch.int32, device="cuda")
         dd = torch.empty((m, k), dtype=torch.float32, device="cuda")
         nns.knn_device(dq, k, di, dd)
@@ -501,11 +501,11 @@ def main():
     # The process's first build also loads code objects and fills the
     # scratch pools; the second is the steady-state figure.
     t0 = time.perf_counter()
-    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0, params)
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, params)
     build_ms_first = (time.perf_counter() - t0) * 1e3
     nns.close()
     t0 = time.perf_counter()
-    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0, params)
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, params)
     build_ms_host = (time.perf_counter() - t0) * 1e3
     info = nns.info()
     # Build from device-resident points as well (excludes the host copy).

@@ -16,7 +16,10 @@
 //     ties by index
 //   KDTREE_CL_* / BRUTE_FORCE_CL                      -> error, as the
 //     reference without OpenCL (nabo/nabo.cpp:148-151)
-// Scalar type float only (NNSearchF); NNSearchD searches throw.
+// Scalar type float only (NNSearchF, over MatrixXf, Matrix3Xf or
+// Map<const Matrix3Xf, Aligned> clouds, the reference's instantiations,
+// nabo/nabo.cpp:181-186); creating an NNSearchD throws, the interface's one
+// intended gap (the GPU searches compute in float).
 //
 // Matrices are Eigen's when <Eigen/Core> is available, otherwise the small
 // column-major subset of nabo_amd/dense.h (aliased as namespace Eigen).
@@ -77,6 +80,11 @@ struct Parameters : public std::map<std::string, std::any> {
     return defaultValue;
   }
 };
+
+// False for every type, but only once instantiated: lets the wrong-cloud
+// overloads below compile until a caller selects one.
+template <typename>
+struct wrong_cloud_type : std::false_type {};
 
 template <typename T, typename Cloud_T = Eigen::Matrix<T, Eigen::Dynamic, Eigen::Dynamic> >
 struct NearestNeighbourSearch {
@@ -140,22 +148,67 @@ struct NearestNeighbourSearch {
                                         const SearchType preferedType = KDTREE_LINEAR_HEAP,
                                         const unsigned creationOptionFlags = 0,
                                         const Parameters& additionalParameters = Parameters());
+  // The helpers of nabo/nabo.cpp:157-179 build the reference's own types:
+  // BRUTE_FORCE (IndexHeapSTL results), KDTREE_LINEAR_HEAP and
+  // KDTREE_TREE_HEAP (IndexHeapSTL results), each on the GPU.
   static NearestNeighbourSearch* createBruteForce(
       const CloudType& cloud, const Index dim = std::numeric_limits<Index>::max(),
       const unsigned creationOptionFlags = 0) {
-    return create(cloud, dim, BRUTE_FORCE_HIP, creationOptionFlags);
+    return create(cloud, dim, BRUTE_FORCE, creationOptionFlags);
   }
   static NearestNeighbourSearch* createKDTreeLinearHeap(
       const CloudType& cloud, const Index dim = std::numeric_limits<Index>::max(),
       const unsigned creationOptionFlags = 0,
       const Parameters& additionalParameters = Parameters()) {
-    return create(cloud, dim, KDTREE_HIP, creationOptionFlags, additionalParameters);
+    return create(cloud, dim, KDTREE_LINEAR_HEAP, creationOptionFlags, additionalParameters);
   }
   static NearestNeighbourSearch* createKDTreeTreeHeap(
       const CloudType& cloud, const Index dim = std::numeric_limits<Index>::max(),
       const unsigned creationOptionFlags = 0,
       const Parameters& additionalParameters = Parameters()) {
-    return create(cloud, dim, KDTREE_HIP, creationOptionFlags, additionalParameters);
+    return create(cloud, dim, KDTREE_TREE_HEAP, creationOptionFlags, additionalParameters);
+  }
+
+  // A cloud whose type is not CloudType (a fixed-row matrix given to a search
+  // over dynamic matrices, for instance) is refused at compile time, as by
+  // nabo/nabo.h:390-424: these overloads take any other argument type and
+  // fail when instantiated.
+  template <typename WrongMatrixType>
+  static NearestNeighbourSearch* create(const WrongMatrixType&,
+                                        const Index = std::numeric_limits<Index>::max(),
+                                        const SearchType = KDTREE_LINEAR_HEAP,
+                                        const unsigned = 0, const Parameters& = Parameters()) {
+    static_assert(wrong_cloud_type<WrongMatrixType>::value,
+                  "Please make sure that the decltype of the first parameter is equal to the "
+                  "Matrix typedef");
+    return nullptr;
+  }
+  template <typename WrongMatrixType>
+  static NearestNeighbourSearch* createBruteForce(const WrongMatrixType&,
+                                                  const Index = std::numeric_limits<Index>::max(),
+                                                  const unsigned = 0) {
+    static_assert(wrong_cloud_type<WrongMatrixType>::value,
+                  "Please make sure that the decltype of the first parameter is equal to the "
+                  "Matrix typedef");
+    return nullptr;
+  }
+  template <typename WrongMatrixType>
+  static NearestNeighbourSearch* createKDTreeLinearHeap(
+      const WrongMatrixType&, const Index = std::numeric_limits<Index>::max(),
+      const unsigned = 0, const Parameters& = Parameters()) {
+    static_assert(wrong_cloud_type<WrongMatrixType>::value,
+                  "Please make sure that the decltype of the first parameter is equal to the "
+                  "Matrix typedef");
+    return nullptr;
+  }
+  template <typename WrongMatrixType>
+  static NearestNeighbourSearch* createKDTreeTreeHeap(
+      const WrongMatrixType&, const Index = std::numeric_limits<Index>::max(),
+      const unsigned = 0, const Parameters& = Parameters()) {
+    static_assert(wrong_cloud_type<WrongMatrixType>::value,
+                  "Please make sure that the decltype of the first parameter is equal to the "
+                  "Matrix typedef");
+    return nullptr;
   }
 
   virtual ~NearestNeighbourSearch() {}
@@ -237,8 +290,13 @@ struct HipSearch : public NearestNeighbourSearch<T, CloudType> {
             int hipType, int heap, const Parameters& params)
       : Base(cloud, dim, creationOptionFlags) {
     static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value, "");
+    // NNSearchD (T = double) is the interface's one intended gap: every GPU
+    // search step computes in float, and a double search would need an FP64
+    // search path, which this library does not have. Creating one throws,
+    // before anything is allocated.
     if (!std::is_same<T, float>::value)
-      throw std::runtime_error("the HIP searches compute in float (NNSearchF) only");
+      throw std::runtime_error(
+          "NNSearchD is not offered: the HIP searches compute in float (use NNSearchF)");
     nabo_hip_params p;
     nabo_hip_params_default(&p);
     p.bucket_size = params.get<unsigned>("bucketSize", 8u);

@@ -261,13 +261,19 @@ class _Serialized:
 class NearestNeighbourSearch:
     """pynabo.NearestNeighbourSearch on the GPU (python/nabo.cpp:95-163).
 
-    ``cloud``: array of points (contiguous axis = coordinates); ``searchType``:
-    KDTREE_HIP (default; KDTREE_LINEAR_HEAP semantics) or BRUTE_FORCE_HIP;
-    ``params``: {"bucketSize": 8, ...}.
+    The positional arguments are pynabo's, in its order (python/nabo.cpp:98):
+    ``cloud`` (array of points, contiguous axis = coordinates),
+    ``searchType`` (default KDTREE_LINEAR_HEAP, as in pynabo; every type runs
+    on the GPU, see SearchType), ``dim`` and ``params`` ({"bucketSize": 8,
+    ...}). ``creationOptionFlags`` (TOUCH_STATISTICS), which pynabo does not
+    expose, is keyword-only.
     """
 
-    def __init__(self, cloud, searchType=SearchType.KDTREE_HIP, dim: int = 2**31 - 1,
-                 creationOptionFlags: int = 0, params: dict | None = None):
+    def __init__(self, cloud, searchType=SearchType.KDTREE_LINEAR_HEAP, dim: int = 2**31 - 1,
+                 params: dict | None = None, *, creationOptionFlags: int = 0):
+        if params is not None and not isinstance(params, dict):
+            raise TypeError("params must be a dict (pynabo's fourth argument); pass "
+                            "creationOptionFlags by keyword")
         self._lib = load_library()
         self._h = None
         if isinstance(cloud, _Serialized):
@@ -390,9 +396,18 @@ class NearestNeighbourSearch:
         (a torch.cuda.Stream, default: the current one)."""
         import torch
 
+        dev = _index_device(self)
+        m, qrows = _check_query_tensor(query, dev)
+        k = int(k)
+        _check_out_tensor(indices, "indices", torch.int32, (m, k), dev)
+        _check_out_tensor(dists2, "dists2", torch.float32, (m, k), dev)
+        if maxRadii is not None:
+            _check_out_tensor(maxRadii, "maxRadii", torch.float32, (m,), dev)
+        if touched is not None:
+            _check_out_tensor(touched, "touched", (torch.int64, torch.uint64), (1,), dev,
+                              any_shape=True)
         if stream is None:
             stream = torch.cuda.current_stream(query.device)
-        m, qrows = query.shape
         _check(self._lib, self._lib.nabo_hip_knn_device(
             self._h, C.c_void_p(query.data_ptr()), qrows, query.stride(0), m, int(k),
             float(epsilon), int(optionFlags), float(maxRadius),
@@ -400,6 +415,56 @@ class NearestNeighbourSearch:
             C.c_void_p(indices.data_ptr()), C.c_void_p(dists2.data_ptr()),
             None if touched is None else C.c_void_p(touched.data_ptr()),
             C.c_void_p(stream.cuda_stream)))
+
+
+def _index_device(index) -> int:
+    if getattr(index, "_device", None) is None:
+        index._device = int(index.info().device)
+    return index._device
+
+
+def _tensor_fail(msg):
+    raise NaboError(1, msg)
+
+
+def _check_query_tensor(query, dev: int):
+    """A (m, rows) float32 CUDA tensor on device ``dev`` whose rows are
+    contiguous (the C ABI reads point i at query + i * stride(0))."""
+    import torch
+
+    if not isinstance(query, torch.Tensor) or not query.is_cuda:
+        _tensor_fail("query must be a CUDA tensor")
+    if query.device.index != dev:
+        _tensor_fail(f"query is on cuda:{query.device.index}, the index on cuda:{dev}")
+    if query.dtype != torch.float32:
+        _tensor_fail(f"query must be float32, not {query.dtype}")
+    if query.dim() != 2:
+        _tensor_fail("query must be a two-dimensional (count, rows) tensor")
+    m, qrows = query.shape
+    if m > 0 and (query.stride(1) != 1 or query.stride(0) < qrows):
+        _tensor_fail("query rows must be contiguous (stride(1) == 1, stride(0) >= rows)")
+    return m, qrows
+
+
+def _check_out_tensor(t, name: str, dtype, shape, dev: int, any_shape: bool = False):
+    """A contiguous CUDA tensor of the given dtype and shape on device ``dev``:
+    the kernels write through its raw pointer."""
+    import torch
+
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        _tensor_fail(f"{name} must be a CUDA tensor")
+    if t.device.index != dev:
+        _tensor_fail(f"{name} is on cuda:{t.device.index}, the index on cuda:{dev}")
+    dtypes = dtype if isinstance(dtype, tuple) else (dtype,)
+    if t.dtype not in dtypes:
+        _tensor_fail(f"{name} must be {' or '.join(str(d) for d in dtypes)}, not {t.dtype}")
+    if any_shape:
+        if t.numel() < 1:
+            _tensor_fail(f"{name} must hold at least one element")
+    elif tuple(t.shape) != tuple(shape):
+        _tensor_fail(f"{name} has shape {tuple(t.shape)}, {tuple(shape)} is needed")
+    if not t.is_contiguous():
+        _tensor_fail(f"{name} must be contiguous")
 
 
 def _stream_handle(stream, device):
@@ -421,6 +486,8 @@ class BatchSearch:
 
     def __init__(self, points, offsets, dim: int = 2**31 - 1, creationOptionFlags: int = 0,
                  params: dict | None = None, stream=None):
+        import torch
+
         self._lib = load_library()
         self._h = None
         kw = {}
@@ -430,7 +497,15 @@ class BatchSearch:
             kw[_PARAM_KEYS[key]] = val
         p = make_params(**kw)
         off = np.ascontiguousarray(offsets, dtype=np.int64)
+        if not isinstance(points, torch.Tensor) or not points.is_cuda:
+            _tensor_fail("points must be a CUDA tensor")
+        if points.dtype != torch.float32 or points.dim() != 2:
+            _tensor_fail("points must be a two-dimensional float32 tensor")
         total, rows = points.shape
+        if total > 0 and (points.stride(1) != 1 or points.stride(0) < rows):
+            _tensor_fail("points rows must be contiguous (stride(1) == 1, stride(0) >= rows)")
+        if off.ndim != 1 or off.size < 2:
+            raise NaboError(1, "a batch needs at least one cloud (clouds + 1 offsets)")
         if int(off[-1]) != total:
             raise NaboError(1, "the last offset must equal the number of points")
         h = C.c_void_p()
@@ -488,11 +563,19 @@ class BatchSearch:
         int64 CUDA tensor. Returns (indices, dists2) CUDA tensors (m, k)."""
         import torch
 
-        m, qrows = queries.shape
+        dev = _index_device(self)
+        m, qrows = _check_query_tensor(queries, dev)
+        k = int(k)
+        _check_out_tensor(query_offsets, "query_offsets", torch.int64, (self.clouds + 1,), dev)
         if indices is None:
             indices = torch.empty((m, k), dtype=torch.int32, device=queries.device)
         if dists2 is None:
             dists2 = torch.empty((m, k), dtype=torch.float32, device=queries.device)
+        _check_out_tensor(indices, "indices", torch.int32, (m, k), dev)
+        _check_out_tensor(dists2, "dists2", torch.float32, (m, k), dev)
+        if touched is not None:
+            _check_out_tensor(touched, "touched", (torch.int64, torch.uint64), (1,), dev,
+                              any_shape=True)
         _check(self._lib, self._lib.nabo_hip_batch_knn(
             self._h, C.c_void_p(queries.data_ptr()), qrows, queries.stride(0), m,
             C.c_void_p(query_offsets.data_ptr()), int(k), float(epsilon), int(optionFlags),

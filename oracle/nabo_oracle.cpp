@@ -380,7 +380,11 @@ void kd_knn_all(const Tree& t, const float* q, int qrows, int64_t m, int k, floa
 #pragma omp for schedule(guided, 32)
     for (int64_t i = 0; i < m; ++i) {
       const float r = radii ? radii[i] : max_radius;
-      s.max_r2 = r * r;
+#if ORACLE_MUTANT == 7
+      s.max_r2 = radii ? r : r * r;  // planted: a per-query radius left unsquared
+#else
+      s.max_r2 = r * r;  // kdtree_cpu.cpp:331-332 (per-query), :280 (scalar)
+#endif
       std::fill(s.off.begin(), s.off.end(), 0.f);
       s.heap.reset();
       s.touched = 0;

@@ -1,31 +1,51 @@
 """The drop-in C++ interface (include/nabo/nabo.h).
 
-CPU: the header compiles with g++ (C++17) for our API example, and libnabo's
-own examples/trivial.cpp and examples/usage.cpp compile unchanged against it
-(read from a libnabo checkout when one is present, nothing is copied).
-GPU: the API example runs through libnabo_hip.so and checks itself.
+CPU: the header compiles with g++ (C++17) for our API programs; libnabo's
+own examples/trivial.cpp, examples/usage.cpp and tests/invalid_matrix_types.cpp
+compile unchanged against it (read from a libnabo checkout when one is
+present, nothing is copied); clouds of another type than the search's
+CloudType are refused at compile time (nabo/nabo.h:390-424).
+GPU: the API example runs through libnabo_hip.so and checks itself; the
+parity program runs every cloud type the reference instantiates
+(nabo/nabo.cpp:181-186) through every factory entry, and each result is
+compared with the oracle bit for bit.
 """
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SRC = os.path.join(ROOT, "tests", "cpp", "api_example.cpp")
-BIN = os.path.join(ROOT, "tests", "cpp", "api_example")
+CPP = os.path.join(ROOT, "tests", "cpp")
+SRC = os.path.join(CPP, "api_example.cpp")
+BIN = os.path.join(CPP, "api_example")
+PARITY_SRC = os.path.join(CPP, "api_parity.cpp")
+PARITY_BIN = os.path.join(CPP, "api_parity")
+WRONG_SRC = os.path.join(CPP, "wrong_cloud_type.cpp")
 LIB = os.path.join(ROOT, "libnabo_amd", "libnabo_hip.so")
 REF_EXAMPLES = "/root/reference/examples"
+REF_TESTS = "/root/reference/tests"
 
 
-def build_example(out=BIN):
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), SRC, "-o",
+def _compile(src, out):
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-o",
            out, LIB, f"-Wl,-rpath,{os.path.dirname(LIB)}"]
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     return out
 
 
+def build_example(out=BIN):
+    return _compile(SRC, out)
+
+
+def build_parity(out=PARITY_BIN):
+    return _compile(PARITY_SRC, out)
+
+
 def test_header_compiles(tmp_path):
     build_example(str(tmp_path / "api_example"))
+    build_parity(str(tmp_path / "api_parity"))
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="no libnabo checkout")
@@ -35,6 +55,35 @@ def test_reference_examples_compile_unchanged(name):
                     os.path.join(REF_EXAMPLES, name)], check=True)
 
 
+def _syntax(src, defines):
+    cmd = ["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+           *[f"-D{d}" for d in defines], src]
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+FACTORIES = ["FACTORY_CREATE", "FACTORY_BRUTE_FORCE", "FACTORY_LINEAR_HEAP", "FACTORY_TREE_HEAP"]
+
+
+@pytest.mark.parametrize("factory", FACTORIES)
+def test_wrong_cloud_type_refused_at_compile_time(factory):
+    ok = _syntax(WRONG_SRC, ["CLOUD_DYNAMIC", factory])
+    assert ok.returncode == 0, ok.stderr
+    bad = _syntax(WRONG_SRC, ["CLOUD_FIXED_ROWS", factory])
+    assert bad.returncode != 0
+    assert "equal to the Matrix typedef" in bad.stderr
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_TESTS), reason="no libnabo checkout")
+@pytest.mark.parametrize("tree", ["NABO_TYPE_CREATE", "NABO_TYPE_BRUTE_FORCE",
+                                  "NABO_TYPE_LINEAR_HEAP", "NABO_TYPE_TREE_HEAP"])
+def test_reference_invalid_matrix_types_unchanged(tree):
+    """tests/CMakeLists.txt:79-111: dynamic matrices compile, Matrix<float, 5,
+    Dynamic> does not."""
+    src = os.path.join(REF_TESTS, "invalid_matrix_types.cpp")
+    assert _syntax(src, ["NABO_EIGEN_DYNAMIC_TYPE", tree]).returncode == 0
+    assert _syntax(src, ["NABO_EIGEN_SEMI_DYNAMIC_TYPE", tree]).returncode != 0
+
+
 @pytest.mark.gpu
 def test_api_example_runs_on_gpu():
     exe = BIN if os.path.exists(BIN) else build_example()
@@ -42,3 +91,101 @@ def test_api_example_runs_on_gpu():
     res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "API OK" in res.stdout
+
+
+def _write_points(path, pts):
+    pts = np.ascontiguousarray(pts, np.float32)
+    with open(path, "wb") as f:
+        np.array(pts.shape, np.int64).tofile(f)
+        pts.tofile(f)
+
+
+def _read_result(path):
+    with open(path, "rb") as f:
+        k, m, touched = np.fromfile(f, np.int64, 3)
+        idx = np.fromfile(f, np.int32, int(k * m)).reshape(int(m), int(k))
+        d2 = np.fromfile(f, np.float32, int(k * m)).reshape(int(m), int(k))
+    return idx, d2, int(touched)
+
+
+def _parity_clouds():
+    """A uniform cloud, and an integer grid whose queries sit on grid points
+    and half-way between them (many equal distances, so the tie order of each
+    heap is exercised)."""
+    from libnabo_amd import datasets
+
+    cloud, query = datasets.uniform_cube(6_000, 1_500, seed=21)
+    g = np.arange(11, dtype=np.float32)
+    grid = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(22)
+    gq = np.concatenate([grid[rng.choice(len(grid), 400, replace=False)],
+                         rng.integers(0, 20, (400, 3)).astype(np.float32) / 2])
+    return [("uniform", cloud, query, 1.0), ("grid", grid, gq.astype(np.float32), 2.5)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [0, 1])
+def test_cpp_cloud_types_and_factories_match_oracle(case, oracle, tmp_path):
+    """Each cloud type x factory x flags x radius form, unsorted and sorted,
+    against the oracle: BRUTE_FORCE and createBruteForce give the oracle's
+    IndexHeapSTL brute force bit for bit; KDTREE_LINEAR_HEAP, KDTREE_HIP and
+    createKDTreeLinearHeap the linear-heap kd-tree; KDTREE_TREE_HEAP and
+    createKDTreeTreeHeap the IndexHeapSTL kd-tree (with equal touched counts);
+    BRUTE_FORCE_HIP equal dists2, indices up to ties."""
+    from tests.helpers import assert_bit_exact, assert_same_up_to_ties
+
+    name, cloud, query, radius = _parity_clouds()[case]
+    k = 7
+    exe = PARITY_BIN if os.path.exists(PARITY_BIN) else build_parity()
+    _write_points(tmp_path / "cloud.bin", cloud)
+    _write_points(tmp_path / "query.bin", query)
+    out = tmp_path / "out"
+    out.mkdir()
+    print(f"case cpp api parity {name} k={k} r={radius}", flush=True)
+    res = subprocess.run([exe, str(tmp_path), str(out), str(k), str(radius)],
+                         capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "PARITY DUMP OK" in res.stdout
+
+    tree = oracle.kdtree(cloud, bucket_size=8)
+    m = len(query)
+    radii = (np.float32(radius) * (1 + np.arange(m) % 3).astype(np.float32) / 2).astype(np.float32)
+    vm = min(m, 64)
+    want = {}
+    for flags in (0, 2):
+        want[("bf", flags)] = oracle.bf_knn(cloud, query, k, flags, radius)
+        li, ld, lt, _ = tree.knn(query, k, 0.0, flags, radius, stats=True)
+        want[("linear", flags)] = (li, ld, int(lt.sum()))
+        si, sd, st, _ = tree.knn(query, k, 0.0, flags, radius, heap="stl", stats=True)
+        want[("stl", flags)] = (si, sd, int(st.sum()))
+    want[("bf", "radii")] = oracle.bf_knn(cloud, query, k, 0, np.inf, radii=radii)
+    want[("linear", "radii")] = tree.knn(query, k, 0.0, 0, np.inf, radii=radii)
+    want[("stl", "radii")] = tree.knn(query, k, 0.0, 0, np.inf, radii=radii, heap="stl")
+    kinds = {"create_bruteforce": "bf", "helper_bruteforce": "bf", "create_linear": "linear",
+             "create_kdhip": "linear", "helper_linear": "linear", "create_treeheap": "stl",
+             "helper_treeheap": "stl", "create_bfhip": "bfhip"}
+    stats = {"create_linear", "create_treeheap"}
+    for tag in ("matrixxf", "matrix3xf", "map3xf"):
+        for fac, kind in kinds.items():
+            for form in ("f0", "f2", "radii", "vector"):
+                gi, gd, touched = _read_result(out / f"{tag}_{fac}_{form}.bin")
+                what = f"{name} {tag} {fac} {form}"
+                key = {"f0": 0, "f2": 2, "radii": "radii", "vector": 0}[form]
+                if kind == "bfhip":
+                    oi, od = want[("bf", 2 if key == 0 else key)][:2]
+                    if form == "radii":
+                        oi, od = want[("bf", "radii")][:2]
+                        order = np.argsort(od, axis=1, kind="stable")
+                        oi = np.take_along_axis(oi, order, 1)
+                        od = np.take_along_axis(od, order, 1)
+                    if form == "vector":
+                        oi, od = oi[:vm], od[:vm]
+                    assert_same_up_to_ties(gi, gd, oi, od, what)
+                    continue
+                w = want[(kind, key)]
+                oi, od = w[0], w[1]
+                if form == "vector":
+                    oi, od = oi[:vm], od[:vm]
+                assert_bit_exact(gi, gd, oi, od, what)
+                if fac in stats and form in ("f0", "f2"):
+                    assert touched == w[2], f"{what}: touched {touched} vs oracle {w[2]}"

@@ -48,7 +48,7 @@ def test_bf_fp32_matches_oracle(nabo, oracle, name, k, flags, radius):
 def test_bf_touch_statistics(nabo):
     cloud, query = CASES["uniform3d"]()
     bf = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, 3,
-                                     nabo.CreationOptionFlags.TOUCH_STATISTICS)
+                                     creationOptionFlags=nabo.CreationOptionFlags.TOUCH_STATISTICS)
     bf.knn(query[:100], 4)
     assert bf.last_touched == 100 * len(cloud)
 
@@ -59,7 +59,7 @@ def test_bf_shortlist(nabo, oracle, prec, extra, max_miss):
     cloud, query = CASES["dirichlet33"]()
     k = 8
     print(f"case bf {prec} m={extra}", flush=True)
-    bf = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, 33, 0,
+    bf = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, 33,
                                      {"bfPrecision": prec, "bfExtra": extra})
     gi, gd = bf.knn(query, k)
     exact = oracle.dist2_f64(cloud, query, gi)

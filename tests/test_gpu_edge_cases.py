@@ -20,9 +20,8 @@ SELF, SORT = 1, 2
 
 def check(nabo, oracle, cloud, query, k, eps=0.0, flags=0, radius=np.inf, bucket=8, heap="linear"):
     st = nabo.SearchType.KDTREE_HIP if heap == "linear" else nabo.SearchType.KDTREE_TREE_HEAP
-    nns = nabo.NearestNeighbourSearch(cloud, st, cloud.shape[1],
-                                      nabo.CreationOptionFlags.TOUCH_STATISTICS,
-                                      {"bucketSize": bucket})
+    nns = nabo.NearestNeighbourSearch(cloud, st, cloud.shape[1], {"bucketSize": bucket},
+                                      creationOptionFlags=nabo.CreationOptionFlags.TOUCH_STATISTICS)
     gi, gd = nns.knn(query, k, eps, flags, radius)
     oi, od, touched, _ = oracle.kdtree(cloud, bucket_size=bucket).knn(
         query, k, eps, flags, radius, heap=heap, threads=8, stats=True)
@@ -112,7 +111,7 @@ def test_deep_tree_stacks_in_hbm(nabo, oracle, k):
     for heap in ("linear", "stl"):
         check(nabo, oracle, cloud, query, k, bucket=2, heap=heap)
         check(nabo, oracle, cloud, query, k, flags=SORT, radius=1.0, bucket=2, heap=heap)
-    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0, {"bucketSize": 2})
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, {"bucketSize": 2})
     assert nns.info().depth == oracle.kdtree(cloud, bucket_size=2).depth > 32
 
 

@@ -71,7 +71,7 @@ def test_kdtree_knn_graph_replay(nabo, k, sort):
     from libnabo_amd import datasets
 
     cloud, query = datasets.lidar_scene(60_000, seed=11)
-    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0,
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3,
                                       {"bucketSize": 8, "querySort": sort})
     dq = torch.from_numpy(query).cuda()
     m = len(query)
@@ -90,7 +90,7 @@ def test_kdtree_capture_refused(nabo):
     from libnabo_amd import datasets
 
     cloud, query = datasets.uniform_cube(5_000, 2_000, seed=14)
-    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0, {"bucketSize": 8})
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, {"bucketSize": 8})
     dq = torch.from_numpy(query).cuda()
     gi = torch.empty((len(query), 4), dtype=torch.int32, device="cuda")
     gd = torch.empty((len(query), 4), dtype=torch.float32, device="cuda")

@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 def make(nabo, cloud, bucket=8, stats=False, dim=None):
     return nabo.NearestNeighbourSearch(
         cloud, nabo.SearchType.KDTREE_HIP, dim if dim is not None else 2**31 - 1,
-        nabo.CreationOptionFlags.TOUCH_STATISTICS if stats else 0, {"bucketSize": bucket})
+        {"bucketSize": bucket},
+        creationOptionFlags=nabo.CreationOptionFlags.TOUCH_STATISTICS if stats else 0)
 
 
 def assert_same_tree(nns, otree):
@@ -97,6 +98,22 @@ def test_knn_bit_exact(nabo, oracle, name, k, eps):
             assert nns.last_touched == int(touched.sum())
 
 
+@pytest.mark.parametrize("k", [1, 16])
+@pytest.mark.parametrize("eps", [0.0, 0.1])
+def test_knn_lidar_headline_radius(nabo, oracle, k, eps):
+    """The headline LiDAR workload's settings at 200k points (scene and
+    queries generated as for 1M): maxRadius 1.0, bucket 8, k in {1, 16},
+    epsilon in {0, 0.1}; bit-exact with equal touched counts."""
+    cloud, query = datasets.lidar_scene(200_000, seed=2)
+    print(f"case knn lidar200k r=1.0 k={k} eps={eps}", flush=True)
+    nns = make(nabo, cloud, 8, stats=True)
+    otree = oracle.kdtree(cloud, bucket_size=8)
+    gi, gd = nns.knn(query, k, eps, 0, 1.0)
+    oi, od, touched, _ = otree.knn(query, k, eps, 0, 1.0, stats=True)
+    assert_bit_exact(gi, gd, oi, od, f"lidar200k k={k} eps={eps} r=1.0")
+    assert nns.last_touched == int(touched.sum())
+
+
 def test_knn_self_queries(nabo, oracle):
     cloud, _ = datasets.lidar_scene(60_000, seed=5)
     nns = make(nabo, cloud, 8)
@@ -160,7 +177,7 @@ def test_knn_dim_subset(nabo, oracle, rng):
 
 def test_knn_no_query_sort(nabo, oracle):
     cloud, query = datasets.uniform_cube(10_000, 4_000, seed=3)
-    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0,
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3,
                                       {"bucketSize": 8, "querySort": 0})
     otree = oracle.kdtree(cloud, bucket_size=8)
     print("case knn unsorted queries", flush=True)

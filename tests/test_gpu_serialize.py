@@ -1,10 +1,10 @@
 """Serialized indices (nabo_hip_serialize / nabo_hip_deserialize) on the GPU.
 
 A deserialized index must be the same index: identical tree words and bucket
-order, identical info, and knn results bit-identical to the original's (which
-the other GPU tests tie to the oracle). The served loop with a host cache
-re-admits evicted trees from their host copies, and its results stay
-bit-identical to the plain search.
+order (the oracle's, node for node), identical info, and knn results
+bit-identical to the oracle's on the same inputs. The served loop with a host
+cache re-admits evicted trees from their host copies, and its results stay
+bit-identical to the oracle's.
 """
 import threading
 
@@ -12,13 +12,15 @@ import numpy as np
 import pytest
 
 from libnabo_amd import datasets
+from tests.helpers import assert_bit_exact, assert_same_up_to_ties
 
 pytestmark = pytest.mark.gpu
 
 
-def test_kdtree_roundtrip(nabo):
+def test_kdtree_roundtrip(nabo, oracle):
     cloud, query = datasets.lidar_scene(50_000, seed=21)
-    a = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, 0, {"bucketSize": 8})
+    print("case serialize kd-tree lidar 50k", flush=True)
+    a = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, {"bucketSize": 8})
     blob = a.serialize()
     b = nabo.NearestNeighbourSearch.deserialize(blob)
     ia, ib = a.info(), b.info()
@@ -28,26 +30,44 @@ def test_kdtree_roundtrip(nabo):
     nb, bb = b.tree()
     assert np.array_equal(na, nb) and np.array_equal(ba, bb)
     assert all(np.array_equal(x, y) for x, y in zip(a.bounds, b.bounds))
+    tree = oracle.kdtree(cloud, bucket_size=8)
+    on, ob, omin, omax = tree.export()
+    assert np.array_equal(nb, on) and np.array_equal(bb, ob)
+    assert np.array_equal(b.bounds[0], omin) and np.array_equal(b.bounds[1], omax)
     for k, eps, r in [(16, 0.0, 1.0), (1, 0.1, np.inf), (5, 0.0, 0.3)]:
         ai, ad = a.knn(query, k, eps, 0, r)
         bi, bd = b.knn(query, k, eps, 0, r)
         assert np.array_equal(ai, bi) and np.array_equal(ad.view(np.uint32), bd.view(np.uint32))
+        oi, od = tree.knn(query, k, eps, 0, r)
+        assert_bit_exact(bi, bd, oi, od, f"deserialized k={k} eps={eps} r={r}")
     # a copy of the copy is byte-identical
     assert np.array_equal(b.serialize(), blob)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
-def test_brute_force_roundtrip(nabo, precision):
+def test_brute_force_roundtrip(nabo, oracle, precision):
     cloud, query = datasets.dirichlet_descriptors(4_000, 500, seed=22)
-    a = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, 33, 0,
+    print(f"case serialize brute force {precision}", flush=True)
+    a = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, 33,
                                     {"bfPrecision": precision, "bfExtra": 8})
     b = nabo.NearestNeighbourSearch.deserialize(a.serialize())
     ai, ad = a.knn(query, 8)
     bi, bd = b.knn(query, 8)
     assert np.array_equal(ai, bi) and np.array_equal(ad.view(np.uint32), bd.view(np.uint32))
+    oi, od = oracle.bf_knn(cloud, query, 8, 2)  # SORT_RESULTS: ascending, as BRUTE_FORCE_HIP
+    if precision == "fp32":
+        assert_same_up_to_ties(bi, bd, oi, od, "deserialized brute force")
+    else:
+        # the bf16 shortlist keeps k + 8 candidates and rescores them in
+        # fp32: every returned distance is the exact one of its point, and
+        # the exact k-best are found for nearly every query
+        exact = oracle.dist2_f64(cloud, query, bi, 33)
+        assert np.allclose(bd, exact, rtol=1e-6)
+        same = (bd.view(np.uint32) == od.view(np.uint32)).all(axis=1)
+        assert same.mean() >= 0.99, f"shortlist misses on {1 - same.mean():.3%} of queries"
 
 
-def test_served_host_cache_readmits(nabo):
+def test_served_host_cache_readmits(nabo, oracle):
     rng = np.random.default_rng(23)
     clouds = [(rng.random((int(rng.integers(20_000, 50_000)), 3)) * 10).astype(np.float32)
               for _ in range(5)]
@@ -69,9 +89,8 @@ def test_served_host_cache_readmits(nabo):
     assert st["evictions"] > 0 and st["readmits"] > 0
     assert st["builds"] == len(set(c for c, _ in reqs))  # each tree built once
     assert st["host_cached_bytes"] > 0
-    plain = [nabo.NearestNeighbourSearch(c, nabo.SearchType.KDTREE_HIP, 3, 0, {"bucketSize": 8})
-             for c in clouds]
-    for (c, q), (gi, gd) in zip(reqs, outs):
-        oi, od = plain[c].knn(q, 8)
-        assert np.array_equal(gi, oi) and np.array_equal(gd.view(np.uint32), od.view(np.uint32))
+    trees = [oracle.kdtree(c, bucket_size=8) for c in clouds]
+    for i, ((c, q), (gi, gd)) in enumerate(zip(reqs, outs)):
+        oi, od = trees[c].knn(q, 8)
+        assert_bit_exact(gi, gd, oi, od, f"request {i} on cloud {c}")
     srv.close()

@@ -53,7 +53,7 @@ def test_tree_heap_matches_oracle(nabo, oracle, name, k, eps, flags, radius):
     cloud, query = KD_CASES[name]()
     print(f"case kd stl {name} k={k} eps={eps} flags={flags} r={radius}", flush=True)
     nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_TREE_HEAP, cloud.shape[1],
-                                      nabo.CreationOptionFlags.TOUCH_STATISTICS)
+                                      creationOptionFlags=nabo.CreationOptionFlags.TOUCH_STATISTICS)
     gi, gd = nns.knn(query, k, eps, flags, radius)
     touched = nns.last_touched
     ot = oracle.kdtree(cloud)
@@ -109,7 +109,7 @@ def test_brute_force_stl_matches_oracle(nabo, oracle, name, k, flags, radius):
 def test_stl_heap_refuses_shortlist(nabo):
     cloud, _ = BF_CASES["uniform3d"]()
     with pytest.raises(nabo.NaboError) as e:
-        nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, 3, 0,
+        nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, 3,
                                     {"heap": "stl", "bfPrecision": "bf16"})
     assert e.value.code == 4
 
@@ -150,9 +150,8 @@ def test_tree_heap_admission_log_refills(nabo, oracle, k, flags):
         np.array([[-1.0, 0.0, 0.0], [-2.0, 0.01, 0.0], [-50.0, 3.0, 1.0]], np.float32),
         (rng.random((200, 3)) * [1000.0, 1.0, 1.0]).astype(np.float32)])
     print(f"case kd stl log refills k={k} flags={flags}", flush=True)
-    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_TREE_HEAP, 3,
-                                      nabo.CreationOptionFlags.TOUCH_STATISTICS,
-                                      {"bucketSize": 4_000})
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_TREE_HEAP, 3, {"bucketSize": 4_000},
+                                      creationOptionFlags=nabo.CreationOptionFlags.TOUCH_STATISTICS)
     gi, gd = nns.knn(query, k, 0.0, flags)
     oi, od, otouch, _ = oracle.kdtree(cloud, bucket_size=4_000).knn(
         query, k, 0.0, flags, heap="stl", threads=8, stats=True)

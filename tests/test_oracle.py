@@ -271,3 +271,29 @@ def test_tree_heap_matches_linear_after_sort(oracle):
     assert np.array_equal(ld, td)
     ui, ud = t.knn(q, 7, 0.0, 0, heap="tree")  # heap order, unsorted
     assert np.array_equal(np.sort(ud, axis=1), ld)
+
+
+def test_per_query_radii_equal_scalar_calls(oracle):
+    """The maxRadii overload (kdtree_cpu.cpp:306-336, brute_force_cpu.cpp)
+    pinned by the scalar form: each query's row equals the scalar-radius call
+    with r = radii[i] (both heaps, epsilon 0 and 0.1, kd-tree and brute force,
+    touched counts included), and every returned dist2 is <= r_i^2."""
+    cloud, q = datasets.uniform_cube(6_000, 1_200, seed=31)
+    levels = np.array([0.0, 0.02, 0.05, 0.1, 0.3, np.inf], F32)
+    radii = levels[np.random.default_rng(32).integers(0, len(levels), len(q))]
+    t = oracle.kdtree(cloud)
+    for heap in ("linear", "tree"):
+        for eps in (0.0, 0.1):
+            ri, rd, rt, rv = t.knn(q, 6, eps, 0, np.inf, radii=radii, heap=heap, stats=True)
+            for r in levels:
+                sel = radii == r
+                si, sd, st, sv = t.knn(q[sel], 6, eps, 0, float(r), heap=heap, stats=True)
+                assert np.array_equal(ri[sel], si) and np.array_equal(rd[sel], sd), (heap, eps, r)
+                assert np.array_equal(rt[sel], st) and np.array_equal(rv[sel], sv), (heap, eps, r)
+                r2 = np.float32(r) * np.float32(r)
+                assert np.all(rd[sel][ri[sel] >= 0] <= r2)
+    bi, bd = oracle.bf_knn(cloud, q, 6, 0, np.inf, radii=radii)
+    for r in levels:
+        sel = radii == r
+        si, sd = oracle.bf_knn(cloud, q[sel], 6, 0, float(r))
+        assert np.array_equal(bi[sel], si) and np.array_equal(bd[sel], sd), ("bf", r)
