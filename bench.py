@@ -32,6 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE 55,908 KiB x 2.000 + WRITE_SIZE 130,507 KiB x 1.000
 # (profiles/kd_search_traffic_lidar1M_k16.md).
 MEASURED_TRAFFIC_BYTES = (55_908 * 2.000 + 130_507 * 1.000) * 1024
+TRAFFIC_SOURCE = ("not measured in this run: rocprofv3 FETCH_SIZE / WRITE_SIZE passes of the "
+                  "same command, profiles/kd_search_traffic_lidar1M_k16.md")
 METRIC = "exact kNN queries/s, 1M 3D pts k=16; batched+served vs plain GPU; p99 ms"
 
 
@@ -61,14 +63,34 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cores() -> int:
+    """Cores this process may run on (its affinity set), not the machine's."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(cloud, query, k, eps, radius, bucket, budget_s):
     """Oracle kd-tree (OpenMP guided,32 over queries, all host cores) on a
-    bounded sample of the same workload; also returns per-query P_q, V_q over
-    all queries for the roofline."""
+    bounded sample of the same workload, and on one core over a 200k-query
+    sample; also returns per-query P_q, V_q over all queries for the
+    roofline."""
     import oracle as orc
 
     o = orc.default()
-    threads = os.cpu_count() or 1
+    threads = host_cores()
     t0 = time.perf_counter()
     tree = o.kdtree(cloud, bucket_size=bucket)
     build_s = time.perf_counter() - t0
@@ -82,6 +104,16 @@ def cpu_baseline(cloud, query, k, eps, radius, bucket, budget_s):
         tree.knn(query, k, eps, 0, radius, threads=threads)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
+    # one core, as libnabo without OpenMP: a 200k-query sample, median of 3
+    sample = query[: min(len(query), 200_000)]
+    one = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        tree.knn(sample, k, eps, 0, radius, threads=1)
+        one.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    o.kdtree(cloud, bucket_size=bucket)
+    build_1 = time.perf_counter() - t0
     return {
         "value": len(query) / med,
         "unit": "queries/s",
@@ -90,6 +122,11 @@ def cpu_baseline(cloud, query, k, eps, radius, bucket, budget_s):
         "sample": f"oracle kd-tree, all {len(query)} queries x {len(times)} passes "
                   f"(median), build {build_s:.3f}s excluded",
         "build_s": build_s,
+        "cpu_model": cpu_model(),
+        "single_core": {"value": round(len(sample) / float(np.median(one)), 1),
+                        "unit": "queries/s", "cores": 1,
+                        "sample": f"{len(sample)} queries x 3 passes (median)",
+                        "build_s": round(build_1, 3)},
     }, (oi, od, touched, visits)
 
 
@@ -199,8 +236,9 @@ def bench_brute_force(log_cpu: bool):
                                  "peak": 157.3, "unit": "TFLOP/s",
                                  "frac": round(flops / (ms / 1e3) / 1e12 / 157.3, 4)}
         else:
-            kp = -(-(dim + 2) // 16) * 16
-            flops = 2.0 * kp * pairs
+            # 2 * d flops per (query, point) pair: the distance's dot product
+            # (padding columns and the rescoring are not counted)
+            flops = 2.0 * dim * pairs
             entry["mfma_tflops"] = round(flops / (ms / 1e3) / 1e12, 1)
             entry["roofline"] = {"bound": "mfma", "achieved": entry["mfma_tflops"], "peak": 2500.0,
                                  "unit": "TFLOP/s",
@@ -216,13 +254,22 @@ def bench_brute_force(log_cpu: bool):
         log(f"brute force {key}: {entry}")
     if log_cpu:
         o = orc.default()
-        threads = os.cpu_count() or 1
+        threads = host_cores()
         nq = 500
         t0 = time.perf_counter()
         oi, od = o.bf_knn(cloud, query[:nq], k, flags=2, threads=threads)
         dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(nq / dt, 1), "unit": "queries/s", "cores": threads,
-                               "kind": "port", "sample": f"oracle brute force, {nq} queries"}
+        # serial, as the reference's brute force (brute_force_cpu.cpp:81-98)
+        ns = 60
+        t0 = time.perf_counter()
+        o.bf_knn(cloud, query[:ns], k, flags=0, threads=1)
+        dt1 = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(ns / dt1, 1), "unit": "queries/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle brute force, serial as the reference, {ns} queries",
+                               "cpu_model": cpu_model(),
+                               "all_cores": {"value": round(nq / dt, 1), "cores": threads,
+                                             "sample": f"{nq} queries, OpenMP over queries"}}
         out["fp32_parity_sample"] = bool(np.array_equal(exact[1][:nq].view(np.uint32),
                                                         od.view(np.uint32)))
         # the STL-heap form returns the reference's own order: bit-identical rows
@@ -571,6 +618,20 @@ def main():
     gi = d_idx.cpu().numpy()
     gd = d_d2.cpu().numpy()
 
+    # The drop-in caller's rate: host query array in, host result arrays out
+    # (nabo_hip_knn: PCIe copies through pinned staging, plus the search).
+    host_ms = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        hi_, hd_ = nns.knn(query, k, args.eps, 0, args.radius)
+        host_ms.append((time.perf_counter() - t0) * 1e3)
+    host_path = {"ms_per_call": round(float(np.median(host_ms)), 3),
+                 "queries_per_s": round(m / (float(np.median(host_ms)) / 1e3), 1),
+                 "equal_device_path": bool(np.array_equal(hi_, gi) and np.array_equal(hd_, gd)),
+                 "note": "nabo_hip_knn with numpy arrays in and out (median of 3 calls)"}
+    del hi_, hd_
+    log(f"host-buffer path: {host_path}")
+
     cpu = None
     p_sum = v_sum = None
     parity = None
@@ -593,6 +654,7 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": int(MEASURED_TRAFFIC_BYTES) if default_cfg else None,
+                    "traffic_source": (TRAFFIC_SOURCE if default_cfg else None),
                     "algorithmic_bytes": int(algo_bytes),
                     "kernel": "k_kd_search<3,16>", "kernel_ms": round(kernel_ms, 4)}
 
@@ -619,6 +681,7 @@ def main():
                      "host_cloud": round(build_ms_host, 2),
                      "device_cloud": None if build_ms_dev is None else round(build_ms_dev, 2)},
         "parity_vs_oracle": parity,
+        "host_buffer_path": host_path,
     }
     if not args.no_extras:
         log("lidar: the other k / epsilon pairs")
