@@ -133,6 +133,25 @@ int nabo_hip_knn_device(const nabo_hip_index* index, const float* d_query, int32
                         int32_t* d_indices, float* d_dists2, uint64_t* d_touched,
                         void* hip_stream);
 
+/* nabo_hip_knn_device that allocates nothing. No counterpart in the
+ * reference: it is for callers that may not allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc). The call's temporaries
+ * come from a static 64 MiB device array that each GPU the library runs on
+ * holds until the process ends, instead of the scratch pool. Same arguments,
+ * semantics and results as nabo_hip_knn_device, for kd-tree indices; refused
+ * with NABO_HIP_ERR_UNSUPPORTED (before any search is launched) for
+ * brute-force indices, IndexHeapSTL searches with k <= 96 (their admission
+ * logs scale with the grid), trees deeper than 32 levels (HBM stacks) and
+ * calls whose temporaries exceed the array (about 24 bytes per query with
+ * query ordering: up to about 2.6 million queries). All calls on one GPU
+ * share the array: calls that overlap on a GPU race and must be serialized
+ * by the caller. Stream capture is refused as for nabo_hip_knn_device. */
+int nabo_hip_knn_device_noalloc(const nabo_hip_index* index, const float* d_query,
+                                int32_t qrows, int64_t qstride, int64_t m, int32_t k,
+                                float epsilon, uint32_t option_flags, float max_radius,
+                                const float* d_max_radii, int32_t* d_indices, float* d_dists2,
+                                uint64_t* d_touched, void* hip_stream);
+
 /* Batched kd-trees: `clouds` independent clouds concatenated in one device
  * buffer (cloud c = columns [offsets[c], offsets[c+1]), host offsets array
  * of clouds + 1 entries), all built in one pass on the GPU. Tree c is node

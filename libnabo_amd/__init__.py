@@ -121,7 +121,7 @@ EXPORTED = [
     "nabo_hip_batch_knn", "nabo_hip_batch_roots", "nabo_hip_server_config_default",
     "nabo_hip_server_create", "nabo_hip_server_destroy", "nabo_hip_server_add_cloud",
     "nabo_hip_server_submit", "nabo_hip_server_wait", "nabo_hip_server_get_stats",
-    "nabo_hip_serialize", "nabo_hip_deserialize",
+    "nabo_hip_serialize", "nabo_hip_deserialize", "nabo_hip_knn_device_noalloc",
 ]
 
 _LIBS: dict = {}
@@ -157,6 +157,8 @@ def load_library(path: str | None = None):
                                  C.POINTER(C.c_uint64)]
     lib.nabo_hip_knn_device.argtypes = [vp, vp, i32, i64, i64, i32, f32, u32, f32, vp, vp, vp,
                                         vp, vp]
+    lib.nabo_hip_knn_device_noalloc.argtypes = [vp, vp, i32, i64, i64, i32, f32, u32, f32, vp,
+                                                vp, vp, vp, vp]
     lib.nabo_hip_get_info.argtypes = [vp, C.POINTER(Info)]
     lib.nabo_hip_get_bounds.argtypes = [vp, vp, vp]
     lib.nabo_hip_get_tree.argtypes = [vp, vp, vp]

@@ -15,10 +15,6 @@
 
 #include "internal.h"
 
-struct nabo_hip_index {
-  nabo_hip::Index ix;
-};
-
 namespace nabo_hip {
 namespace {
 
@@ -307,6 +303,20 @@ std::vector<Scratch::Block> g_scratch;
 void Scratch::commit(int dev) {
   size_t total = 256;
   for (const Req& r : reqs_) total += r.bytes;
+  if (ext_) {
+    // caller-owned memory: carve, or refuse; no pool, no event, no allocation
+    if (total - 256 > ext_bytes_)
+      throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                  "the call needs " + std::to_string(total - 256) +
+                      " bytes of scratch; the caller's scratch holds " +
+                      std::to_string(ext_bytes_));
+    char* cur = static_cast<char*>(ext_);
+    for (const Req& r : reqs_) {
+      *r.slot = cur;
+      cur += r.bytes;
+    }
+    return;
+  }
   {
     std::lock_guard<std::mutex> lock(g_scratch_mu);
     // A block last used on this stream is reused at once (stream order); a
@@ -364,7 +374,7 @@ void Scratch::commit(int dev) {
 }
 
 Scratch::~Scratch() {
-  if (!block_.p) return;
+  if (ext_ || !block_.p) return;
   auto drop = [this](const Block& b) {
     // stream-ordered after the block's last use, on this call's stream
     (void)hipStreamWaitEvent(s_, b.done, 0);
@@ -714,15 +724,32 @@ int nabo_hip_knn_device(const nabo_hip_index* index, const float* d_query, int32
                         uint32_t option_flags, float max_radius, const float* d_max_radii,
                         int32_t* d_indices, float* d_dists2, uint64_t* d_touched,
                         void* hip_stream) {
+  return knn_device_call(index, d_query, qrows, qstride, m, k, epsilon, option_flags, max_radius,
+                         d_max_radii, d_indices, d_dists2, d_touched, hip_stream, nullptr);
+}
+
+}  // extern "C"
+
+int nabo_hip::knn_device_call(const nabo_hip_index* index, const float* d_query, int32_t qrows,
+                              int64_t qstride, int64_t m, int32_t k, float epsilon,
+                              uint32_t option_flags, float max_radius, const float* d_max_radii,
+                              int32_t* d_indices, float* d_dists2, uint64_t* d_touched,
+                              void* hip_stream, ScratchSource source) {
   return guarded([&] {
     if (!index) arg_error("index is null");
     const Index& ix = index->ix;
     check_knn(ix, qrows, qstride, m, k, option_flags);
     if (m > 0 && k > 0 && (!d_query || !d_indices || !d_dists2))
       arg_error("query or output pointer is null");
+    if (source && ix.search_type != NABO_HIP_SEARCH_KDTREE)
+      throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                  "the no-allocation knn entry point serves kd-tree indices only");
     DeviceScope scope(ix.device);
     hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    ExternalScratch ext;
+    if (source) ext = source(ix.device);
     KnnArgs a;
+    a.scratch = source ? &ext : nullptr;
     a.query = d_query;
     a.qrows = qrows;
     a.qstride = qstride;
@@ -743,6 +770,8 @@ int nabo_hip_knn_device(const nabo_hip_index* index, const float* d_query, int32
     }
   });
 }
+
+extern "C" {
 
 int nabo_hip_knn(const nabo_hip_index* index, const float* query, int32_t qrows, int64_t qstride,
                  int64_t m, int32_t k, float epsilon, uint32_t option_flags, float max_radius,

@@ -84,6 +84,12 @@ constexpr size_t kScratchKeepBytes = size_t(1) << 30;
 class Scratch {
  public:
   explicit Scratch(hipStream_t s) : s_(s) {}
+  // Carves the arrays from caller-owned device memory [base, base + bytes)
+  // instead of a pool block: commit() then allocates nothing and throws
+  // NABO_HIP_ERR_UNSUPPORTED when the arrays do not fit; nothing is given
+  // back on destruction.
+  // (base null: the pool, as the one-argument form.)
+  Scratch(hipStream_t s, void* base, size_t bytes) : s_(s), ext_(base), ext_bytes_(bytes) {}
   Scratch(const Scratch&) = delete;
   Scratch& operator=(const Scratch&) = delete;
   ~Scratch();
@@ -110,6 +116,8 @@ class Scratch {
     size_t bytes;
   };
   hipStream_t s_;
+  void* ext_ = nullptr;
+  size_t ext_bytes_ = 0;
   Block block_{-1, nullptr, 0, nullptr, nullptr};
   std::vector<Req> reqs_;
 };
@@ -162,6 +170,16 @@ struct Index {
   ~Index();
 };
 
+// Caller-owned device scratch for one knn call (the no-allocation entry
+// point): every temporary of the call is carved from it; calls whose
+// temporaries do not fit, or that need memory sized by the grid (admission
+// logs of the logged IndexHeapSTL search, HBM stacks of trees deeper than
+// the LDS stacks), are refused with NABO_HIP_ERR_UNSUPPORTED.
+struct ExternalScratch {
+  void* base = nullptr;
+  size_t bytes = 0;
+};
+
 struct KnnArgs {
   const float* query = nullptr;  // device
   int qrows = 0;
@@ -176,7 +194,32 @@ struct KnnArgs {
   float* out_d2 = nullptr;       // device
   uint64_t* touched = nullptr;   // device, nullable
   const int32_t* qcloud = nullptr;  // device, per-query cloud id (batched trees)
+  const ExternalScratch* scratch = nullptr;  // null: the per-device scratch pool
 };
+
+// Supplies the caller-owned scratch of a knn call on device `dev` (current
+// when called); throws Error on failure.
+typedef ExternalScratch (*ScratchSource)(int dev);
+
+}  // namespace nabo_hip
+
+// The opaque handle of the C ABI.
+struct nabo_hip_index {
+  nabo_hip::Index ix;
+};
+
+namespace nabo_hip {
+
+// Body of nabo_hip_knn_device (capi.hip): argument checks, then the search
+// enqueued on hip_stream. With `source`, the call's temporaries come from the
+// memory it returns and kd-tree indices only are served; without, from the
+// per-device scratch pool. Returns a status code and sets the thread's
+// message.
+int knn_device_call(const nabo_hip_index* index, const float* d_query, int32_t qrows,
+                    int64_t qstride, int64_t m, int32_t k, float epsilon, uint32_t option_flags,
+                    float max_radius, const float* d_max_radii, int32_t* d_indices,
+                    float* d_dists2, uint64_t* d_touched, void* hip_stream,
+                    ScratchSource source);
 
 // Builds the kd-tree of d_cloud on the GPU (kdtree_build.hip).
 // offsets: clouds + 1 point offsets into d_cloud ({0, n} for one cloud).

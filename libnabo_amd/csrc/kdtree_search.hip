@@ -846,9 +846,20 @@ uint32_t search_log_cap(int k) { return uint32_t(std::min(256, std::max(64, 8 * 
 
 template <int DIM, int K, bool EXACT, bool LOG>
 void launch_search_as(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& stack,
-                      hipStream_t s) {
+                      const ExternalScratch* ext, hipStream_t s) {
   const size_t lds = search_lds<LOG>(depth, p.k);
   const bool lds_stack = depth <= kLdsStackSlots;
+  // Caller-owned scratch holds the call's fixed-size temporaries only: the
+  // admission logs and HBM stacks scale with the resident grid.
+  if (ext && LOG)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                "IndexHeapSTL searches with k <= 96 keep per-lane admission logs, which a "
+                "caller-owned scratch does not hold");
+  if (ext && !lds_stack)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                "trees deeper than " + std::to_string(kLdsStackSlots) +
+                    " levels keep their search stacks in HBM, which a caller-owned scratch "
+                    "does not hold");
   const unsigned grid = lds_stack ? resident_grid<DIM, K, true, EXACT, LOG>(lds, cus, dev)
                                   : resident_grid<DIM, K, false, EXACT, LOG>(lds, cus, dev);
   const int64_t lanes = int64_t(grid) * kSearchBlock;
@@ -879,55 +890,55 @@ void launch_search_as(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>
 
 template <int DIM, int K, bool LOG = false>
 void launch_search(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& stack,
-                   hipStream_t s) {
+                   const ExternalScratch* ext, hipStream_t s) {
   // epsilon = 0 (max_err2 == 1, so the multiply is exact) takes the
   // one-compare admission tests.
   if (p.max_err2 == 1.f)
-    launch_search_as<DIM, K, true, LOG>(p, cus, dev, depth, stack, s);
+    launch_search_as<DIM, K, true, LOG>(p, cus, dev, depth, stack, ext, s);
   else
-    launch_search_as<DIM, K, false, LOG>(p, cus, dev, depth, stack, s);
+    launch_search_as<DIM, K, false, LOG>(p, cus, dev, depth, stack, ext, s);
 }
 
 // Sorted k-best of K >= k entries (LOG: with the logged IndexHeapSTL).
 template <int DIM, bool LOG>
 void dispatch_sorted(const SearchParams& p, int cus, int dev, int depth, DevBuf<uint2>& stack,
-                     hipStream_t s) {
+                     const ExternalScratch* ext, hipStream_t s) {
   const int k = p.k;
   if (k <= 1)
-    launch_search<DIM, 1, LOG>(p, cus, dev, depth, stack, s);
+    launch_search<DIM, 1, LOG>(p, cus, dev, depth, stack, ext, s);
   else if (k <= 2)
-    launch_search<DIM, 2, LOG>(p, cus, dev, depth, stack, s);
+    launch_search<DIM, 2, LOG>(p, cus, dev, depth, stack, ext, s);
   else if (k <= 4)
-    launch_search<DIM, 4, LOG>(p, cus, dev, depth, stack, s);
+    launch_search<DIM, 4, LOG>(p, cus, dev, depth, stack, ext, s);
   else if (k <= 8)
-    launch_search<DIM, 8, LOG>(p, cus, dev, depth, stack, s);
+    launch_search<DIM, 8, LOG>(p, cus, dev, depth, stack, ext, s);
   else if (k <= 16)
-    launch_search<DIM, 16, LOG>(p, cus, dev, depth, stack, s);
+    launch_search<DIM, 16, LOG>(p, cus, dev, depth, stack, ext, s);
   else if (k <= 32)
-    launch_search<DIM, 32, LOG>(p, cus, dev, depth, stack, s);
+    launch_search<DIM, 32, LOG>(p, cus, dev, depth, stack, ext, s);
   else if (k <= 64)
-    launch_search<DIM, 64, LOG>(p, cus, dev, depth, stack, s);
+    launch_search<DIM, 64, LOG>(p, cus, dev, depth, stack, ext, s);
   else if (k <= 96)
-    launch_search<DIM, 96, LOG>(p, cus, dev, depth, stack, s);
+    launch_search<DIM, 96, LOG>(p, cus, dev, depth, stack, ext, s);
   else if constexpr (LOG)
     throw Error(NABO_HIP_ERR_UNSUPPORTED, "the logged IndexHeapSTL search runs up to k = 96");
   else
-    launch_search<DIM, 128, LOG>(p, cus, dev, depth, stack, s);
+    launch_search<DIM, 128, LOG>(p, cus, dev, depth, stack, ext, s);
 }
 
 template <int DIM>
 void dispatch_k(const SearchParams& p, bool stl, int cus, int dev, int depth,
-                DevBuf<uint2>& stack, hipStream_t s) {
+                DevBuf<uint2>& stack, const ExternalScratch* ext, hipStream_t s) {
   // IndexHeapSTL: the logged form up to k = 96 (its heaps in the stacks'
   // LDS slots), else the heap in the output rows.
   static_assert(std::max(kLdsStackSlots, 96) * kSearchBlock * sizeof(uint2) <= 65536,
                 "96-entry heaps fit in 64 KiB of LDS");
   if (stl && p.k <= 96)
-    dispatch_sorted<DIM, true>(p, cus, dev, depth, stack, s);
+    dispatch_sorted<DIM, true>(p, cus, dev, depth, stack, ext, s);
   else if (stl)
-    launch_search<DIM, -1>(p, cus, dev, depth, stack, s);
+    launch_search<DIM, -1>(p, cus, dev, depth, stack, ext, s);
   else
-    dispatch_sorted<DIM, false>(p, cus, dev, depth, stack, s);
+    dispatch_sorted<DIM, false>(p, cus, dev, depth, stack, ext, s);
 }
 
 uint32_t bits_needed(uint64_t v) {
@@ -943,6 +954,18 @@ uint32_t bits_needed(uint64_t v) {
 
 void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   if (a.m == 0) return;
+  if (a.scratch) {
+    // refused before anything is launched (launch_search_as checks again)
+    if (ix.params.heap == NABO_HIP_HEAP_STL && a.k <= 96)
+      throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                  "IndexHeapSTL searches with k <= 96 keep per-lane admission logs, which a "
+                  "caller-owned scratch does not hold");
+    if (ix.depth > kLdsStackSlots)
+      throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                  "trees deeper than " + std::to_string(kLdsStackSlots) +
+                      " levels keep their search stacks in HBM, which a caller-owned scratch "
+                      "does not hold");
+  }
   if (a.k > kMaxK)
     throw Error(NABO_HIP_ERR_UNSUPPORTED,
                 "k = " + std::to_string(a.k) + " exceeds the HIP kd-tree limit of " +
@@ -1011,7 +1034,7 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   const int64_t tiles = ceil_div(a.m, kSortTile);
   const int64_t ncounts = int64_t(coarse_bins) * tiles;
 
-  Scratch scratch(s);
+  Scratch scratch(s, a.scratch ? a.scratch->base : nullptr, a.scratch ? a.scratch->bytes : 0);
   Span<uint32_t> counters, keys, counts, starts, fkeys;
   Span<int32_t> fidx, order;
   Span<uint8_t> scan_tmp;
@@ -1094,11 +1117,11 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   const int depth = std::max(1, ix.depth);
   DevBuf<uint2> stack;
   if (ix.dim == 3)
-    dispatch_k<3>(p, stl, cus, dev, depth, stack, s);
+    dispatch_k<3>(p, stl, cus, dev, depth, stack, a.scratch, s);
   else if (ix.dim == 2)
-    dispatch_k<2>(p, stl, cus, dev, depth, stack, s);
+    dispatch_k<2>(p, stl, cus, dev, depth, stack, a.scratch, s);
   else
-    dispatch_k<0>(p, stl, cus, dev, depth, stack, s);
+    dispatch_k<0>(p, stl, cus, dev, depth, stack, a.scratch, s);
   NH_LAUNCH_CHECK();
 }
 
