@@ -131,15 +131,17 @@ struct SearchParams {
 // value, else min(hd[j], value): the median of (hd[j-1], hd[j], value).
 // Indices follow with two selects per entry; both compares are taken on
 // the old distances, so the update has no branch.
-// For K = 32, 64 and 128 each entry's own select is taken inside the
-// descending loop (entry j still holds its old value there), so that no
-// K-entry temporary is live beside the k-best: K = 128 then fits its
-// registers without spills, K = 64 two waves per SIMD, K = 32 three (for
-// K = 96 the first form measured faster).
+// Each entry's own select is taken inside the descending loop (entry j
+// still holds its old value there), so that no K-entry temporary is live
+// beside the k-best: K = 128 then fits its registers without spills, K = 64
+// two waves per SIMD, K = 32 three, and smaller K need no register moves
+// after the update (for K = 96 the temporary form measured faster).
+// min(hd[0], value) is med3(-inf, hd[0], value): fminf would also quiet
+// signalling NaNs, which neither operand can be, at two extra instructions.
 template <int K>
 __device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], float value,
                                             int32_t index) {
-  if constexpr (K == 32 || K == 64 || K >= 128) {
+  if constexpr (K != 96) {
 #pragma unroll
     for (int j = K - 1; j > 0; --j) {
       const int32_t own = hd[j] > value ? index : hi[j];
@@ -147,7 +149,7 @@ __device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], fl
       hd[j] = __builtin_amdgcn_fmed3f(hd[j - 1], hd[j], value);
     }
     hi[0] = hd[0] > value ? index : hi[0];
-    hd[0] = fminf(hd[0], value);
+    hd[0] = __builtin_amdgcn_fmed3f(-INFINITY, hd[0], value);
   } else {
     int32_t moved[K];
 #pragma unroll
@@ -240,11 +242,14 @@ struct Admit {
                      : (r2_ == INFINITY ? INFINITY
                                         : __uint_as_float(__float_as_uint(fmaxf(r2_, 0.f)) + 1u));
   }
+  // min(head, r2n) as med3(-inf, head, r2n): no NaN-quieting instructions
+  // (head is never NaN; a NaN r2n was mapped to -inf above).
   __device__ __forceinline__ bool node(float rd, float head) const {
-    return EXACT ? rd < fminf(head, r2n) : (rd <= r2 && rd * err2 < head);
+    return EXACT ? rd < __builtin_amdgcn_fmed3f(-INFINITY, head, r2n)
+                 : (rd <= r2 && rd * err2 < head);
   }
   __device__ __forceinline__ bool point(float d, float head) const {
-    return EXACT ? d < fminf(head, r2n) : (d <= r2 && d < head);
+    return EXACT ? d < __builtin_amdgcn_fmed3f(-INFINITY, head, r2n) : (d <= r2 && d < head);
   }
 };
 
