@@ -101,6 +101,12 @@ class Scratch {
   }
   // Takes a block holding every array asked for and points the spans into it.
   void commit(int dev);
+  // Bytes the arrays asked for so far take (each 256-byte aligned).
+  size_t bytes() const {
+    size_t t = 0;
+    for (const Req& r : reqs_) t += r.bytes;
+    return t;
+  }
 
   struct Block {
     int dev;
@@ -220,6 +226,43 @@ int knn_device_call(const nabo_hip_index* index, const float* d_query, int32_t q
                     float max_radius, const float* d_max_radii, int32_t* d_indices,
                     float* d_dists2, uint64_t* d_touched, void* hip_stream,
                     ScratchSource source);
+
+// A kd-tree build level's state in device memory: its segment count and
+// the BFS id of its first segment.
+struct BuildLevel {
+  int32_t S;
+  int32_t pad;
+  int64_t base;
+};
+
+// What a build is built from and into.
+struct KdBuildShape {
+  int64_t n;
+  int dim;
+  int pts_stride;
+  int32_t clouds;
+  uint32_t bucket;
+  uint32_t dim_bits, dim_mask;
+};
+struct KdBuildOut {
+  float* pts = nullptr;       // (n + kPtsPad) x pts_stride floats, leaf-ordered on return
+  uint2* nodes = nullptr;     // eager: allocated by the build; recorded: 2n + 1 nodes
+  uint4* links = nullptr;     // eager: allocated by the build; recorded: 2n
+  int32_t* roots = nullptr;   // per-cloud roots (nullable)
+  float* boxes = nullptr;     // per-cloud query-order boxes, 6 floats each (nullable)
+  int32_t* status = nullptr;  // recorded builds: 1 when the levels did not suffice
+};
+
+class Launcher;
+
+// Scratch bytes of a recorded build of this shape.
+size_t kd_build_record_bytes(const KdBuildShape& sh);
+// Records the build's launches (with `levels` levels) through L, with its
+// scratch in [scratch, scratch + scratch_bytes), which must live as long as
+// the recording is replayed; copies the host offsets into it first.
+void kd_build_record(const KdBuildShape& sh, const float* d_cloud, int64_t stride,
+                     const int64_t* h_offsets, void* scratch, size_t scratch_bytes,
+                     KdBuildOut& out, Launcher& L, int levels);
 
 // Builds the kd-tree of d_cloud on the GPU (kdtree_build.hip).
 // offsets: clouds + 1 point offsets into d_cloud ({0, n} for one cloud).

@@ -14,15 +14,30 @@
 //      a global exclusive scan of the out-of-place flags gives every rank and
 //      the swap pairs follow without any sequential scan;
 //   4. leftCount rule (:180-190) and the children's clipped cells (:207-212).
-// Nodes are recorded in BFS order; subtree sizes (bottom-up) then give the
-// reference's pre-order numbering (left child = parent + 1, :215-223).
-// Bucket starts are the positions of the leaf segments in the final
-// permutation, which is the reference's bucket vector order.
+// The points travel as records (coordinates, then the point's index within
+// its cloud), swapped in place by the passes level after level: each level
+// reads the cut coordinate contiguously, a pass touches only the points it
+// moves, and the final order is the leaf-ordered point array the search
+// reads (the reference's bucket vector order).
+//
+// A level's segment count and first BFS id live in device memory
+// (BuildLevel, two alternating slots), and every kernel reads them there:
+// the launches of a level are the same whatever its size. The eager build
+// reads the number of splits of each level from a polled pinned mailbox and
+// stops at the first level without one; a build recorded in a HIP graph
+// launches a fixed number of levels, and kernels of levels past the last
+// return at once (a status word tells whether the levels sufficed).
+//
+// Node numbering: the reference emits nodes in pre-order with the left
+// child right after its parent (:215-223). With a full binary tree, the
+// nodes before node v in pre-order are its level(v) ancestors and the
+// complete subtrees left of it, 2 L - R nodes, where L counts the leaves
+// whose bucket starts before v's first point and R the right-child edges on
+// v's path; trees of batched clouds are stored one after the other, so
+//   pre(v) = 2 * leaves_before(start(v)) + level(v) - R(v) - cloud(v),
+// with leaves_before an exclusive scan of leaf-start flags over the points.
 
 #include <cstring>
-
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cfloat>
@@ -32,17 +47,21 @@
 
 #include "device_common.h"
 #include "internal.h"
+#include "launch.h"
+#include "scan.h"
 
 namespace nabo_hip {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kGridStride = 2048;  // workgroups of launches over segments / nodes
 
 struct Seg {  // one level of segments
   int32_t* start;
   int32_t* count;
-  float* lo;  // S x dim
-  float* hi;  // S x dim
+  int32_t* rc;  // right-child edges on the segment's path
+  float* lo;    // S x dim
+  float* hi;    // S x dim
 };
 
 struct SegStat {
@@ -64,7 +83,9 @@ struct Bfs {  // all nodes in BFS order
   uint32_t* count;  // leaf point count
   int32_t* left;
   int32_t* right;
-  uint32_t* size;   // nodes in the subtree
+  int32_t* start;  // first point of the node's range
+  int32_t* level;
+  int32_t* rc;      // right-child edges on the path
   uint32_t* pre;    // pre-order id
 };
 
@@ -203,7 +224,9 @@ __device__ void tile_reduce(const int32_t (&seg)[kItems], const uint32_t (&va)[k
 template <int OPA, int OPB>
 __global__ __launch_bounds__(kBlock) void k_tile_partials(int64_t nparts, TileParts parts,
                                                           uint32_t* ta, uint32_t* tb,
-                                                          int64_t stride, int64_t gstride) {
+                                                          int64_t stride, int64_t gstride,
+                                                          const int32_t* gate) {
+  if (gate && *gate == 0) return;
   const int64_t g = blockIdx.y;
   const int64_t base = int64_t(blockIdx.x) * kTile + int64_t(threadIdx.x) * kItems;
   int32_t sg[kItems];
@@ -252,10 +275,16 @@ __global__ void k_init_keys(int32_t clouds, int dim, uint32_t* keys) {
   keys[c * 2 * dim + dim + d] = float_key(FLT_MIN);  // maxBound starts at min() > 0 (nabo.cpp:74)
 }
 
-// Level 0: one root segment per cloud, its cell = the cloud's bounds.
+// Level 0: one root segment per cloud, its cell = the cloud's bounds; the
+// first level's state (clouds segments, BFS ids from 0).
 __global__ void k_roots(int32_t clouds, int dim, const int64_t* __restrict__ offsets,
-                        const uint32_t* __restrict__ keys, Seg seg, float* __restrict__ bounds) {
+                        const uint32_t* __restrict__ keys, Seg seg, float* __restrict__ bounds,
+                        BuildLevel* level0) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e == 0) {
+    level0->S = clouds;
+    level0->base = 0;
+  }
   if (e >= int64_t(clouds) * dim) return;
   const int64_t c = e / dim;
   const int d = int(e % dim);
@@ -268,13 +297,16 @@ __global__ void k_roots(int32_t clouds, int dim, const int64_t* __restrict__ off
   if (d == 0) {
     seg.start[c] = int32_t(offsets[c]);
     seg.count[c] = int32_t(offsets[c + 1] - offsets[c]);
+    seg.rc[c] = 0;
   }
 }
 
-// Cloud id of every point (binary search in the offsets), identity order,
-// every point in its cloud's root segment.
+// Cloud id of every point (binary search in the offsets), its record
+// (coordinates, then its index within its cloud as bits), every point in its
+// cloud's root segment.
 __global__ void k_init_points(int64_t n, int32_t clouds, const int64_t* __restrict__ offsets,
-                              int32_t* __restrict__ cid, int32_t* __restrict__ perm,
+                              const float* __restrict__ cloud, int64_t stride, int dim, int ps,
+                              int32_t* __restrict__ cid, float* __restrict__ recs,
                               int32_t* __restrict__ pseg) {
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -287,66 +319,99 @@ __global__ void k_init_points(int64_t n, int32_t clouds, const int64_t* __restri
       hi = mid - 1;
   }
   cid[p] = lo;
-  perm[p] = int32_t(p);
   pseg[p] = lo;
+  float* r = recs + p * ps;
+  const float* src = cloud + p * stride;
+  for (int d = 0; d < dim; ++d) r[d] = src[d];
+  r[dim] = __int_as_float(int32_t(p - offsets[lo]));
+  for (int d = dim + 1; d < ps; ++d) r[d] = 0.f;
 }
 
-__global__ void k_seg_classify(int32_t S, Seg seg, SegStat st, Bfs bfs, int64_t node_base,
-                               uint32_t bucket, int dim) {
-  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S) return;
-  const int32_t cnt = seg.count[s];
-  const int64_t node = node_base + s;
-  if (cnt <= int32_t(bucket)) {
-    st.split[s] = 0;
-    bfs.cd[node] = uint32_t(dim);
-    bfs.aux[node] = uint32_t(seg.start[s]);
-    bfs.count[node] = uint32_t(cnt);
-    return;
-  }
-  st.split[s] = 1;
-  const float* lo = seg.lo + int64_t(s) * dim;
-  const float* hi = seg.hi + int64_t(s) * dim;
-  float best = 0.f;
-  uint32_t axis = 0;
-  for (int d = 0; d < dim; ++d) {
-    const float extent = hi[d] - lo[d];
-    if (extent > best) {
-      best = extent;
-      axis = uint32_t(d);
+__global__ void k_seg_classify(const BuildLevel* __restrict__ cur, int32_t level, Seg seg,
+                               SegStat st, Bfs bfs, uint32_t bucket, int dim) {
+  const int32_t S = cur->S;
+  const int64_t base = cur->base;
+  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
+    const int32_t cnt = seg.count[s];
+    const int64_t node = base + s;
+    bfs.start[node] = seg.start[s];
+    bfs.level[node] = level;
+    bfs.rc[node] = seg.rc[s];
+    if (cnt <= int32_t(bucket)) {
+      st.split[s] = 0;
+      bfs.cd[node] = uint32_t(dim);
+      bfs.aux[node] = uint32_t(seg.start[s]);
+      bfs.count[node] = uint32_t(cnt);
+      continue;
     }
+    st.split[s] = 1;
+    const float* lo = seg.lo + int64_t(s) * dim;
+    const float* hi = seg.hi + int64_t(s) * dim;
+    float best = 0.f;
+    uint32_t axis = 0;
+    for (int d = 0; d < dim; ++d) {
+      const float extent = hi[d] - lo[d];
+      if (extent > best) {
+        best = extent;
+        axis = uint32_t(d);
+      }
+    }
+    st.cd[s] = axis;
+    st.ideal[s] = (hi[axis] + lo[axis]) * 0.5f;  // exact: same value as /2
+    st.minkey[s] = 0xffffffffu;
+    st.maxkey[s] = 0u;
+    st.lt[s] = 0u;
+    st.eq[s] = 0u;
   }
-  st.cd[s] = axis;
-  st.ideal[s] = (hi[axis] + lo[axis]) * 0.5f;  // exact: same value as /2
-  st.minkey[s] = 0xffffffffu;
-  st.maxkey[s] = 0u;
-  st.lt[s] = 0u;
-  st.eq[s] = 0u;
 }
 
-// Number of segments that split at this level, posted to the host: the
+// Number of splits of a level, posted to the host (eager builds): the
 // count, a system-scope fence, then the sequence number the host waits for.
 struct Mailbox {
   int32_t total;
   uint32_t seq;
 };
-__global__ void k_split_total(int32_t S, const uint32_t* split, const uint32_t* childbase,
-                              Mailbox* box, uint32_t seq) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const int32_t total = int32_t(childbase[S - 1] + split[S - 1]);
+
+// The next level's state: twice the splits of this one, BFS ids after it.
+__global__ void k_level_next(const BuildLevel* __restrict__ cur, BuildLevel* __restrict__ nxt,
+                             const uint32_t* __restrict__ splits, Mailbox* box, uint32_t seq) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int32_t total = cur->S > 0 ? int32_t(*splits) : 0;
+  nxt->S = 2 * total;
+  nxt->base = cur->base + cur->S;
+  if (box) {
     __hip_atomic_store(&box->total, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __threadfence_system();
     __hip_atomic_store(&box->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
-// Gathers the cut coordinate of every point still inside a splitting segment
-// and reduces its min/max per segment.
-__global__ __launch_bounds__(kBlock) void k_pt_gather(int64_t n, int32_t* __restrict__ pseg,
-                                                      const int32_t* __restrict__ perm,
-                                                      const float* __restrict__ cloud,
-                                                      int64_t stride, SegStat st,
-                                                      float* __restrict__ xs, TileParts parts) {
+// The point kernels below return at once when the level splits nothing
+// (*gate, the next level's segment count, is 0).
+
+// Reads the cut coordinate of every point still inside a splitting segment
+// (one point per thread: each point's loads depend on its segment's).
+__global__ void k_pt_gather(int64_t n, const int32_t* __restrict__ gate,
+                            int32_t* __restrict__ pseg, const float* __restrict__ recs, int ps,
+                            SegStat st, float* __restrict__ xs) {
+  if (*gate == 0) return;
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int32_t s = pseg[p];
+  if (s < 0) return;
+  if (st.split[s] == 0) {
+    pseg[p] = -1;  // its segment became a leaf
+    return;
+  }
+  xs[p] = recs[p * ps + st.cd[s]];
+}
+
+// Min / max of the gathered cut coordinates per segment.
+__global__ __launch_bounds__(kBlock) void k_pt_minmax(int64_t n, const int32_t* __restrict__ gate,
+                                                      const int32_t* __restrict__ pseg,
+                                                      const float* __restrict__ xs, SegStat st,
+                                                      TileParts parts) {
+  if (*gate == 0) return;
   const int64_t tile = blockIdx.x;
   const int64_t base = tile * kTile + int64_t(threadIdx.x) * kItems;
   int32_t sg[kItems];
@@ -354,41 +419,36 @@ __global__ __launch_bounds__(kBlock) void k_pt_gather(int64_t n, int32_t* __rest
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int64_t p = base + j;
-    int s = p < n ? pseg[p] : -1;
-    kv[j] = 0u;
-    if (s >= 0 && st.split[s] == 0) {
-      pseg[p] = -1;  // its segment became a leaf
-      s = -1;
-    }
-    if (s >= 0) {
-      const float x = cloud[int64_t(perm[p]) * stride + st.cd[s]];
-      xs[p] = x;
-      kv[j] = float_key(x);
-    }
+    const int s = p < n ? pseg[p] : -1;
     sg[j] = s;
+    kv[j] = s >= 0 ? float_key(xs[p]) : 0u;
   }
   tile_reduce<0, 1>(sg, kv, kv, st.minkey, st.maxkey, 1, &parts, tile);
 }
 
-__global__ void k_seg_cut(int32_t S, SegStat st) {
-  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S || st.split[s] == 0) return;
-  const float pmin = key_float(st.minkey[s]);
-  const float pmax = key_float(st.maxkey[s]);
-  const float ideal = st.ideal[s];
-  float cut;
-  if (ideal < pmin)
-    cut = pmin;
-  else if (ideal > pmax)
-    cut = pmax;
-  else
-    cut = ideal;
-  st.cut[s] = cut;
+__global__ void k_seg_cut(const BuildLevel* __restrict__ cur, SegStat st) {
+  const int32_t S = cur->S;
+  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
+    if (st.split[s] == 0) continue;
+    const float pmin = key_float(st.minkey[s]);
+    const float pmax = key_float(st.maxkey[s]);
+    const float ideal = st.ideal[s];
+    float cut;
+    if (ideal < pmin)
+      cut = pmin;
+    else if (ideal > pmax)
+      cut = pmax;
+    else
+      cut = ideal;
+    st.cut[s] = cut;
+  }
 }
 
-__global__ __launch_bounds__(kBlock) void k_pt_count(int64_t n, const int32_t* __restrict__ pseg,
+__global__ __launch_bounds__(kBlock) void k_pt_count(int64_t n, const int32_t* __restrict__ gate,
+                                                     const int32_t* __restrict__ pseg,
                                                      const float* __restrict__ xs, SegStat st,
                                                      TileParts parts) {
+  if (*gate == 0) return;
   const int64_t tile = blockIdx.x;
   const int64_t base = tile * kTile + int64_t(threadIdx.x) * kItems;
   int32_t sg[kItems];
@@ -412,9 +472,10 @@ __global__ __launch_bounds__(kBlock) void k_pt_count(int64_t n, const int32_t* _
 
 // Out-of-place flags of one Hoare pass: low word = belongs right but sits
 // left, high word = belongs left but sits right.
-__global__ void k_pt_flags(int64_t n, int pass, const int32_t* __restrict__ pseg,
-                           const float* __restrict__ xs, Seg seg, SegStat st,
-                           uint64_t* __restrict__ flags) {
+__global__ void k_pt_flags(int64_t n, const int32_t* __restrict__ gate, int pass,
+                           const int32_t* __restrict__ pseg, const float* __restrict__ xs,
+                           Seg seg, SegStat st, uint64_t* __restrict__ flags) {
+  if (*gate == 0) return;
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int s = pseg[p];
@@ -462,90 +523,115 @@ __device__ __forceinline__ void pass_ranks(int64_t p, int s, Seg seg,
   }
 }
 
-__global__ void k_pt_slots(int64_t n, const int32_t* __restrict__ pseg, Seg seg,
+// Slot of every right-misplaced point, by rank: the i-th left-misplaced
+// point of a segment (from the left) swaps with the i-th right-misplaced
+// one counted from the right.
+__global__ void k_pt_slots(int64_t n, const int32_t* __restrict__ gate,
+                           const int32_t* __restrict__ pseg, Seg seg,
                            const uint64_t* __restrict__ flags, const uint64_t* __restrict__ excl,
-                           int32_t* __restrict__ slot_l, int32_t* __restrict__ slot_r) {
+                           int32_t* __restrict__ slot_r) {
+  if (*gate == 0) return;
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const uint64_t f = flags[p];
-  if (f == 0) return;
+  if ((f >> 32) == 0) return;
   const int s = pseg[p];
   int32_t start;
   uint32_t rl, rr;
   pass_ranks(p, s, seg, flags, excl, f, &start, &rl, &rr);
-  if (uint32_t(f))
-    slot_l[start + rl] = int32_t(p);
-  else
-    slot_r[start + rr] = int32_t(p);
+  slot_r[start + rr] = int32_t(p);
 }
 
-__global__ void k_pt_move(int64_t n, const int32_t* __restrict__ pseg, Seg seg,
+// The pass's swaps, in place: the thread of each left-misplaced point swaps
+// its record and cut coordinate with its partner's (ps floats per record;
+// 4 for 2-D and 3-D clouds, one 16-byte access). Pairs are disjoint, so no
+// two threads touch one point.
+__global__ void k_pt_swap(int64_t n, const int32_t* __restrict__ gate,
+                          const int32_t* __restrict__ pseg, Seg seg,
                           const uint64_t* __restrict__ flags, const uint64_t* __restrict__ excl,
-                          const int32_t* __restrict__ slot_l, const int32_t* __restrict__ slot_r,
-                          const int32_t* __restrict__ perm, const float* __restrict__ xs,
-                          int32_t* __restrict__ perm_out, float* __restrict__ xs_out) {
+                          const int32_t* __restrict__ slot_r, float* __restrict__ recs,
+                          float* __restrict__ xs, int ps) {
+  if (*gate == 0) return;
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const uint64_t f = flags[p];
-  int64_t dst = p;
-  if (f) {
-    const int s = pseg[p];
-    int32_t start;
-    uint32_t rl, rr;
-    pass_ranks(p, s, seg, flags, excl, f, &start, &rl, &rr);
-    dst = uint32_t(f) ? slot_r[start + rl] : slot_l[start + rr];
-  }
-  perm_out[dst] = perm[p];
-  xs_out[dst] = xs[p];
-}
-
-__global__ void k_seg_children(int32_t S, Seg seg, SegStat st, Bfs bfs, int64_t node_base,
-                               int64_t next_base, Seg next, int dim) {
-  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S || st.split[s] == 0) return;
-  const int32_t cnt = seg.count[s];
+  if (uint32_t(f) == 0) return;
+  const int s = pseg[p];
   const int32_t start = seg.start[s];
-  const float ideal = st.ideal[s];
-  const float pmin = key_float(st.minkey[s]);
-  const float pmax = key_float(st.maxkey[s]);
-  const int32_t br1 = int32_t(st.lt[s]);
-  const int32_t br2 = br1 + int32_t(st.eq[s]);
-  int32_t left;
-  if (ideal < pmin)
-    left = 1;
-  else if (ideal > pmax)
-    left = cnt - 1;
-  else if (br1 > cnt / 2)
-    left = br1;
-  else if (br2 < cnt / 2)
-    left = br2;
-  else
-    left = cnt / 2;
-  left = min(max(left, 1), cnt - 1);  // no-op for finite coordinates
-  st.left[s] = left;
-  const uint32_t cd = st.cd[s];
-  const float cut = st.cut[s];
-  const int32_t c = int32_t(2 * st.childbase[s]);
-  next.start[c] = start;
-  next.count[c] = left;
-  next.start[c + 1] = start + left;
-  next.count[c + 1] = cnt - left;
-  for (int d = 0; d < dim; ++d) {
-    const float lo = seg.lo[int64_t(s) * dim + d];
-    const float hi = seg.hi[int64_t(s) * dim + d];
-    next.lo[int64_t(c) * dim + d] = lo;
-    next.hi[int64_t(c) * dim + d] = (uint32_t(d) == cd) ? cut : hi;
-    next.lo[int64_t(c + 1) * dim + d] = (uint32_t(d) == cd) ? cut : lo;
-    next.hi[int64_t(c + 1) * dim + d] = hi;
+  const uint32_t rl = uint32_t(excl[p]) - uint32_t(excl[start]);
+  const int64_t q = slot_r[start + rl];
+  if (ps == 4) {
+    float4* r4 = reinterpret_cast<float4*>(recs);
+    const float4 a = r4[p], b = r4[q];
+    r4[p] = b;
+    r4[q] = a;
+  } else {
+    for (int d = 0; d < ps; ++d) {
+      const float a = recs[p * ps + d], b = recs[q * ps + d];
+      recs[p * ps + d] = b;
+      recs[q * ps + d] = a;
+    }
   }
-  const int64_t node = node_base + s;
-  bfs.cd[node] = cd;
-  bfs.aux[node] = __float_as_uint(cut);
-  bfs.left[node] = int32_t(next_base + c);
-  bfs.right[node] = int32_t(next_base + c + 1);
+  const float xa = xs[p], xb = xs[q];
+  xs[p] = xb;
+  xs[q] = xa;
 }
 
-__global__ void k_pt_reseg(int64_t n, int32_t* __restrict__ pseg, Seg seg, SegStat st) {
+__global__ void k_seg_children(const BuildLevel* __restrict__ cur, Seg seg, SegStat st, Bfs bfs,
+                               Seg next, int dim) {
+  const int32_t S = cur->S;
+  const int64_t node_base = cur->base;
+  const int64_t next_base = node_base + S;
+  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
+    if (st.split[s] == 0) continue;
+    const int32_t cnt = seg.count[s];
+    const int32_t start = seg.start[s];
+    const float ideal = st.ideal[s];
+    const float pmin = key_float(st.minkey[s]);
+    const float pmax = key_float(st.maxkey[s]);
+    const int32_t br1 = int32_t(st.lt[s]);
+    const int32_t br2 = br1 + int32_t(st.eq[s]);
+    int32_t left;
+    if (ideal < pmin)
+      left = 1;
+    else if (ideal > pmax)
+      left = cnt - 1;
+    else if (br1 > cnt / 2)
+      left = br1;
+    else if (br2 < cnt / 2)
+      left = br2;
+    else
+      left = cnt / 2;
+    left = min(max(left, 1), cnt - 1);  // no-op for finite coordinates
+    st.left[s] = left;
+    const uint32_t cd = st.cd[s];
+    const float cut = st.cut[s];
+    const int32_t c = int32_t(2 * st.childbase[s]);
+    next.start[c] = start;
+    next.count[c] = left;
+    next.start[c + 1] = start + left;
+    next.count[c + 1] = cnt - left;
+    next.rc[c] = seg.rc[s];
+    next.rc[c + 1] = seg.rc[s] + 1;
+    for (int d = 0; d < dim; ++d) {
+      const float lo = seg.lo[int64_t(s) * dim + d];
+      const float hi = seg.hi[int64_t(s) * dim + d];
+      next.lo[int64_t(c) * dim + d] = lo;
+      next.hi[int64_t(c) * dim + d] = (uint32_t(d) == cd) ? cut : hi;
+      next.lo[int64_t(c + 1) * dim + d] = (uint32_t(d) == cd) ? cut : lo;
+      next.hi[int64_t(c + 1) * dim + d] = hi;
+    }
+    const int64_t node = node_base + s;
+    bfs.cd[node] = cd;
+    bfs.aux[node] = __float_as_uint(cut);
+    bfs.left[node] = int32_t(next_base + c);
+    bfs.right[node] = int32_t(next_base + c + 1);
+  }
+}
+
+__global__ void k_pt_reseg(int64_t n, const int32_t* __restrict__ gate,
+                           int32_t* __restrict__ pseg, Seg seg, SegStat st) {
+  if (*gate == 0) return;
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int s = pseg[p];
@@ -554,64 +640,49 @@ __global__ void k_pt_reseg(int64_t n, int32_t* __restrict__ pseg, Seg seg, SegSt
   pseg[p] = int32_t(2 * st.childbase[s]) + (pos < st.left[s] ? 0 : 1);
 }
 
-__global__ void k_subtree_size(int64_t base, int32_t S, Bfs bfs, int dim) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= S) return;
-  const int64_t node = base + i;
-  if (bfs.cd[node] == uint32_t(dim))
-    bfs.size[node] = 1;
-  else
-    bfs.size[node] = 1 + bfs.size[bfs.left[node]] + bfs.size[bfs.right[node]];
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t count, uint32_t v) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
 }
 
-// Pre-order ids: a root takes its tree's base (trees are stored one after
-// the other), then left = parent + 1 and right = parent + 1 + |left subtree|.
-__global__ void k_preorder(int64_t base, int32_t S, Bfs bfs, int dim,
-                           const uint32_t* __restrict__ root_base) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= S) return;
-  const int64_t node = base + i;
-  if (root_base) bfs.pre[node] = root_base[node];
-  if (bfs.cd[node] == uint32_t(dim)) return;
-  const uint32_t pre = bfs.pre[node];
-  const int32_t l = bfs.left[node];
-  bfs.pre[l] = pre + 1;
-  bfs.pre[bfs.right[node]] = pre + 1 + bfs.size[l];
+// Nodes: fin->base (the BFS id after the last level) in all.
+__global__ void k_mark_leaves(const BuildLevel* __restrict__ fin, Bfs bfs, int dim,
+                              uint32_t* __restrict__ leaf_start) {
+  const int64_t count = fin->base;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < count;
+       v += int64_t(gridDim.x) * blockDim.x)
+    if (bfs.cd[v] == uint32_t(dim)) leaf_start[bfs.start[v]] = 1u;
 }
 
-__global__ void k_encode(int64_t count, Bfs bfs, uint32_t dim_bits, int dim, uint2* nodes) {
-  const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (node >= count) return;
-  const uint32_t cd = bfs.cd[node];
-  uint2 out;
-  if (cd == uint32_t(dim)) {
-    out.x = uint32_t(dim) | (bfs.count[node] << dim_bits);
-    out.y = bfs.aux[node];
-  } else {
-    out.x = cd | (bfs.pre[bfs.right[node]] << dim_bits);
-    out.y = bfs.aux[node];
+__global__ void k_preorder(const BuildLevel* __restrict__ fin, Bfs bfs,
+                           const uint32_t* __restrict__ leaves_before,
+                           const int32_t* __restrict__ cid) {
+  const int64_t count = fin->base;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < count;
+       v += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t s = bfs.start[v];
+    bfs.pre[v] = 2u * leaves_before[s] + uint32_t(bfs.level[v]) - uint32_t(bfs.rc[v]) -
+                 uint32_t(cid[s]);
   }
-  nodes[bfs.pre[node]] = out;
 }
 
-// Leaf-ordered copy of the points: pts_stride floats per entry, coordinates
-// then the point's index within its own cloud (as bits) in slot dim.
-__global__ void k_pack_points(int64_t n, const int32_t* __restrict__ perm,
-                              const int32_t* __restrict__ cid, const int64_t* __restrict__ offsets,
-                              const float* __restrict__ cloud, int64_t stride, int dim,
-                              int pts_stride, float* __restrict__ pts) {
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= n + kPtsPad) return;
-  if (p >= n) {
-    for (int d = 0; d < pts_stride; ++d) pts[p * pts_stride + d] = 0.f;
-    return;
+__global__ void k_encode(const BuildLevel* __restrict__ fin, Bfs bfs, uint32_t dim_bits, int dim,
+                         uint2* __restrict__ nodes) {
+  const int64_t count = fin->base;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < count;
+       v += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t cd = bfs.cd[v];
+    uint2 out;
+    if (cd == uint32_t(dim)) {
+      out.x = uint32_t(dim) | (bfs.count[v] << dim_bits);
+      out.y = bfs.aux[v];
+    } else {
+      out.x = cd | (bfs.pre[bfs.right[v]] << dim_bits);
+      out.y = bfs.aux[v];
+    }
+    nodes[bfs.pre[v]] = out;
   }
-  const int32_t idx = perm[p];
-  float* dst = pts + p * pts_stride;
-  const float* src = cloud + int64_t(idx) * stride;
-  for (int d = 0; d < dim; ++d) dst[d] = src[d];
-  dst[dim] = __int_as_float(int32_t(idx - offsets[cid[idx]]));
-  for (int d = dim + 1; d < pts_stride; ++d) dst[d] = 0.f;
 }
 
 // Search-side links, by pre-order node p: links[p].xy is the node word of
@@ -619,41 +690,85 @@ __global__ void k_pack_points(int64_t n, const int32_t* __restrict__ perm,
 // node and both its children; links[c].zw is {cut dim, cut bits} of the
 // split of c's parent, so a far child is entered without reading its parent
 // again. nodes[count] is a padding leaf, so nodes[p + 1] exists for every p.
-__global__ void k_links_self(int64_t count, const uint2* __restrict__ nodes, uint32_t dim_bits,
-                             uint32_t dim_mask, int dim, uint4* __restrict__ links,
-                             uint2* __restrict__ pad) {
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= count) return;
-  const uint2 nd = nodes[p];
-  uint4 l = make_uint4(0u, 0u, 0u, 0u);
-  if ((nd.x & dim_mask) != uint32_t(dim)) {
-    const uint2 r = nodes[nd.x >> dim_bits];
-    l.x = r.x;
-    l.y = r.y;
+__global__ void k_links_self(const BuildLevel* __restrict__ fin, const uint2* __restrict__ nodes,
+                             uint32_t dim_bits, uint32_t dim_mask, int dim,
+                             uint4* __restrict__ links, uint2* __restrict__ all_nodes) {
+  const int64_t count = fin->base;
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < count;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    const uint2 nd = nodes[p];
+    uint4 l = make_uint4(0u, 0u, 0u, 0u);
+    if ((nd.x & dim_mask) != uint32_t(dim)) {
+      const uint2 r = nodes[nd.x >> dim_bits];
+      l.x = r.x;
+      l.y = r.y;
+    }
+    links[p] = l;
+    if (p == 0) all_nodes[count] = make_uint2(uint32_t(dim), 0u);
   }
-  links[p] = l;
-  if (p == 0) *pad = make_uint2(uint32_t(dim), 0u);
 }
 
-__global__ void k_links_parent(int64_t count, const uint2* __restrict__ nodes, uint32_t dim_bits,
+__global__ void k_links_parent(const BuildLevel* __restrict__ fin,
+                               const uint2* __restrict__ nodes, uint32_t dim_bits,
                                uint32_t dim_mask, int dim, uint4* __restrict__ links) {
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= count) return;
-  const uint2 nd = nodes[p];
-  const uint32_t cd = nd.x & dim_mask;
-  if (cd == uint32_t(dim)) return;
-  const uint2 split = make_uint2(cd, nd.y);
-  reinterpret_cast<uint2*>(links + p + 1)[1] = split;
-  reinterpret_cast<uint2*>(links + (nd.x >> dim_bits))[1] = split;
+  const int64_t count = fin->base;
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < count;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    const uint2 nd = nodes[p];
+    const uint32_t cd = nd.x & dim_mask;
+    if (cd == uint32_t(dim)) continue;
+    const uint2 split = make_uint2(cd, nd.y);
+    reinterpret_cast<uint2*>(links + p + 1)[1] = split;
+    reinterpret_cast<uint2*>(links + (nd.x >> dim_bits))[1] = split;
+  }
 }
 
-__global__ void k_roots_out(int32_t clouds, const uint32_t* __restrict__ root_base,
+// Zeroed records after the last point (a leaf scan may read kPtsPad entries
+// past its bucket).
+__global__ void k_pad_points(int64_t n, int ps, float* __restrict__ recs) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < int64_t(kPtsPad) * ps) recs[n * ps + e] = 0.f;
+}
+
+// Root of cloud c: the nodes of the trees before it, 2 leaves - 1 each.
+__global__ void k_roots_out(int32_t clouds, const int64_t* __restrict__ offsets,
+                            const uint32_t* __restrict__ leaves_before,
                             int32_t* __restrict__ roots) {
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < clouds) roots[c] = int32_t(root_base[c]);
+  if (c < clouds) roots[c] = int32_t(2u * leaves_before[offsets[c]] - uint32_t(c));
+}
+
+// Per-cloud query-order boxes of 2-D / 3-D batched trees, {min bound[3],
+// 1 / extent[3]}: a cloud of at most bucketSize points keeps the
+// reference's initial bounds (kdtree_cpu.cpp:236-243), whose extent is
+// negative, so its scale is 0.
+__global__ void k_boxes(int32_t clouds, int dim, const int64_t* __restrict__ offsets,
+                        uint32_t bucket, const float* __restrict__ bounds,
+                        float* __restrict__ boxes) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(clouds) * 3) return;
+  const int64_t c = e / 3;
+  const int d = int(e % 3);
+  float lo = 0.f, sc = 0.f;
+  if (d < dim) {
+    const bool one_bucket = offsets[c + 1] - offsets[c] <= int64_t(bucket);
+    lo = one_bucket ? FLT_MAX : bounds[c * 2 * dim + d];
+    const float hi = one_bucket ? FLT_MIN : bounds[c * 2 * dim + dim + d];
+    const float ext = hi - lo;
+    sc = ext > 0.f ? 1.f / ext : 0.f;
+  }
+  boxes[c * 6 + d] = lo;
+  boxes[c * 6 + 3 + d] = sc;
+}
+
+// 1 when the recorded levels did not finish the tree (the level after the
+// last one still has segments), else 0.
+__global__ void k_build_status(const BuildLevel* __restrict__ fin, int32_t* __restrict__ status) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *status = fin->S > 0 ? 1 : 0;
 }
 
 inline unsigned grid_for(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, kBlock))); }
+inline unsigned tiles_for(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, kTile))); }
 
 // Pinned host mailboxes for the per-level split counts, pooled: the host
 // polls one instead of a device-to-host copy and a stream synchronization
@@ -715,9 +830,224 @@ class LevelMailbox {
   HostMailbox m_{nullptr, 0};
 };
 
-inline unsigned tiles_for(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, kTile))); }
+// Scratch arrays of one build, carved from one block.
+struct BuildScratch {
+  Span<int32_t> seg_start[2], seg_count[2], seg_rc[2];
+  Span<float> seg_lo[2], seg_hi[2];
+  Span<uint32_t> st_cd, st_minkey, st_maxkey, st_lt, st_eq, st_split, st_childbase;
+  Span<float> st_ideal, st_cut;
+  Span<int32_t> st_left;
+  Span<uint32_t> b_cd, b_aux, b_count, b_pre;
+  Span<int32_t> b_left, b_right, b_start, b_level, b_rc;
+  Span<int32_t> pseg, slot_r, cid;
+  Span<float> xs;
+  Span<uint64_t> flags, excl, partials64;
+  Span<uint32_t> keys, leaf_start, leaves_before, partials32, splits;
+  Span<float> bounds;
+  Span<int64_t> d_offsets;
+  Span<int32_t> part_seg;
+  Span<uint32_t> part_a, part_b;
+  Span<BuildLevel> levels;  // two alternating level states
+
+  void want(Scratch& sc, int64_t n, int dim, int ps, int32_t clouds) {
+    const int64_t seg_cap = std::max<int64_t>(n, clouds);
+    const int64_t max_nodes = 2 * n;
+    for (int b = 0; b < 2; ++b) {
+      sc.want(seg_start[b], size_t(seg_cap));
+      sc.want(seg_count[b], size_t(seg_cap));
+      sc.want(seg_rc[b], size_t(seg_cap));
+      sc.want(seg_lo[b], size_t(seg_cap) * dim);
+      sc.want(seg_hi[b], size_t(seg_cap) * dim);
+    }
+    for (Span<uint32_t>* a : {&st_cd, &st_minkey, &st_maxkey, &st_lt, &st_eq, &st_split,
+                              &st_childbase})
+      sc.want(*a, size_t(seg_cap));
+    sc.want(st_ideal, size_t(seg_cap));
+    sc.want(st_cut, size_t(seg_cap));
+    sc.want(st_left, size_t(seg_cap));
+    for (Span<uint32_t>* a : {&b_cd, &b_aux, &b_count, &b_pre}) sc.want(*a, size_t(max_nodes));
+    for (Span<int32_t>* a : {&b_left, &b_right, &b_start, &b_level, &b_rc})
+      sc.want(*a, size_t(max_nodes));
+    for (Span<int32_t>* a : {&pseg, &slot_r, &cid}) sc.want(*a, size_t(n));
+    sc.want(xs, size_t(n));
+    sc.want(flags, size_t(n));
+    sc.want(excl, size_t(n));
+    sc.want(partials64, kScanGrid);
+    sc.want(keys, size_t(2) * clouds * dim);
+    sc.want(leaf_start, size_t(n) + 1);
+    sc.want(leaves_before, size_t(n) + 1);
+    sc.want(partials32, kScanGrid);
+    sc.want(splits, 1);
+    sc.want(bounds, size_t(2) * clouds * dim);
+    sc.want(d_offsets, size_t(clouds) + 1);
+    const int64_t nparts = 2 * int64_t(tiles_for(n));
+    sc.want(part_seg, size_t(nparts) * dim);
+    sc.want(part_a, size_t(nparts) * dim);
+    sc.want(part_b, size_t(nparts) * dim);
+    sc.want(levels, 2);
+  }
+};
+
+// The build's launch sequence. The offsets must already be in
+// scratch d_offsets. Eager (levels == 0): levels are launched until one
+// splits nothing, reading each level's split count from the mailbox;
+// *levels_out and *nodes_out receive the level count and node count, and the
+// node arrays are allocated once the count is known (out.nodes / out.links
+// null on entry). Recorded (levels > 0): exactly `levels` levels, node
+// arrays preallocated for 2n nodes, and out.status receives 1 if the tree
+// needed more levels.
+void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t stride,
+                       BuildScratch& sc, KdBuildOut& out, Launcher& L, int levels,
+                       int* levels_out, int64_t* nodes_out) {
+  const int64_t n = sh.n;
+  const int dim = sh.dim;
+  const int ps = sh.pts_stride;
+  const int32_t clouds = sh.clouds;
+  const bool eager = levels == 0;
+  Seg seg[2];
+  for (int b = 0; b < 2; ++b)
+    seg[b] = Seg{sc.seg_start[b].p, sc.seg_count[b].p, sc.seg_rc[b].p, sc.seg_lo[b].p,
+                 sc.seg_hi[b].p};
+  SegStat st{sc.st_cd.p, sc.st_ideal.p, sc.st_cut.p, sc.st_minkey.p, sc.st_maxkey.p, sc.st_lt.p,
+             sc.st_eq.p, sc.st_split.p, sc.st_childbase.p, sc.st_left.p};
+  Bfs bfs{sc.b_cd.p, sc.b_aux.p, sc.b_count.p, sc.b_left.p, sc.b_right.p,
+          sc.b_start.p, sc.b_level.p, sc.b_rc.p, sc.b_pre.p};
+  const int64_t nparts = 2 * int64_t(tiles_for(n));
+  const TileParts parts{sc.part_seg.p, sc.part_a.p, sc.part_b.p};
+  const unsigned gpart = unsigned(ceil_div(nparts, kTile));
+  float* recs = out.pts;
+  BuildLevel* lvl = sc.levels.p;
+
+  // Bounds pass (kdtree_cpu.cpp:256-267) and the root segments.
+  const int64_t cd_elems = int64_t(clouds) * dim;
+  L(k_init_points, dim3(grid_for(n)), dim3(kBlock), 0, n, clouds, sc.d_offsets.p, d_cloud, stride,
+    dim, ps, sc.cid.p, recs, sc.pseg.p);
+  L(k_init_keys, dim3(grid_for(cd_elems)), dim3(kBlock), 0, clouds, dim, sc.keys.p);
+  L(k_bounds, dim3(tiles_for(n)), dim3(kBlock), 0, n, d_cloud, stride, dim, sc.cid.p, sc.keys.p,
+    parts, nparts);
+  L(k_tile_partials<0, 1>, dim3(gpart, unsigned(dim)), dim3(kBlock), 0, nparts, parts, sc.keys.p,
+    sc.keys.p + dim, int64_t(2 * dim), int64_t(1), nullptr);
+  L(k_roots, dim3(grid_for(cd_elems)), dim3(kBlock), 0, clouds, dim, sc.d_offsets.p, sc.keys.p,
+    seg[0], sc.bounds.p, lvl);
+
+  const unsigned gs = unsigned(std::min<int64_t>(kGridStride, grid_for(std::max<int64_t>(n, clouds))));
+  const unsigned gp = grid_for(n), gt = tiles_for(n);
+  LevelMailbox mailbox;
+  int32_t S = clouds;  // eager: the host's copy of the level's segment count
+  int64_t node_base = 0;
+  int level = 0;
+  for (;; ++level) {
+    if (!eager && level == levels) break;
+    BuildLevel* cur = lvl + (level & 1);
+    BuildLevel* nxt = lvl + ((level + 1) & 1);
+    const int32_t* gate = &nxt->S;
+    const Seg sg = seg[level & 1], nx = seg[(level + 1) & 1];
+    if (eager && node_base + S > 2 * n)
+      throw Error(NABO_HIP_ERR_HIP, "kd-tree build overflow");
+    L(k_seg_classify, dim3(gs), dim3(kBlock), 0, static_cast<const BuildLevel*>(cur), level, sg,
+      st, bfs, sh.bucket, dim);
+    exclusive_scan<uint32_t>(L, st.split, st.childbase, ScanCount{&cur->S, 0, nullptr},
+                             sc.partials32.p, sc.splits.p);
+    const uint32_t seq = eager ? mailbox.next() : 0u;
+    L(k_level_next, dim3(1), dim3(64), 0, static_cast<const BuildLevel*>(cur), nxt,
+      static_cast<const uint32_t*>(sc.splits.p), eager ? mailbox.box() : nullptr, seq);
+    int32_t T = 1;
+    if (eager) {
+      T = mailbox.wait(seq, L.stream());
+      node_base += S;
+      S = 2 * T;
+      if (T == 0) {
+        ++level;
+        break;
+      }
+    }
+    L(k_pt_gather, dim3(gp), dim3(kBlock), 0, n, gate, sc.pseg.p,
+      static_cast<const float*>(recs), ps, st, sc.xs.p);
+    L(k_pt_minmax, dim3(gt), dim3(kBlock), 0, n, gate, static_cast<const int32_t*>(sc.pseg.p),
+      static_cast<const float*>(sc.xs.p), st, parts);
+    L(k_tile_partials<0, 1>, dim3(gpart), dim3(kBlock), 0, nparts, parts, st.minkey, st.maxkey,
+      int64_t(1), int64_t(0), gate);
+    L(k_seg_cut, dim3(gs), dim3(kBlock), 0, static_cast<const BuildLevel*>(cur), st);
+    L(k_pt_count, dim3(gt), dim3(kBlock), 0, n, gate, static_cast<const int32_t*>(sc.pseg.p),
+      static_cast<const float*>(sc.xs.p), st, parts);
+    L(k_tile_partials<2, 2>, dim3(gpart), dim3(kBlock), 0, nparts, parts, st.lt, st.eq,
+      int64_t(1), int64_t(0), gate);
+    for (int pass = 1; pass <= 2; ++pass) {
+      L(k_pt_flags, dim3(gp), dim3(kBlock), 0, n, gate, pass,
+        static_cast<const int32_t*>(sc.pseg.p), static_cast<const float*>(sc.xs.p), sg, st,
+        sc.flags.p);
+      exclusive_scan<uint64_t>(L, sc.flags.p, sc.excl.p, ScanCount{nullptr, n, gate},
+                               sc.partials64.p, static_cast<uint64_t*>(nullptr));
+      L(k_pt_slots, dim3(gp), dim3(kBlock), 0, n, gate, static_cast<const int32_t*>(sc.pseg.p),
+        sg, static_cast<const uint64_t*>(sc.flags.p), static_cast<const uint64_t*>(sc.excl.p),
+        sc.slot_r.p);
+      L(k_pt_swap, dim3(gp), dim3(kBlock), 0, n, gate, static_cast<const int32_t*>(sc.pseg.p),
+        sg, static_cast<const uint64_t*>(sc.flags.p), static_cast<const uint64_t*>(sc.excl.p),
+        static_cast<const int32_t*>(sc.slot_r.p), recs, sc.xs.p, ps);
+    }
+    L(k_seg_children, dim3(gs), dim3(kBlock), 0, static_cast<const BuildLevel*>(cur), sg, st,
+      bfs, nx, dim);
+    L(k_pt_reseg, dim3(gp), dim3(kBlock), 0, n, gate, sc.pseg.p, sg, st);
+  }
+  const int levels_run = level;
+  BuildLevel* fin = lvl + (levels_run & 1);
+  if (levels_out) *levels_out = levels_run;
+  if (nodes_out) *nodes_out = eager ? node_base : -1;
+
+  // Pre-order ids from the leaf-start scan, then the node words and links.
+  if (eager) {
+    NH_CHECK(hipMalloc(reinterpret_cast<void**>(&out.nodes), sizeof(uint2) * (node_base + 1)));
+    NH_CHECK(hipMalloc(reinterpret_cast<void**>(&out.links), sizeof(uint4) * node_base));
+  }
+  const unsigned gn = unsigned(std::min<int64_t>(kGridStride, grid_for(2 * n)));
+  L(k_fill_u32, dim3(gn), dim3(kBlock), 0, sc.leaf_start.p, n + 1, 0u);
+  L(k_mark_leaves, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs, dim,
+    sc.leaf_start.p);
+  exclusive_scan<uint32_t>(L, sc.leaf_start.p, sc.leaves_before.p,
+                           ScanCount{nullptr, n + 1, nullptr}, sc.partials32.p,
+                           static_cast<uint32_t*>(nullptr));
+  L(k_preorder, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs,
+    static_cast<const uint32_t*>(sc.leaves_before.p), static_cast<const int32_t*>(sc.cid.p));
+  L(k_encode, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs, sh.dim_bits,
+    dim, out.nodes);
+  L(k_links_self, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin),
+    static_cast<const uint2*>(out.nodes), sh.dim_bits, sh.dim_mask, dim, out.links, out.nodes);
+  L(k_links_parent, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin),
+    static_cast<const uint2*>(out.nodes), sh.dim_bits, sh.dim_mask, dim, out.links);
+  L(k_pad_points, dim3(grid_for(int64_t(kPtsPad) * ps)), dim3(kBlock), 0, n, ps, out.pts);
+  if (out.roots)
+    L(k_roots_out, dim3(grid_for(clouds)), dim3(kBlock), 0, clouds,
+      static_cast<const int64_t*>(sc.d_offsets.p),
+      static_cast<const uint32_t*>(sc.leaves_before.p), out.roots);
+  if (out.boxes)
+    L(k_boxes, dim3(grid_for(int64_t(clouds) * 3)), dim3(kBlock), 0, clouds, dim,
+      static_cast<const int64_t*>(sc.d_offsets.p), sh.bucket,
+      static_cast<const float*>(sc.bounds.p), out.boxes);
+  if (out.status)
+    L(k_build_status, dim3(1), dim3(64), 0, static_cast<const BuildLevel*>(fin), out.status);
+}
 
 }  // namespace
+
+size_t kd_build_record_bytes(const KdBuildShape& sh) {
+  Scratch sc(nullptr);
+  BuildScratch b;
+  b.want(sc, sh.n, sh.dim, sh.pts_stride, sh.clouds);
+  return sc.bytes();
+}
+
+void kd_build_record(const KdBuildShape& sh, const float* d_cloud, int64_t stride,
+                     const int64_t* h_offsets, void* scratch, size_t scratch_bytes,
+                     KdBuildOut& out, Launcher& L, int levels) {
+  if (levels < 1) throw Error(NABO_HIP_ERR_ARGUMENT, "a recorded build needs at least one level");
+  Scratch sc(nullptr, scratch, scratch_bytes);
+  BuildScratch b;
+  b.want(sc, sh.n, sh.dim, sh.pts_stride, sh.clouds);
+  sc.commit(0);
+  NH_CHECK(hipMemcpy(b.d_offsets.p, h_offsets, sizeof(int64_t) * (size_t(sh.clouds) + 1),
+                     hipMemcpyHostToDevice));
+  kd_build_sequence(sh, d_cloud, stride, b, out, L, levels, nullptr, nullptr);
+}
 
 void kd_build(Index& ix, const float* d_cloud, int64_t stride,
               const std::vector<int64_t>& offsets, hipStream_t s) {
@@ -727,177 +1057,37 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
   ix.clouds = clouds;
   ix.cloud_offsets = offsets;
   ix.pts_stride = int((dim + 1 + 3) / 4 * 4);
+  const KdBuildShape sh{n, dim, ix.pts_stride, clouds, ix.bucket_size, ix.dim_bits, ix.dim_mask};
 
   NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_pts),
                      sizeof(float) * size_t(n + kPtsPad) * ix.pts_stride));
-  float* pts = ix.d_pts;
-
-  const int64_t max_nodes = 2 * n;
-  // Level buffers (double-buffered), per-segment stats, BFS node records,
-  // per-point arrays: one scratch block for the whole build.
   Scratch scratch(s);
-  Span<int32_t> seg_start[2], seg_count[2];
-  Span<float> seg_lo[2], seg_hi[2];
-  const int64_t seg_cap = std::max<int64_t>(n, clouds);
-  for (int b = 0; b < 2; ++b) {
-    scratch.want(seg_start[b], size_t(seg_cap));
-    scratch.want(seg_count[b], size_t(seg_cap));
-    scratch.want(seg_lo[b], size_t(seg_cap) * dim);
-    scratch.want(seg_hi[b], size_t(seg_cap) * dim);
-  }
-  Span<uint32_t> st_cd, st_minkey, st_maxkey, st_lt, st_eq, st_split, st_childbase;
-  Span<float> st_ideal, st_cut;
-  Span<int32_t> st_left;
-  for (Span<uint32_t>* a : {&st_cd, &st_minkey, &st_maxkey, &st_lt, &st_eq, &st_split,
-                            &st_childbase})
-    scratch.want(*a, size_t(seg_cap));
-  scratch.want(st_ideal, size_t(seg_cap));
-  scratch.want(st_cut, size_t(seg_cap));
-  scratch.want(st_left, size_t(seg_cap));
-  Span<uint32_t> b_cd, b_aux, b_count, b_size, b_pre;
-  Span<int32_t> b_left, b_right;
-  for (Span<uint32_t>* a : {&b_cd, &b_aux, &b_count, &b_size, &b_pre})
-    scratch.want(*a, size_t(max_nodes));
-  scratch.want(b_left, size_t(max_nodes));
-  scratch.want(b_right, size_t(max_nodes));
-
-  Span<int32_t> perm, perm2, pseg, slot_l, slot_r, cid;
-  for (Span<int32_t>* a : {&perm, &perm2, &pseg, &slot_l, &slot_r, &cid})
-    scratch.want(*a, size_t(n));
-  Span<float> xs, xs2;
-  scratch.want(xs, size_t(n));
-  scratch.want(xs2, size_t(n));
-  Span<uint64_t> flags, excl;
-  scratch.want(flags, size_t(n));
-  scratch.want(excl, size_t(n));
-  Span<uint32_t> keys, root_base;
-  scratch.want(keys, size_t(2) * clouds * dim);
-  scratch.want(root_base, size_t(clouds));
-  Span<float> bounds;
-  scratch.want(bounds, size_t(2) * clouds * dim);
-  Span<int64_t> d_offsets;
-  scratch.want(d_offsets, offsets.size());
-
-  // Scan scratch, sized for the largest scan (n uint64).
-  size_t scan_bytes64 = 0, scan_bytes32 = 0;
-  NH_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes64, flags.p, excl.p, uint64_t(0), size_t(n),
-                                   rocprim::plus<uint64_t>(), s));
-  NH_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes32, st_split.p, st_childbase.p, uint32_t(0),
-                                   size_t(seg_cap), rocprim::plus<uint32_t>(), s));
-  Span<uint8_t> scan_tmp;
-  scratch.want(scan_tmp, std::max(scan_bytes64, scan_bytes32) + 16);
-  // Tile partials of the reductions: 2 per tile, dim groups for the bounds.
-  const int64_t nparts = 2 * int64_t(tiles_for(n));
-  Span<int32_t> part_seg;
-  Span<uint32_t> part_a, part_b;
-  scratch.want(part_seg, size_t(nparts) * dim);
-  scratch.want(part_a, size_t(nparts) * dim);
-  scratch.want(part_b, size_t(nparts) * dim);
+  BuildScratch sc;
+  sc.want(scratch, n, dim, ix.pts_stride, clouds);
   int dev = 0;
   NH_CHECK(hipGetDevice(&dev));
   scratch.commit(dev);
-  NH_CHECK(hipMemcpyAsync(d_offsets.p, offsets.data(), sizeof(int64_t) * offsets.size(),
+  NH_CHECK(hipMemcpyAsync(sc.d_offsets.p, offsets.data(), sizeof(int64_t) * offsets.size(),
                           hipMemcpyHostToDevice, s));
-  const TileParts parts{part_seg.p, part_a.p, part_b.p};
-  const unsigned gpart = unsigned(ceil_div(nparts, kTile));
-
-  SegStat st{st_cd.p, st_ideal.p, st_cut.p, st_minkey.p, st_maxkey.p, st_lt.p, st_eq.p,
-             st_split.p, st_childbase.p, st_left.p};
-  Bfs bfs{b_cd.p, b_aux.p, b_count.p, b_left.p, b_right.p, b_size.p, b_pre.p};
-  auto level = [&](int b) {
-    return Seg{seg_start[b].p, seg_count[b].p, seg_lo[b].p, seg_hi[b].p};
-  };
-
-  // Bounds pass (kdtree_cpu.cpp:256-267) and the root segments.
-  const int64_t cd_elems = int64_t(clouds) * dim;
-  k_init_points<<<grid_for(n), kBlock, 0, s>>>(n, clouds, d_offsets.p, cid.p, perm.p, pseg.p);
-  k_init_keys<<<grid_for(cd_elems), kBlock, 0, s>>>(clouds, dim, keys.p);
-  k_bounds<<<tiles_for(n), kBlock, 0, s>>>(n, d_cloud, stride, dim, cid.p, keys.p, parts, nparts);
-  k_tile_partials<0, 1><<<dim3(gpart, unsigned(dim)), kBlock, 0, s>>>(nparts, parts, keys.p,
-                                                                      keys.p + dim, 2 * dim, 1);
-  k_roots<<<grid_for(cd_elems), kBlock, 0, s>>>(clouds, dim, d_offsets.p, keys.p, level(0),
-                                                 bounds.p);
-  NH_LAUNCH_CHECK();
-  std::vector<float> hb(size_t(2) * clouds * dim);
-  NH_CHECK(hipMemcpyAsync(hb.data(), bounds.p, sizeof(float) * hb.size(), hipMemcpyDeviceToHost, s));
-
-  std::vector<int64_t> level_base;
-  std::vector<int32_t> level_size;
-  int32_t S = clouds;
-  int64_t node_base = 0;
-  int cur = 0;
-  LevelMailbox mailbox;
-  while (S > 0) {
-    if (node_base + S > max_nodes || int64_t(level_base.size()) > n)
-      throw Error(NABO_HIP_ERR_HIP, "kd-tree build overflow");
-    level_base.push_back(node_base);
-    level_size.push_back(S);
-    Seg sg = level(cur);
-    Seg nx = level(cur ^ 1);
-    const unsigned gs = grid_for(S);
-    k_seg_classify<<<gs, kBlock, 0, s>>>(S, sg, st, bfs, node_base, ix.bucket_size, dim);
-    size_t bytes = scan_tmp.count;
-    NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, st.split, st.childbase, uint32_t(0),
-                                     size_t(S), rocprim::plus<uint32_t>(), s));
-    const uint32_t seq = mailbox.next();
-    k_split_total<<<1, 64, 0, s>>>(S, st.split, st.childbase, mailbox.box(), seq);
-    NH_LAUNCH_CHECK();
-    const int32_t T = mailbox.wait(seq, s);
-    if (T > 0) {
-      const unsigned gp = grid_for(n);
-      const unsigned gt = tiles_for(n);
-      k_pt_gather<<<gt, kBlock, 0, s>>>(n, pseg.p, perm.p, d_cloud, stride, st, xs.p, parts);
-      k_tile_partials<0, 1><<<gpart, kBlock, 0, s>>>(nparts, parts, st.minkey, st.maxkey, 1, 0);
-      k_seg_cut<<<gs, kBlock, 0, s>>>(S, st);
-      k_pt_count<<<gt, kBlock, 0, s>>>(n, pseg.p, xs.p, st, parts);
-      k_tile_partials<2, 2><<<gpart, kBlock, 0, s>>>(nparts, parts, st.lt, st.eq, 1, 0);
-      for (int pass = 1; pass <= 2; ++pass) {
-        k_pt_flags<<<gp, kBlock, 0, s>>>(n, pass, pseg.p, xs.p, sg, st, flags.p);
-        bytes = scan_tmp.count;
-        NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, flags.p, excl.p, uint64_t(0),
-                                         size_t(n), rocprim::plus<uint64_t>(), s));
-        k_pt_slots<<<gp, kBlock, 0, s>>>(n, pseg.p, sg, flags.p, excl.p, slot_l.p, slot_r.p);
-        k_pt_move<<<gp, kBlock, 0, s>>>(n, pseg.p, sg, flags.p, excl.p, slot_l.p, slot_r.p,
-                                        perm.p, xs.p, perm2.p, xs2.p);
-        std::swap(perm.p, perm2.p);
-        std::swap(xs.p, xs2.p);
-      }
-      k_seg_children<<<gs, kBlock, 0, s>>>(S, sg, st, bfs, node_base, node_base + S, nx, dim);
-      k_pt_reseg<<<gp, kBlock, 0, s>>>(n, pseg.p, sg, st);
-      NH_LAUNCH_CHECK();
-    }
-    node_base += S;
-    S = 2 * T;
-    cur ^= 1;
-  }
-  const int64_t count = node_base;
-  const int levels = int(level_base.size());
-
-  for (int L = levels - 1; L >= 0; --L)
-    k_subtree_size<<<grid_for(level_size[L]), kBlock, 0, s>>>(level_base[L], level_size[L], bfs,
-                                                              dim);
-  // Trees one after the other: the root of cloud c (BFS id c) starts at the
-  // sum of the sizes of the trees before it.
-  size_t bytes = scan_tmp.count;
-  NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, bfs.size, root_base.p, uint32_t(0),
-                                   size_t(clouds), rocprim::plus<uint32_t>(), s));
-  for (int L = 0; L < levels; ++L)
-    k_preorder<<<grid_for(level_size[L]), kBlock, 0, s>>>(level_base[L], level_size[L], bfs, dim,
-                                                          L == 0 ? root_base.p : nullptr);
-  NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_nodes), sizeof(uint2) * (count + 1)));
-  NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_links), sizeof(uint4) * count));
-  k_encode<<<grid_for(count), kBlock, 0, s>>>(count, bfs, ix.dim_bits, dim, ix.d_nodes);
-  k_links_self<<<grid_for(count), kBlock, 0, s>>>(count, ix.d_nodes, ix.dim_bits, ix.dim_mask,
-                                                  dim, ix.d_links, ix.d_nodes + count);
-  k_links_parent<<<grid_for(count), kBlock, 0, s>>>(count, ix.d_nodes, ix.dim_bits, ix.dim_mask,
-                                                    dim, ix.d_links);
-  k_pack_points<<<grid_for(n + kPtsPad), kBlock, 0, s>>>(n, perm.p, cid.p, d_offsets.p, d_cloud, stride,
-                                                dim, ix.pts_stride, pts);
+  KdBuildOut out{};
+  out.pts = ix.d_pts;
   if (clouds > 1) {
     NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_roots), sizeof(int32_t) * clouds));
-    k_roots_out<<<grid_for(clouds), kBlock, 0, s>>>(clouds, root_base.p, ix.d_roots);
+    out.roots = ix.d_roots;
+    if (dim == 2 || dim == 3) {
+      NH_CHECK(hipMalloc(reinterpret_cast<void**>(&ix.d_boxes), sizeof(float) * 6 * clouds));
+      out.boxes = ix.d_boxes;
+    }
   }
-  NH_LAUNCH_CHECK();
+  Launcher L(s);
+  int levels = 0;
+  int64_t count = 0;
+  kd_build_sequence(sh, d_cloud, stride, sc, out, L, 0, &levels, &count);
+  ix.d_nodes = out.nodes;
+  ix.d_links = out.links;
+  std::vector<float> hb(size_t(2) * clouds * dim);
+  NH_CHECK(hipMemcpyAsync(hb.data(), sc.bounds.p, sizeof(float) * hb.size(), hipMemcpyDeviceToHost,
+                          s));
   NH_CHECK(hipStreamSynchronize(s));
   // A cloud of at most bucketSize points is one bucket and keeps the initial
   // bounds (kdtree_cpu.cpp:236-243).
@@ -912,7 +1102,6 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
   }
   ix.node_count = count;
   ix.depth = levels - 1;
-  kd_make_boxes(ix, s);
 }
 
 void kd_make_boxes(Index& ix, hipStream_t s) {
