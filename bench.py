@@ -302,7 +302,7 @@ def bench_batched():
     k = 8
     best, table = autotune.tune_batch(pts, off, qs, qoff, k)
     log(f"autotune best {best}")
-    params = {"bucketSize": best["bucket_size"], "querySort": best["query_sort"]}
+    params = autotune.params_of(best)
     # batched: build + search
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -317,7 +317,8 @@ def bench_batched():
     knn_ms = (time.perf_counter() - t0) * 1e3
     # plain: one index and one knn call per cloud, same parameters
     lib = nabo.load_library()
-    p = nabo.make_params(bucket_size=best["bucket_size"], query_sort=bool(best["query_sort"]))
+    p = nabo.make_params(bucket_size=best["bucket_size"], query_sort=bool(best["query_sort"]),
+                         wave_queries=best["wave_queries"], stack_place=best["stack_place"])
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     pi = torch.empty_like(bi)
     pd = torch.empty_like(bd)
@@ -348,11 +349,31 @@ def bench_batched():
     total_q = int(qoff[-1])
     batched_total = build_ms + knn_ms
     plain_total = plain_build_ms + plain_knn_ms
+    # One graph: every build and the search in one launch (the plan's stacks
+    # are in LDS; its level budget is the data's depth + 1 + 4).
+    plan_params = dict(params, stackPlace="auto")
+    plan = nabo.BatchPlan(pts, off, qs, qoff, k, params=plan_params)
+    plan.launch()
+    complete, levels = plan.status()
+    reps = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.launch()
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / reps
+    complete, _ = plan.status()
+    graph_same = bool(complete and torch.equal(plan.indices, bi) and torch.equal(plan.dists2, bd))
+    plan.close()
     return {"workload": "512_uneven_clouds_2k_200k_k8", "clouds": len(clouds),
             "points": int(off[-1]), "queries": total_q,
             "autotune": {"best": best, "table": table},
             "batched_ms": {"build": round(build_ms, 2), "knn": round(knn_ms, 2),
                            "total": round(batched_total, 2)},
+            "graph": {"ms_per_launch": round(graph_ms, 2), "levels": levels,
+                      "note": "build + search of all clouds, one HIP graph launch",
+                      "results_equal_batched": graph_same,
+                      "speedup_vs_plain": round(plain_total / graph_ms, 2)},
             "plain_ms": {"build": round(plain_build_ms, 2), "knn": round(plain_knn_ms, 2),
                          "total": round(plain_total, 2)},
             "speedup_total": round(plain_total / batched_total, 2),

@@ -69,7 +69,9 @@ inline constexpr ValueType invalidValue() {
 
 // Parameter map (nabo/nabo.h:231-254). Keys read: "bucketSize" (unsigned,
 // the reference's key), "querySort" (int), "bfPrecision" (int, one of
-// NABO_HIP_BF_*), "bfExtra" (unsigned), "device" (int).
+// NABO_HIP_BF_*), "bfExtra" (unsigned), "device" (int), "heap" (int,
+// NABO_HIP_HEAP_*), and the kd-tree search's schedule: "waveQueries",
+// "stackPlace" (NABO_HIP_STACK_*), "stlLog" (int; nabo_hip.h).
 struct Parameters : public std::map<std::string, std::any> {
   Parameters() {}
   Parameters(const std::string& key, const std::any& value) { (*this)[key] = value; }
@@ -305,6 +307,9 @@ struct HipSearch : public NearestNeighbourSearch<T, CloudType> {
     p.bf_extra = int(params.get<unsigned>("bfExtra", 32u));
     p.device = params.get<int>("device", -1);
     p.heap = params.get<int>("heap", heap);
+    p.wave_queries = params.get<int>("waveQueries", 0);
+    p.stack_place = params.get<int>("stackPlace", NABO_HIP_STACK_AUTO);
+    p.stl_log = params.get<int>("stlLog", 0);
     const float* data = reinterpret_cast<const float*>(&cloud.coeff(0, 0));
     check(nabo_hip_create(data, int64_t(cloud.cols()), int32_t(cloud.rows()),
                           int64_t(cloud.outerStride()), int32_t(dim), hipType,

@@ -17,9 +17,10 @@
  * their temporaries from a per-device pool of scratch blocks that the
  * library keeps between calls (at most eight blocks of at most 1 GiB per
  * device, until the process ends). A kd-tree knn call with the IndexHeapSTL
- * heap (NABO_HIP_HEAP_STL) and k <= 96 also takes 8 * (min(256, max(64, 8k))
- * + k) bytes per resident search lane (about 0.2 GB for k = 16 to 96 on MI355X)
- * for its admission logs.
+ * heap (NABO_HIP_HEAP_STL) and k <= 96 also takes 8 * (L + k) bytes per
+ * resident search lane for its admission logs, L = nabo_hip_params.stl_log
+ * or by default min(256, max(64, 8k)) (about 0.2 GB for k = 16 to 96 on
+ * MI355X).
  */
 #ifndef NABO_HIP_H
 #define NABO_HIP_H
@@ -82,8 +83,18 @@ typedef struct nabo_hip_params {
   int32_t bf_extra;      /* m: shortlist size is k + m for bf16/fp16 */
   int32_t device;        /* HIP device ordinal, -1 = current */
   int32_t heap;          /* NABO_HIP_HEAP_*; STL brute force needs bf_precision FP32 */
-  int32_t reserved[10];
+  int32_t wave_queries;  /* kd-tree search: queries a wavefront claims at a time, 64 x 1, 2,
+                            4 or 8 (0: 64) */
+  int32_t stack_place;   /* kd-tree search: NABO_HIP_STACK_* */
+  int32_t stl_log;       /* kd-tree IndexHeapSTL search, k <= 96: admission-log entries per
+                            lane, 16 to 4096 (0: 8k clamped to 64..256) */
+  int32_t reserved[7];
 } nabo_hip_params;
+
+/* Where the kd-tree search keeps its per-lane stacks (nabo_hip_params.
+ * stack_place): AUTO in LDS when the tree is at most 32 levels deep (else in
+ * HBM), HBM always in device memory (more lanes resident, slower stack). */
+enum { NABO_HIP_STACK_AUTO = 0, NABO_HIP_STACK_HBM = 1 };
 
 /* Message of the last failure on this thread ("" if none). */
 const char* nabo_hip_last_error(void);
@@ -173,6 +184,44 @@ int nabo_hip_batch_knn(const nabo_hip_index* index, const float* d_query, int32_
                        float epsilon, uint32_t option_flags, float max_radius,
                        int32_t* d_indices, float* d_dists2, uint64_t* d_touched,
                        void* hip_stream);
+
+/* Batched build + search recorded once as a HIP graph. A plan records the
+ * batched kd-tree build of `clouds` clouds (as nabo_hip_batch_create) and
+ * the search of their query sets (as nabo_hip_batch_knn: query set c =
+ * columns [query_offsets[c], query_offsets[c+1]) of d_query, host offsets)
+ * as the kernel nodes of one graph, with every array allocated at creation.
+ * Each nabo_hip_batch_plan_launch then rebuilds all trees from the current
+ * contents of d_points and searches the current d_query into d_indices /
+ * d_dists2 with one graph launch: no host round trip, no allocation. Change
+ * the buffers' contents (same sizes) between launches to process new data;
+ * they must stay valid until the plan is destroyed. No counterpart in the
+ * reference, which builds one tree per NearestNeighbourSearch object
+ * (nabo/kdtree_cpu.cpp:227-272); for callers that rebuild and search many
+ * clouds per frame. The graph launches a fixed number of build levels:
+ * `levels` (1 to 33), or 0 to build the data once at creation and take its
+ * depth + 1 plus a margin of 4. A launch whose data needs more levels
+ * builds and searches nothing (results stay as they were) and
+ * nabo_hip_batch_plan_status reports complete = 0. Results equal
+ * nabo_hip_batch_knn's; KDTREE_HIP (linear heap) semantics, 2-D and 3-D
+ * clouds, trees of at most 32 split levels, no touched count. Launches of
+ * one plan share its memory: do not overlap them (consecutive launches on
+ * one stream are ordered). */
+typedef struct nabo_hip_batch_plan nabo_hip_batch_plan;
+int nabo_hip_batch_plan_create(const float* d_points, const int64_t* offsets, int32_t clouds,
+                               int32_t rows, int64_t stride, int32_t dim,
+                               const nabo_hip_params* params, const float* d_query,
+                               int64_t qstride, const int64_t* query_offsets, int32_t k,
+                               float epsilon, uint32_t option_flags, float max_radius,
+                               int32_t* d_indices, float* d_dists2, int32_t levels,
+                               nabo_hip_batch_plan** out);
+/* Enqueues one launch of the plan's graph on hip_stream. */
+int nabo_hip_batch_plan_launch(nabo_hip_batch_plan* plan, void* hip_stream);
+/* Waits for the plan's last launch. *complete (nullable): 1 if its build
+ * finished within the recorded levels (results valid), else 0 (also before
+ * any launch); *levels (nullable): the recorded level count. */
+int nabo_hip_batch_plan_status(nabo_hip_batch_plan* plan, int32_t* complete, int32_t* levels);
+/* Waits for the plan's last launch and frees it. NULL is accepted. */
+int nabo_hip_batch_plan_destroy(nabo_hip_batch_plan* plan);
 
 /* Root node of each batched tree (clouds entries; {0} for a single tree). */
 int nabo_hip_batch_roots(const nabo_hip_index* index, int32_t* roots);

@@ -22,7 +22,7 @@ import numpy as np
 
 __all__ = [
     "SearchType", "SearchOptionFlags", "CreationOptionFlags", "NearestNeighbourSearch",
-    "NaboError", "load_library", "LIB_PATH", "Params",
+    "NaboError", "load_library", "LIB_PATH", "Params", "BatchSearch", "BatchPlan", "Server",
 ]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
@@ -75,7 +75,10 @@ class Params(C.Structure):
         ("bf_extra", C.c_int32),
         ("device", C.c_int32),
         ("heap", C.c_int32),
-        ("reserved", C.c_int32 * 10),
+        ("wave_queries", C.c_int32),
+        ("stack_place", C.c_int32),
+        ("stl_log", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -122,6 +125,8 @@ EXPORTED = [
     "nabo_hip_server_create", "nabo_hip_server_destroy", "nabo_hip_server_add_cloud",
     "nabo_hip_server_submit", "nabo_hip_server_wait", "nabo_hip_server_get_stats",
     "nabo_hip_serialize", "nabo_hip_deserialize", "nabo_hip_knn_device_noalloc",
+    "nabo_hip_batch_plan_create", "nabo_hip_batch_plan_launch", "nabo_hip_batch_plan_status",
+    "nabo_hip_batch_plan_destroy",
 ]
 
 _LIBS: dict = {}
@@ -159,6 +164,12 @@ def load_library(path: str | None = None):
                                         vp, vp]
     lib.nabo_hip_knn_device_noalloc.argtypes = [vp, vp, i32, i64, i64, i32, f32, u32, f32, vp,
                                                 vp, vp, vp, vp]
+    lib.nabo_hip_batch_plan_create.argtypes = [vp, vp, i32, i32, i64, i32, C.POINTER(Params), vp,
+                                               i64, vp, i32, f32, u32, f32, vp, vp, i32,
+                                               C.POINTER(vp)]
+    lib.nabo_hip_batch_plan_launch.argtypes = [vp, vp]
+    lib.nabo_hip_batch_plan_status.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
+    lib.nabo_hip_batch_plan_destroy.argtypes = [vp]
     lib.nabo_hip_get_info.argtypes = [vp, C.POINTER(Info)]
     lib.nabo_hip_get_bounds.argtypes = [vp, vp, vp]
     lib.nabo_hip_get_tree.argtypes = [vp, vp, vp]
@@ -231,13 +242,23 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+STACK_PLACE = {"auto": 0, "hbm": 1}  # NABO_HIP_STACK_*
+
+
 def make_params(bucket_size: int = 8, query_sort: bool = True, bf_precision: str = "fp32",
-                bf_extra: int = 32, device: int = -1, heap="linear", **unknown) -> Params:
+                bf_extra: int = 32, device: int = -1, heap="linear", wave_queries: int = 0,
+                stack_place="auto", stl_log: int = 0, **unknown) -> Params:
     """Parameters map (nabo/nabo.h:231-254): only "bucketSize" is a reference key.
 
     ``heap``: "linear" (IndexHeapBruteForceVector results) or "stl"
-    (IndexHeapSTL results: heap order unless SORT_RESULTS).
+    (IndexHeapSTL results: heap order unless SORT_RESULTS). The kd-tree
+    search's schedule (results do not depend on it): ``wave_queries``
+    (queries a wavefront claims at a time: 64, 128, 256 or 512; 0 = 64),
+    ``stack_place`` ("auto": LDS stacks for trees up to 32 levels; "hbm"),
+    ``stl_log`` (IndexHeapSTL, k <= 96: admission-log entries per lane).
     """
+    if unknown:
+        raise NaboError(1, f"unknown parameters {sorted(unknown)}")
     p = Params()
     load_library().nabo_hip_params_default(C.byref(p))
     p.bucket_size = int(bucket_size)
@@ -246,12 +267,16 @@ def make_params(bucket_size: int = 8, query_sort: bool = True, bf_precision: str
     p.bf_extra = int(bf_extra)
     p.device = int(device)
     p.heap = HEAP[heap] if isinstance(heap, str) else int(heap)
+    p.wave_queries = int(wave_queries)
+    p.stack_place = STACK_PLACE[stack_place] if isinstance(stack_place, str) else int(stack_place)
+    p.stl_log = int(stl_log)
     return p
 
 
 _PARAM_KEYS = {"bucketSize": "bucket_size", "querySort": "query_sort",
                "bfPrecision": "bf_precision", "bfExtra": "bf_extra", "device": "device",
-               "heap": "heap"}
+               "heap": "heap", "waveQueries": "wave_queries", "stackPlace": "stack_place",
+               "stlLog": "stl_log"}
 
 
 class _Serialized:
@@ -585,6 +610,90 @@ class BatchSearch:
             None if touched is None else C.c_void_p(touched.data_ptr()),
             _stream_handle(stream, queries.device)))
         return indices, dists2
+
+
+class BatchPlan:
+    """A batched build + search recorded as one HIP graph (nabo_hip_batch_plan_*).
+
+    ``points`` (total, rows) and ``queries`` (m, rows): float32 CUDA tensors,
+    row-contiguous, read at every launch; ``offsets`` / ``query_offsets``:
+    clouds + 1 host offsets. ``indices`` / ``dists2`` (m, k) are allocated
+    here (or given) and rewritten at every launch. ``levels``: build levels
+    recorded (0: the data's depth + 1 + 4, from one build at creation).
+    """
+
+    def __init__(self, points, offsets, queries, query_offsets, k: int, epsilon: float = 0.0,
+                 optionFlags: int = 0, maxRadius: float = np.inf, params: dict | None = None,
+                 levels: int = 0, indices=None, dists2=None):
+        import torch
+
+        self._lib = load_library()
+        self._h = None
+        kw = {}
+        for key, val in (params or {}).items():
+            if key not in _PARAM_KEYS:
+                raise NaboError(1, f"unknown parameter {key!r}")
+            kw[_PARAM_KEYS[key]] = val
+        p = make_params(**kw)
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        qoff = np.ascontiguousarray(query_offsets, dtype=np.int64)
+        if off.ndim != 1 or off.size < 2 or qoff.shape != off.shape:
+            raise NaboError(1, "offsets and query_offsets need clouds + 1 entries each")
+        dev = points.device.index
+        for name, t in (("points", points), ("queries", queries)):
+            if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float32 \
+                    or t.dim() != 2 or t.device.index != dev:
+                _tensor_fail(f"{name} must be a 2-D float32 CUDA tensor on cuda:{dev}")
+            if t.shape[0] > 0 and (t.stride(1) != 1 or t.stride(0) < t.shape[1]):
+                _tensor_fail(f"{name} rows must be contiguous")
+        total, rows = points.shape
+        m = queries.shape[0]
+        if int(off[-1]) != total or int(qoff[-1]) != m:
+            raise NaboError(1, "the last offsets must equal the numbers of points and queries")
+        if queries.shape[1] != rows:
+            raise NaboError(1, "queries must have the points' rows")
+        k = int(k)
+        if indices is None:
+            indices = torch.empty((m, k), dtype=torch.int32, device=points.device)
+        if dists2 is None:
+            dists2 = torch.empty((m, k), dtype=torch.float32, device=points.device)
+        _check_out_tensor(indices, "indices", torch.int32, (m, k), dev)
+        _check_out_tensor(dists2, "dists2", torch.float32, (m, k), dev)
+        self.points, self.queries, self.indices, self.dists2 = points, queries, indices, dists2
+        h = C.c_void_p()
+        _check(self._lib, self._lib.nabo_hip_batch_plan_create(
+            C.c_void_p(points.data_ptr()), _ptr(off), len(off) - 1, rows, points.stride(0), rows,
+            C.byref(p), C.c_void_p(queries.data_ptr()), queries.stride(0), _ptr(qoff), k,
+            float(epsilon), int(optionFlags), float(maxRadius), C.c_void_p(indices.data_ptr()),
+            C.c_void_p(dists2.data_ptr()), int(levels), C.byref(h)))
+        self._h = h
+        self.offsets, self.query_offsets = off, qoff
+
+    def launch(self, stream=None):
+        """Rebuilds every tree from ``points`` and searches ``queries``: one
+        graph launch on ``stream`` (default: the current one)."""
+        _check(self._lib, self._lib.nabo_hip_batch_plan_launch(
+            self._h, _stream_handle(stream, self.points.device)))
+        return self.indices, self.dists2
+
+    def status(self):
+        """(complete, levels): waits for the last launch; complete is False when
+        its build needed more levels than recorded (results not written)."""
+        done, lv = C.c_int32(0), C.c_int32(0)
+        _check(self._lib, self._lib.nabo_hip_batch_plan_status(self._h, C.byref(done),
+                                                               C.byref(lv)))
+        return bool(done.value), lv.value
+
+    def close(self):
+        if self._h:
+            self._lib.nabo_hip_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ServedRequest:

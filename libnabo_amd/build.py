@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libnabo_hip.so")
 SOURCES = ["capi.hip", "kdtree_build.hip", "kdtree_search.hip", "brute_force.hip", "server.hip",
-           "noalloc.hip"]
+           "noalloc.hip", "plan.hip"]
 ARCH = os.environ.get("NABO_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

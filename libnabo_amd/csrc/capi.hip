@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -46,15 +47,6 @@ int guarded(F&& f) {
 
 [[noreturn]] void arg_error(const std::string& msg) { throw Error(NABO_HIP_ERR_ARGUMENT, msg); }
 
-uint32_t storage_bits(uint32_t v) {  // getStorageBitCount, nabo/kdtree_cpu.cpp:60-70
-  uint32_t b = 0;
-  while (v) {
-    ++b;
-    v >>= 1;
-  }
-  return b;
-}
-
 // Stream-ordered allocations are served from the device's default pool; keep
 // freed blocks in the pool instead of returning them at every synchronize.
 void prepare_device(int dev) {
@@ -80,54 +72,6 @@ hipStream_t thread_stream(int dev) {
   NH_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   streams.emplace_back(dev, s);
   return s;
-}
-
-struct DeviceScope {
-  int prev = -1;
-  explicit DeviceScope(int dev) {
-    NH_CHECK(hipGetDevice(&prev));
-    if (dev != prev) NH_CHECK(hipSetDevice(dev));
-  }
-  ~DeviceScope() {
-    int cur = -1;
-    if (hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-  }
-};
-
-void check_create(int64_t n, int32_t rows, int64_t stride, int32_t dim, int32_t type,
-                  const nabo_hip_params& p) {
-  if (dim <= 0) arg_error("Your space must have at least one dimension");
-  if (n == 0) arg_error("Cloud has no points");
-  if (rows == 0) arg_error("Cloud has 0 dimensions");
-  if (n < 0 || rows < 0) arg_error("Cloud has a negative size");
-  if (stride < rows) arg_error("Cloud column stride is smaller than its number of rows");
-  if (type >= 0 && type <= 5)
-    throw Error(NABO_HIP_ERR_UNSUPPORTED,
-                "search type " + std::to_string(type) +
-                    " is a CPU search of the reference; the HIP library offers KDTREE_HIP (6) "
-                    "and BRUTE_FORCE_HIP (7)");
-  if (type != NABO_HIP_SEARCH_KDTREE && type != NABO_HIP_SEARCH_BRUTE_FORCE)
-    arg_error("Unknown search type");
-  if (n >= (int64_t(1) << 30))
-    throw Error(NABO_HIP_ERR_UNSUPPORTED, "clouds of 2^30 points or more are not supported");
-  if (p.heap != NABO_HIP_HEAP_LINEAR && p.heap != NABO_HIP_HEAP_STL)
-    arg_error("Unknown heap type " + std::to_string(p.heap));
-  if (type == NABO_HIP_SEARCH_KDTREE) {
-    const uint32_t bucket = p.bucket_size;
-    if (bucket < 2)
-      arg_error("Requested bucket size " + std::to_string(bucket) + ", but must be larger than 2");
-    const uint32_t used_dim = uint32_t(std::min(dim, rows));
-    const uint32_t bits = storage_bits(used_dim);
-    if (n > int64_t(bucket)) {
-      const uint64_t max_nodes = (uint64_t(1) << (32 - bits)) - 1;
-      const uint64_t est = uint64_t(n) / (bucket / 2);
-      if (est > max_nodes)
-        arg_error("Cloud has a risk to have more nodes (" + std::to_string(est) +
-                  ") than the kd-tree allows (" + std::to_string(max_nodes) +
-                  "). The kd-tree has " + std::to_string(bits) + " bits for dimensions and " +
-                  std::to_string(32 - bits) + " bits for node indices");
-    }
-  }
 }
 
 void build_index(nabo_hip_index* h, const float* d_cloud, int64_t n, int32_t rows, int64_t stride,
@@ -219,50 +163,8 @@ SerialHeader serial_layout(const Index& ix) {
   return h;
 }
 
-void check_knn(const Index& ix, int32_t qrows, int64_t qstride, int64_t m, int32_t k,
-               uint32_t flags) {
-  const bool allow_self = flags & NABO_HIP_ALLOW_SELF_MATCH;
-  // Batched trees: every cloud must satisfy the reference's k rule.
-  int64_t npts = ix.n;
-  for (int32_t c = 0; c + 1 < int32_t(ix.cloud_offsets.size()); ++c)
-    npts = std::min(npts, ix.cloud_offsets[c + 1] - ix.cloud_offsets[c]);
-  if (allow_self) {
-    if (k > npts)
-      arg_error("Requesting more points (" + std::to_string(k) + ") than available in cloud (" +
-                std::to_string(npts) + ")");
-  } else if (k > npts - 1) {
-    arg_error("Requesting more points (" + std::to_string(k) +
-              ") than available in cloud minus 1 (" + std::to_string(npts - 1) +
-              ") (as self match is forbidden)");
-  }
-  if (qrows < ix.dim)
-    arg_error("Query has less dimensions (" + std::to_string(qrows) +
-              ") than requested for cloud (" + std::to_string(ix.dim) + ")");
-  if (flags > 3u)
-    arg_error("OR-ed value of option flags (" + std::to_string(flags) +
-              ") is larger than maximal valid value (3)");
-  if (k < 0) arg_error("k must not be negative");
-  if (m < 0) arg_error("Query has a negative number of columns");
-  if (m > 0 && qstride < qrows) arg_error("Query column stride is smaller than its number of rows");
-}
-
 __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
 
-// Cloud id of each query of a batched search (binary search in the offsets).
-__global__ void k_query_clouds(int64_t m, int32_t clouds, const int64_t* __restrict__ offsets,
-                               int32_t* __restrict__ qcloud) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  int lo = 0, hi = clouds - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) / 2;
-    if (offsets[mid] <= i)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  qcloud[i] = lo;
-}
 
 // Stream capture is refused: replays of captured knn calls (which allocate
 // their scratch stream-ordered) made illegal memory accesses on the GPU,
@@ -294,6 +196,89 @@ void run_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
 }
 
 }  // namespace
+
+
+void check_create(int64_t n, int32_t rows, int64_t stride, int32_t dim, int32_t type,
+                  const nabo_hip_params& p) {
+  if (dim <= 0) arg_error("Your space must have at least one dimension");
+  if (n == 0) arg_error("Cloud has no points");
+  if (rows == 0) arg_error("Cloud has 0 dimensions");
+  if (n < 0 || rows < 0) arg_error("Cloud has a negative size");
+  if (stride < rows) arg_error("Cloud column stride is smaller than its number of rows");
+  if (type >= 0 && type <= 5)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                "search type " + std::to_string(type) +
+                    " is a CPU search of the reference; the HIP library offers KDTREE_HIP (6) "
+                    "and BRUTE_FORCE_HIP (7)");
+  if (type != NABO_HIP_SEARCH_KDTREE && type != NABO_HIP_SEARCH_BRUTE_FORCE)
+    arg_error("Unknown search type");
+  if (n >= (int64_t(1) << 30))
+    throw Error(NABO_HIP_ERR_UNSUPPORTED, "clouds of 2^30 points or more are not supported");
+  if (p.heap != NABO_HIP_HEAP_LINEAR && p.heap != NABO_HIP_HEAP_STL)
+    arg_error("Unknown heap type " + std::to_string(p.heap));
+  if (p.wave_queries != 0 && p.wave_queries != 64 && p.wave_queries != 128 &&
+      p.wave_queries != 256 && p.wave_queries != 512)
+    arg_error("wave_queries must be 0, 64, 128, 256 or 512, not " +
+              std::to_string(p.wave_queries));
+  if (p.stack_place != NABO_HIP_STACK_AUTO && p.stack_place != NABO_HIP_STACK_HBM)
+    arg_error("Unknown stack placement " + std::to_string(p.stack_place));
+  if (p.stl_log != 0 && (p.stl_log < 16 || p.stl_log > 4096))
+    arg_error("stl_log must be 0 or 16 to 4096, not " + std::to_string(p.stl_log));
+  if (type == NABO_HIP_SEARCH_KDTREE) {
+    const uint32_t bucket = p.bucket_size;
+    if (bucket < 2)
+      arg_error("Requested bucket size " + std::to_string(bucket) + ", but must be larger than 2");
+    const uint32_t used_dim = uint32_t(std::min(dim, rows));
+    const uint32_t bits = storage_bits(used_dim);
+    if (n > int64_t(bucket)) {
+      const uint64_t max_nodes = (uint64_t(1) << (32 - bits)) - 1;
+      const uint64_t est = uint64_t(n) / (bucket / 2);
+      if (est > max_nodes)
+        arg_error("Cloud has a risk to have more nodes (" + std::to_string(est) +
+                  ") than the kd-tree allows (" + std::to_string(max_nodes) +
+                  "). The kd-tree has " + std::to_string(bits) + " bits for dimensions and " +
+                  std::to_string(32 - bits) + " bits for node indices");
+    }
+  }
+}
+
+void check_knn(const Index& ix, int32_t qrows, int64_t qstride, int64_t m, int32_t k,
+               uint32_t flags) {
+  const bool allow_self = flags & NABO_HIP_ALLOW_SELF_MATCH;
+  // Batched trees: every cloud must satisfy the reference's k rule.
+  int64_t npts = ix.n;
+  for (int32_t c = 0; c + 1 < int32_t(ix.cloud_offsets.size()); ++c)
+    npts = std::min(npts, ix.cloud_offsets[c + 1] - ix.cloud_offsets[c]);
+  if (allow_self) {
+    if (k > npts)
+      arg_error("Requesting more points (" + std::to_string(k) + ") than available in cloud (" +
+                std::to_string(npts) + ")");
+  } else if (k > npts - 1) {
+    arg_error("Requesting more points (" + std::to_string(k) +
+              ") than available in cloud minus 1 (" + std::to_string(npts - 1) +
+              ") (as self match is forbidden)");
+  }
+  if (qrows < ix.dim)
+    arg_error("Query has less dimensions (" + std::to_string(qrows) +
+              ") than requested for cloud (" + std::to_string(ix.dim) + ")");
+  if (flags > 3u)
+    arg_error("OR-ed value of option flags (" + std::to_string(flags) +
+              ") is larger than maximal valid value (3)");
+  if (k < 0) arg_error("k must not be negative");
+  if (m < 0) arg_error("Query has a negative number of columns");
+  if (m > 0 && qstride < qrows) arg_error("Query column stride is smaller than its number of rows");
+}
+
+int run_guarded(const std::function<void()>& f) { return guarded(f); }
+
+uint32_t storage_bits(uint32_t v) {  // getStorageBitCount, nabo/kdtree_cpu.cpp:60-70
+  uint32_t b = 0;
+  while (v) {
+    ++b;
+    v >>= 1;
+  }
+  return b;
+}
 
 namespace {
 std::mutex g_scratch_mu;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -201,7 +202,35 @@ struct KnnArgs {
   uint64_t* touched = nullptr;   // device, nullable
   const int32_t* qcloud = nullptr;  // device, per-query cloud id (batched trees)
   const ExternalScratch* scratch = nullptr;  // null: the per-device scratch pool
+  const int32_t* ok = nullptr;  // device, nullable: the search is skipped while *ok == 0
+  size_t* measure = nullptr;    // set: kd_knn_launch only computes the call's scratch bytes
 };
+
+// Makes `dev` the current device for the scope, and restores the previous
+// one after.
+struct DeviceScope {
+  int prev = -1;
+  explicit DeviceScope(int dev) {
+    NH_CHECK(hipGetDevice(&prev));
+    if (dev != prev) NH_CHECK(hipSetDevice(dev));
+  }
+  ~DeviceScope() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+// Runs f, mapping its exceptions to a status code and the thread's message
+// (the C ABI's entry points).
+int run_guarded(const std::function<void()>& f);
+// Argument checks of a create / knn call (nabo/nabo.cpp:68-80, 102-131).
+void check_create(int64_t n, int32_t rows, int64_t stride, int32_t dim, int32_t type,
+                  const nabo_hip_params& p);
+void check_knn(const Index& ix, int32_t qrows, int64_t qstride, int64_t m, int32_t k,
+               uint32_t flags);
+// Bits of the node word holding a cut dimension (getStorageBitCount,
+// nabo/kdtree_cpu.cpp:60-70).
+uint32_t storage_bits(uint32_t v);
 
 // Supplies the caller-owned scratch of a knn call on device `dev` (current
 // when called); throws Error on failure.
@@ -250,7 +279,8 @@ struct KdBuildOut {
   uint4* links = nullptr;     // eager: allocated by the build; recorded: 2n
   int32_t* roots = nullptr;   // per-cloud roots (nullable)
   float* boxes = nullptr;     // per-cloud query-order boxes, 6 floats each (nullable)
-  int32_t* status = nullptr;  // recorded builds: 1 when the levels did not suffice
+  int32_t* ok = nullptr;      // recorded builds: 1 when the levels sufficed, else 0
+                              // (node arrays then unwritten)
 };
 
 class Launcher;
@@ -270,8 +300,15 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
               const std::vector<int64_t>& offsets, hipStream_t s);
 // Fills d_boxes from the per-cloud bounds (batched 2-D / 3-D trees).
 void kd_make_boxes(Index& ix, hipStream_t s);
-// Searches a built kd-tree (kdtree_search.hip).
+// Cloud id of each of m queries from clouds + 1 device query offsets.
+__global__ void k_query_clouds(int64_t m, int32_t clouds, const int64_t* __restrict__ offsets,
+                               int32_t* __restrict__ qcloud);
+
+// Searches a built kd-tree (kdtree_search.hip); the _launch form launches
+// or records through a Launcher (a recorded search needs a.scratch, which
+// must live as long as the recording is replayed).
 void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s);
+void kd_knn_launch(const Index& ix, const KnnArgs& a, Launcher& L);
 // Prepares and searches a brute-force index (brute_force.hip).
 void bf_prepare(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s);
 void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s);

@@ -26,7 +26,10 @@
 // reads the number of splits of each level from a polled pinned mailbox and
 // stops at the first level without one; a build recorded in a HIP graph
 // launches a fixed number of levels, and kernels of levels past the last
-// return at once (a status word tells whether the levels sufficed).
+// return at once; a word in device memory tells whether the levels
+// sufficed, and when they did not, nothing is built from the partial
+// records (the finalize kernels, and the search that follows in a graph,
+// return at once).
 //
 // Node numbering: the reference emits nodes in pre-order with the left
 // child right after its parent (:215-223). With a full binary tree, the
@@ -640,7 +643,12 @@ __global__ void k_pt_reseg(int64_t n, const int32_t* __restrict__ gate,
   pseg[p] = int32_t(2 * st.childbase[s]) + (pos < st.left[s] ? 0 : 1);
 }
 
-__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t count, uint32_t v) {
+// The finalize kernels below return at once when *ok is 0 (a recorded
+// build whose levels did not finish the tree: its records are partial);
+// eager builds pass null.
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t count, uint32_t v,
+                           const int32_t* __restrict__ ok) {
+  if (ok && *ok == 0) return;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
        i += int64_t(gridDim.x) * blockDim.x)
     p[i] = v;
@@ -648,7 +656,8 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t count, uint32_t v) 
 
 // Nodes: fin->base (the BFS id after the last level) in all.
 __global__ void k_mark_leaves(const BuildLevel* __restrict__ fin, Bfs bfs, int dim,
-                              uint32_t* __restrict__ leaf_start) {
+                              uint32_t* __restrict__ leaf_start, const int32_t* __restrict__ ok) {
+  if (ok && *ok == 0) return;
   const int64_t count = fin->base;
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < count;
        v += int64_t(gridDim.x) * blockDim.x)
@@ -657,7 +666,8 @@ __global__ void k_mark_leaves(const BuildLevel* __restrict__ fin, Bfs bfs, int d
 
 __global__ void k_preorder(const BuildLevel* __restrict__ fin, Bfs bfs,
                            const uint32_t* __restrict__ leaves_before,
-                           const int32_t* __restrict__ cid) {
+                           const int32_t* __restrict__ cid, const int32_t* __restrict__ ok) {
+  if (ok && *ok == 0) return;
   const int64_t count = fin->base;
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < count;
        v += int64_t(gridDim.x) * blockDim.x) {
@@ -668,7 +678,8 @@ __global__ void k_preorder(const BuildLevel* __restrict__ fin, Bfs bfs,
 }
 
 __global__ void k_encode(const BuildLevel* __restrict__ fin, Bfs bfs, uint32_t dim_bits, int dim,
-                         uint2* __restrict__ nodes) {
+                         uint2* __restrict__ nodes, const int32_t* __restrict__ ok) {
+  if (ok && *ok == 0) return;
   const int64_t count = fin->base;
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < count;
        v += int64_t(gridDim.x) * blockDim.x) {
@@ -692,7 +703,9 @@ __global__ void k_encode(const BuildLevel* __restrict__ fin, Bfs bfs, uint32_t d
 // again. nodes[count] is a padding leaf, so nodes[p + 1] exists for every p.
 __global__ void k_links_self(const BuildLevel* __restrict__ fin, const uint2* __restrict__ nodes,
                              uint32_t dim_bits, uint32_t dim_mask, int dim,
-                             uint4* __restrict__ links, uint2* __restrict__ all_nodes) {
+                             uint4* __restrict__ links, uint2* __restrict__ all_nodes,
+                             const int32_t* __restrict__ ok) {
+  if (ok && *ok == 0) return;
   const int64_t count = fin->base;
   for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < count;
        p += int64_t(gridDim.x) * blockDim.x) {
@@ -710,7 +723,9 @@ __global__ void k_links_self(const BuildLevel* __restrict__ fin, const uint2* __
 
 __global__ void k_links_parent(const BuildLevel* __restrict__ fin,
                                const uint2* __restrict__ nodes, uint32_t dim_bits,
-                               uint32_t dim_mask, int dim, uint4* __restrict__ links) {
+                               uint32_t dim_mask, int dim, uint4* __restrict__ links,
+                               const int32_t* __restrict__ ok) {
+  if (ok && *ok == 0) return;
   const int64_t count = fin->base;
   for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < count;
        p += int64_t(gridDim.x) * blockDim.x) {
@@ -733,7 +748,8 @@ __global__ void k_pad_points(int64_t n, int ps, float* __restrict__ recs) {
 // Root of cloud c: the nodes of the trees before it, 2 leaves - 1 each.
 __global__ void k_roots_out(int32_t clouds, const int64_t* __restrict__ offsets,
                             const uint32_t* __restrict__ leaves_before,
-                            int32_t* __restrict__ roots) {
+                            int32_t* __restrict__ roots, const int32_t* __restrict__ ok) {
+  if (ok && *ok == 0) return;
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < clouds) roots[c] = int32_t(2u * leaves_before[offsets[c]] - uint32_t(c));
 }
@@ -761,10 +777,10 @@ __global__ void k_boxes(int32_t clouds, int dim, const int64_t* __restrict__ off
   boxes[c * 6 + 3 + d] = sc;
 }
 
-// 1 when the recorded levels did not finish the tree (the level after the
-// last one still has segments), else 0.
-__global__ void k_build_status(const BuildLevel* __restrict__ fin, int32_t* __restrict__ status) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *status = fin->S > 0 ? 1 : 0;
+// 1 when the recorded levels finished the tree (the level after the last
+// one has no segments), else 0.
+__global__ void k_build_ok(const BuildLevel* __restrict__ fin, int32_t* __restrict__ ok) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *ok = fin->S == 0 ? 1 : 0;
 }
 
 inline unsigned grid_for(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, kBlock))); }
@@ -894,8 +910,8 @@ struct BuildScratch {
 // *levels_out and *nodes_out receive the level count and node count, and the
 // node arrays are allocated once the count is known (out.nodes / out.links
 // null on entry). Recorded (levels > 0): exactly `levels` levels, node
-// arrays preallocated for 2n nodes, and out.status receives 1 if the tree
-// needed more levels.
+// arrays preallocated for 2n nodes; out.ok receives 0 if the tree needed
+// more levels, in which case the node arrays are left unwritten.
 void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t stride,
                        BuildScratch& sc, KdBuildOut& out, Launcher& L, int levels,
                        int* levels_out, int64_t* nodes_out) {
@@ -999,32 +1015,31 @@ void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t str
     NH_CHECK(hipMalloc(reinterpret_cast<void**>(&out.nodes), sizeof(uint2) * (node_base + 1)));
     NH_CHECK(hipMalloc(reinterpret_cast<void**>(&out.links), sizeof(uint4) * node_base));
   }
+  const int32_t* ok = out.ok;
+  if (out.ok) L(k_build_ok, dim3(1), dim3(64), 0, static_cast<const BuildLevel*>(fin), out.ok);
   const unsigned gn = unsigned(std::min<int64_t>(kGridStride, grid_for(2 * n)));
-  L(k_fill_u32, dim3(gn), dim3(kBlock), 0, sc.leaf_start.p, n + 1, 0u);
+  L(k_fill_u32, dim3(gn), dim3(kBlock), 0, sc.leaf_start.p, n + 1, 0u, ok);
   L(k_mark_leaves, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs, dim,
-    sc.leaf_start.p);
-  exclusive_scan<uint32_t>(L, sc.leaf_start.p, sc.leaves_before.p,
-                           ScanCount{nullptr, n + 1, nullptr}, sc.partials32.p,
-                           static_cast<uint32_t*>(nullptr));
+    sc.leaf_start.p, ok);
+  exclusive_scan<uint32_t>(L, sc.leaf_start.p, sc.leaves_before.p, ScanCount{nullptr, n + 1, ok},
+                           sc.partials32.p, static_cast<uint32_t*>(nullptr));
   L(k_preorder, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs,
-    static_cast<const uint32_t*>(sc.leaves_before.p), static_cast<const int32_t*>(sc.cid.p));
+    static_cast<const uint32_t*>(sc.leaves_before.p), static_cast<const int32_t*>(sc.cid.p), ok);
   L(k_encode, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs, sh.dim_bits,
-    dim, out.nodes);
+    dim, out.nodes, ok);
   L(k_links_self, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin),
-    static_cast<const uint2*>(out.nodes), sh.dim_bits, sh.dim_mask, dim, out.links, out.nodes);
+    static_cast<const uint2*>(out.nodes), sh.dim_bits, sh.dim_mask, dim, out.links, out.nodes, ok);
   L(k_links_parent, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin),
-    static_cast<const uint2*>(out.nodes), sh.dim_bits, sh.dim_mask, dim, out.links);
+    static_cast<const uint2*>(out.nodes), sh.dim_bits, sh.dim_mask, dim, out.links, ok);
   L(k_pad_points, dim3(grid_for(int64_t(kPtsPad) * ps)), dim3(kBlock), 0, n, ps, out.pts);
   if (out.roots)
     L(k_roots_out, dim3(grid_for(clouds)), dim3(kBlock), 0, clouds,
       static_cast<const int64_t*>(sc.d_offsets.p),
-      static_cast<const uint32_t*>(sc.leaves_before.p), out.roots);
+      static_cast<const uint32_t*>(sc.leaves_before.p), out.roots, ok);
   if (out.boxes)
     L(k_boxes, dim3(grid_for(int64_t(clouds) * 3)), dim3(kBlock), 0, clouds, dim,
       static_cast<const int64_t*>(sc.d_offsets.p), sh.bucket,
       static_cast<const float*>(sc.bounds.p), out.boxes);
-  if (out.status)
-    L(k_build_status, dim3(1), dim3(64), 0, static_cast<const BuildLevel*>(fin), out.status);
 }
 
 }  // namespace

@@ -33,12 +33,14 @@
 #include <cstring>
 #include <type_traits>
 
-#include <rocprim/device/device_scan.hpp>
-
 #include <cfloat>
+
+#include <optional>
 
 #include "device_common.h"
 #include "internal.h"
+#include "launch.h"
+#include "scan.h"
 
 namespace nabo_hip {
 namespace {
@@ -123,6 +125,11 @@ struct SearchParams {
   uint2* log_heap;     // LOG kernels: heaps of lanes whose log filled, [entry][grid lane]
   uint32_t log_cap;    // entries per lane
   uint32_t log_lanes;  // lanes of the grid
+  int rounds;          // sub-chunks of kWave queries per work claim
+  int hbm_stack;       // host side: stacks in HBM even when LDS would hold them
+  int log_entries;     // host side: admission-log entries per lane (0: by k)
+  const int32_t* ok;   // nullable: the search returns at once while *ok == 0
+                       // (a tree whose recorded build did not finish)
 };
 
 // Sorted insertion of (value, index): entries greater than value move up by
@@ -345,6 +352,7 @@ template <int DIM, int K, bool LDS_STACK, bool EXACT, bool LOG = false>
 __global__ __launch_bounds__(kSearchBlock, (K <= 32 ? 3 : K <= 64 ? 2 : 1)) void k_kd_search(SearchParams a) {
   // Per-lane stack: in LDS when the tree is shallow enough ([slot][lane] of
   // this workgroup), else in HBM ([slot][lane] over the grid).
+  if (a.ok && *a.ok == 0) return;
   extern __shared__ uint2 lds_stack[];
   char* const stack_base = reinterpret_cast<char*>(LDS_STACK ? lds_stack : a.stack);
   auto slot = [stack_base](uint32_t off) -> uint2& {
@@ -353,7 +361,11 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 32 ? 3 : K <= 64 ? 2 : 1)) void
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t lane_id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int xcd = blockIdx.x % kXcds;
-  const int64_t chunks = (a.m + kWave - 1) / kWave;
+  // Work units: chunks of a.rounds sub-chunks of kWave queries (one query
+  // per lane); a wave claims a chunk and searches its sub-chunks in turn.
+  const int rounds = a.rounds > 0 ? a.rounds : 1;
+  const int64_t subs = (a.m + kWave - 1) / kWave;
+  const int64_t chunks = (subs + rounds - 1) / rounds;
 
   const int dim = DIM > 0 ? DIM : a.dim;
   const uint32_t leaf_tag = uint32_t(dim);
@@ -378,27 +390,38 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 32 ? 3 : K <= 64 ? 2 : 1)) void
   int visited = 0;   // eighths found exhausted
   uint32_t claimed = 0;
   if (lane == 0) claimed = atomicAdd(&a.work[owner], 1u);
+  int64_t c = -1;
+  int r = rounds;  // next sub-chunk of chunk c (rounds: claim a chunk)
   for (;;) {
-    int64_t c = -1;
-    uint32_t got = __shfl(claimed, 0, kWave);
-    while (visited < kXcds) {
-      const int64_t begin = chunks * owner / kXcds;
-      const int64_t end = chunks * (owner + 1) / kXcds;
-      if (begin + int64_t(got) < end) {
-        c = begin + got;
-        break;
+    if (r == rounds) {
+      c = -1;
+      uint32_t got = __shfl(claimed, 0, kWave);
+      while (visited < kXcds) {
+        const int64_t begin = chunks * owner / kXcds;
+        const int64_t end = chunks * (owner + 1) / kXcds;
+        if (begin + int64_t(got) < end) {
+          c = begin + got;
+          break;
+        }
+        owner = (owner + 1) % kXcds;
+        if (++visited < kXcds) {
+          got = 0;
+          if (lane == 0) got = atomicAdd(&a.work[owner], 1u);
+          got = __shfl(got, 0, kWave);
+        }
       }
-      owner = (owner + 1) % kXcds;
-      if (++visited < kXcds) {
-        got = 0;
-        if (lane == 0) got = atomicAdd(&a.work[owner], 1u);
-        got = __shfl(got, 0, kWave);
-      }
+      if (c < 0) break;
+      r = 0;
     }
-    if (c < 0) break;
-    const int64_t i = c * kWave + lane;
+    // (wave-uniform) the last sub-chunk of this chunk: the next chunk is
+    // claimed before this one's results are stored
+    const int64_t sub = c * rounds + r;
+    ++r;
+    const bool last = r == rounds || sub + 1 >= subs;
+    if (last) r = rounds;
+    const int64_t i = sub * kWave + lane;
     if (i >= a.m) {
-      // Only the last chunk has such lanes, and its lane 0 is never one.
+      // Only the last sub-chunk has such lanes, and its lane 0 is never one.
       continue;
     }
     const int64_t qi = a.order ? a.order[i] : i;
@@ -527,7 +550,7 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 32 ? 3 : K <= 64 ? 2 : 1)) void
       }
     }
 
-    if (lane == 0) claimed = atomicAdd(&a.work[owner], 1u);
+    if (last && lane == 0) claimed = atomicAdd(&a.work[owner], 1u);
     int32_t* oi = a.out_idx + qi * a.k;
     float* od = a.out_d2 + qi * a.k;
     if constexpr (LOG) {
@@ -847,13 +870,15 @@ size_t search_lds(int depth, int k) {
 
 // Admission log entries per lane (LOG kernels); a full log is replayed and
 // emptied, so this bounds memory, not the number of admissions.
-uint32_t search_log_cap(int k) { return uint32_t(std::min(256, std::max(64, 8 * k))); }
+uint32_t search_log_cap(int k, int entries) {
+  return uint32_t(entries > 0 ? entries : std::min(256, std::max(64, 8 * k)));
+}
 
 template <int DIM, int K, bool EXACT, bool LOG>
 void launch_search_as(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& stack,
-                      const ExternalScratch* ext, hipStream_t s) {
+                      const ExternalScratch* ext, Launcher& L) {
   const size_t lds = search_lds<LOG>(depth, p.k);
-  const bool lds_stack = depth <= kLdsStackSlots;
+  const bool lds_stack = depth <= kLdsStackSlots && !p.hbm_stack;
   // Caller-owned scratch holds the call's fixed-size temporaries only: the
   // admission logs and HBM stacks scale with the resident grid.
   if (ext && LOG)
@@ -868,10 +893,10 @@ void launch_search_as(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>
   const unsigned grid = lds_stack ? resident_grid<DIM, K, true, EXACT, LOG>(lds, cus, dev)
                                   : resident_grid<DIM, K, false, EXACT, LOG>(lds, cus, dev);
   const int64_t lanes = int64_t(grid) * kSearchBlock;
-  Scratch scratch(s);  // the LOG kernels' admission logs and spilled heaps
+  Scratch scratch(L.stream());  // the LOG kernels' admission logs and spilled heaps
   Span<uint2> log, log_heap;
   if constexpr (LOG) {
-    p.log_cap = search_log_cap(p.k);
+    p.log_cap = search_log_cap(p.k, p.log_entries);
     p.log_lanes = uint32_t(lanes);
     scratch.want(log, size_t(p.log_cap) * size_t(lanes));
     scratch.want(log_heap, size_t(p.k) * size_t(lanes));
@@ -879,71 +904,73 @@ void launch_search_as(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>
     p.log = log.p;
     p.log_heap = log_heap.p;
   }
+  // measurement events bracket eager launches only
+  std::optional<KernelTimer> timer;
   if (lds_stack) {
-    KernelTimer timer(s);
-    k_kd_search<DIM, K, true, EXACT, LOG><<<grid, kSearchBlock, lds, s>>>(p);
+    if (!L.recording()) timer.emplace(L.stream());
+    L(k_kd_search<DIM, K, true, EXACT, LOG>, dim3(grid), dim3(kSearchBlock), lds, p);
   } else {
     if (uint64_t(lanes) * uint64_t(depth + 1) * sizeof(uint2) >= (uint64_t(1) << 32))
       throw Error(NABO_HIP_ERR_UNSUPPORTED, "kd-tree too deep for the search stack");
-    stack.alloc(size_t(lanes) * size_t(depth), s);
+    stack.alloc(size_t(lanes) * size_t(depth), L.stream());
     p.stack = stack.p;
     p.stack_lanes = lanes;
-    KernelTimer timer(s);
-    k_kd_search<DIM, K, false, EXACT, LOG><<<grid, kSearchBlock, lds, s>>>(p);
+    if (!L.recording()) timer.emplace(L.stream());
+    L(k_kd_search<DIM, K, false, EXACT, LOG>, dim3(grid), dim3(kSearchBlock), lds, p);
   }
 }
 
 template <int DIM, int K, bool LOG = false>
 void launch_search(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>& stack,
-                   const ExternalScratch* ext, hipStream_t s) {
+                   const ExternalScratch* ext, Launcher& L) {
   // epsilon = 0 (max_err2 == 1, so the multiply is exact) takes the
   // one-compare admission tests.
   if (p.max_err2 == 1.f)
-    launch_search_as<DIM, K, true, LOG>(p, cus, dev, depth, stack, ext, s);
+    launch_search_as<DIM, K, true, LOG>(p, cus, dev, depth, stack, ext, L);
   else
-    launch_search_as<DIM, K, false, LOG>(p, cus, dev, depth, stack, ext, s);
+    launch_search_as<DIM, K, false, LOG>(p, cus, dev, depth, stack, ext, L);
 }
 
 // Sorted k-best of K >= k entries (LOG: with the logged IndexHeapSTL).
 template <int DIM, bool LOG>
 void dispatch_sorted(const SearchParams& p, int cus, int dev, int depth, DevBuf<uint2>& stack,
-                     const ExternalScratch* ext, hipStream_t s) {
+                     const ExternalScratch* ext, Launcher& L) {
   const int k = p.k;
   if (k <= 1)
-    launch_search<DIM, 1, LOG>(p, cus, dev, depth, stack, ext, s);
+    launch_search<DIM, 1, LOG>(p, cus, dev, depth, stack, ext, L);
   else if (k <= 2)
-    launch_search<DIM, 2, LOG>(p, cus, dev, depth, stack, ext, s);
+    launch_search<DIM, 2, LOG>(p, cus, dev, depth, stack, ext, L);
   else if (k <= 4)
-    launch_search<DIM, 4, LOG>(p, cus, dev, depth, stack, ext, s);
+    launch_search<DIM, 4, LOG>(p, cus, dev, depth, stack, ext, L);
   else if (k <= 8)
-    launch_search<DIM, 8, LOG>(p, cus, dev, depth, stack, ext, s);
+    launch_search<DIM, 8, LOG>(p, cus, dev, depth, stack, ext, L);
   else if (k <= 16)
-    launch_search<DIM, 16, LOG>(p, cus, dev, depth, stack, ext, s);
+    launch_search<DIM, 16, LOG>(p, cus, dev, depth, stack, ext, L);
   else if (k <= 32)
-    launch_search<DIM, 32, LOG>(p, cus, dev, depth, stack, ext, s);
+    launch_search<DIM, 32, LOG>(p, cus, dev, depth, stack, ext, L);
   else if (k <= 64)
-    launch_search<DIM, 64, LOG>(p, cus, dev, depth, stack, ext, s);
+    launch_search<DIM, 64, LOG>(p, cus, dev, depth, stack, ext, L);
   else if (k <= 96)
-    launch_search<DIM, 96, LOG>(p, cus, dev, depth, stack, ext, s);
+    launch_search<DIM, 96, LOG>(p, cus, dev, depth, stack, ext, L);
   else if constexpr (LOG)
     throw Error(NABO_HIP_ERR_UNSUPPORTED, "the logged IndexHeapSTL search runs up to k = 96");
   else
-    launch_search<DIM, 128, LOG>(p, cus, dev, depth, stack, ext, s);
+    launch_search<DIM, 128, LOG>(p, cus, dev, depth, stack, ext, L);
 }
 
 template <int DIM>
 void dispatch_k(const SearchParams& p, bool stl, int cus, int dev, int depth,
-                DevBuf<uint2>& stack, const ExternalScratch* ext, hipStream_t s) {
+                DevBuf<uint2>& stack, const ExternalScratch* ext, Launcher& L) {
   // IndexHeapSTL: the logged form up to k = 96 (its heaps in the stacks'
   // LDS slots), else the heap in the output rows.
   static_assert(std::max(kLdsStackSlots, 96) * kSearchBlock * sizeof(uint2) <= 65536,
                 "96-entry heaps fit in 64 KiB of LDS");
   if (stl && p.k <= 96)
-    dispatch_sorted<DIM, true>(p, cus, dev, depth, stack, ext, s);
+    dispatch_sorted<DIM, true>(p, cus, dev, depth, stack, ext, L);
   else if (stl)
-    launch_search<DIM, -1>(p, cus, dev, depth, stack, ext, s);
+    launch_search<DIM, -1>(p, cus, dev, depth, stack, ext, L);
   else
-    dispatch_sorted<DIM, false>(p, cus, dev, depth, stack, ext, s);
+    dispatch_sorted<DIM, false>(p, cus, dev, depth, stack, ext, L);
 }
 
 uint32_t bits_needed(uint64_t v) {
@@ -957,7 +984,28 @@ uint32_t bits_needed(uint64_t v) {
 
 }  // namespace
 
+// Cloud id of each query of a batched search (binary search in the offsets).
+__global__ void k_query_clouds(int64_t m, int32_t clouds, const int64_t* __restrict__ offsets,
+                               int32_t* __restrict__ qcloud) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int lo = 0, hi = clouds - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (offsets[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  qcloud[i] = lo;
+}
+
 void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
+  Launcher L(s);
+  kd_knn_launch(ix, a, L);
+}
+
+void kd_knn_launch(const Index& ix, const KnnArgs& a, Launcher& L) {
   if (a.m == 0) return;
   if (a.scratch) {
     // refused before anything is launched (launch_search_as checks again)
@@ -965,11 +1013,11 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
       throw Error(NABO_HIP_ERR_UNSUPPORTED,
                   "IndexHeapSTL searches with k <= 96 keep per-lane admission logs, which a "
                   "caller-owned scratch does not hold");
-    if (ix.depth > kLdsStackSlots)
+    if (ix.depth > kLdsStackSlots || ix.params.stack_place == NABO_HIP_STACK_HBM)
       throw Error(NABO_HIP_ERR_UNSUPPORTED,
                   "trees deeper than " + std::to_string(kLdsStackSlots) +
-                      " levels keep their search stacks in HBM, which a caller-owned scratch "
-                      "does not hold");
+                      " levels, and indices made with HBM stacks, keep their search stacks in "
+                      "HBM, which a caller-owned scratch does not hold");
   }
   if (a.k > kMaxK)
     throw Error(NABO_HIP_ERR_UNSUPPORTED,
@@ -1000,9 +1048,17 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   p.touched = reinterpret_cast<unsigned long long*>(a.touched);
   p.allow_self = (a.flags & NABO_HIP_ALLOW_SELF_MATCH) ? 1 : 0;
   p.sort_results = (a.flags & NABO_HIP_SORT_RESULTS) ? 1 : 0;
+  p.ok = a.ok;
+  p.rounds = ix.params.wave_queries > 0 ? ix.params.wave_queries / kWave : 1;
+  p.hbm_stack = ix.params.stack_place == NABO_HIP_STACK_HBM ? 1 : 0;
+  p.log_entries = ix.params.stl_log;
   const bool stl = ix.params.heap == NABO_HIP_HEAP_STL;
-  if (ix.clouds > 1) {
-    if (!a.qcloud) throw Error(NABO_HIP_ERR_ARGUMENT, "batched trees need per-query cloud ids");
+  // Per-query cloud ids select the tree (batched trees, and single trees
+  // searched in batched form, whose roots and boxes are in device memory).
+  const bool batched = ix.clouds > 1 || (a.qcloud && ix.d_roots);
+  if (batched) {
+    if (!a.qcloud || !ix.d_roots)
+      throw Error(NABO_HIP_ERR_ARGUMENT, "batched trees need per-query cloud ids");
     p.roots = ix.d_roots;
     p.qcloud = a.qcloud;
   }
@@ -1039,12 +1095,11 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   const int64_t tiles = ceil_div(a.m, kSortTile);
   const int64_t ncounts = int64_t(coarse_bins) * tiles;
 
-  Scratch scratch(s, a.scratch ? a.scratch->base : nullptr, a.scratch ? a.scratch->bytes : 0);
-  Span<uint32_t> counters, keys, counts, starts, fkeys;
+  Scratch scratch(L.stream(), a.scratch ? a.scratch->base : nullptr,
+                  a.scratch ? a.scratch->bytes : 0);
+  Span<uint32_t> counters, keys, counts, starts, fkeys, partials;
   Span<int32_t> fidx, order;
-  Span<uint8_t> scan_tmp;
   scratch.want(counters, kXcds);
-  size_t scan_bytes = 0;
   if (sort) {
     scratch.want(keys, size_t(a.m));
     scratch.want(counts, size_t(ncounts));
@@ -1052,68 +1107,56 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     scratch.want(fkeys, size_t(a.m));
     scratch.want(fidx, size_t(a.m));
     scratch.want(order, size_t(a.m));
-    NH_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes, counts.p, starts.p, uint32_t(0),
-                                     size_t(ncounts), rocprim::plus<uint32_t>(), s));
-    scratch.want(scan_tmp, scan_bytes + 16);
+    scratch.want(partials, kScanGrid);
+  }
+  if (a.measure) {
+    *a.measure = scratch.bytes();
+    return;
   }
   int dev = 0;
   NH_CHECK(hipGetDevice(&dev));
   scratch.commit(dev);
 
   p.work = counters.p;
-  if (!sort) k_zero_counters<<<1, kBlock, 0, s>>>(counters.p, kXcds);
+  if (!sort) L(k_zero_counters, dim3(1), dim3(kBlock), 0, counters.p, int64_t(kXcds));
   if (sort) {
     const unsigned g = unsigned(ceil_div(a.m, kBlock));
     CurveBox box{axis_bits, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
     if (curve) {
-      // Cell scale of the cloud's box (one tree); batched trees read theirs
-      // from d_boxes (a cloud of at most bucketSize points keeps the
-      // reference's initial bounds, max < min: scale 0).
+      // Cell scale of the cloud's box (one tree, host bounds); batched
+      // trees read theirs from d_boxes (a cloud of at most bucketSize
+      // points keeps the reference's initial bounds, max < min: scale 0).
       const float top = float((1u << axis_bits) - 1u);
-      if (ix.clouds == 1) {
+      if (!batched) {
         for (int d = 0; d < ix.dim; ++d) {
           box.lo[d] = ix.min_bound[size_t(d)];
           const float ext = ix.max_bound[size_t(d)] - box.lo[d];
           box.sc[d] = ext > 0.f ? top / ext : 0.f;
         }
       }
-      if (ix.clouds > 1 && !ix.d_boxes) throw Error(NABO_HIP_ERR_HIP, "batched tree without boxes");
-      const float* d_boxes = ix.clouds > 1 ? ix.d_boxes : nullptr;
+      if (batched && !ix.d_boxes) throw Error(NABO_HIP_ERR_HIP, "batched tree without boxes");
+      const float* d_boxes = batched ? ix.d_boxes : nullptr;
       const uint32_t curve_bits = uint32_t(axis_bits * ix.dim);
-      if (ix.dim == 3)
-        k_sort_hist<3><<<unsigned(tiles), kHistBlock, 0, s>>>(
-            keys.p, a.query, a.qstride, box, p.qcloud, d_boxes, curve_bits, a.m, fine_bits,
-            coarse_bins, tiles, counts.p, counters.p);
-      else
-        k_sort_hist<2><<<unsigned(tiles), kHistBlock, 0, s>>>(
-            keys.p, a.query, a.qstride, box, p.qcloud, d_boxes, curve_bits, a.m, fine_bits,
-            coarse_bins, tiles, counts.p, counters.p);
+      L(ix.dim == 3 ? k_sort_hist<3> : k_sort_hist<2>, dim3(unsigned(tiles)), dim3(kHistBlock), 0,
+        keys.p, a.query, a.qstride, box, p.qcloud, d_boxes, curve_bits, a.m, fine_bits,
+        coarse_bins, tiles, counts.p, counters.p);
     } else {
-      if (ix.dim == 3)
-        k_locate<3><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
-                                         a.query, a.qstride, a.m, locate_shift, p.roots, p.qcloud,
-                                         keys.p);
-      else if (ix.dim == 2)
-        k_locate<2><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
-                                         a.query, a.qstride, a.m, locate_shift, p.roots, p.qcloud,
-                                         keys.p);
-      else
-        k_locate<0><<<g, kBlock, 0, s>>>(ix.d_nodes, ix.d_links, ix.dim_bits, ix.dim_mask, ix.dim,
-                                         a.query, a.qstride, a.m, locate_shift, p.roots, p.qcloud,
-                                         keys.p);
-      k_sort_hist<0><<<unsigned(tiles), kHistBlock, 0, s>>>(
-          keys.p, a.query, a.qstride, box, nullptr, nullptr, 0, a.m, fine_bits, coarse_bins,
-          tiles, counts.p, counters.p);
+      L(ix.dim == 3 ? k_locate<3> : ix.dim == 2 ? k_locate<2> : k_locate<0>, dim3(g), dim3(kBlock),
+        0, static_cast<const uint2*>(ix.d_nodes), static_cast<const uint4*>(ix.d_links),
+        ix.dim_bits, ix.dim_mask, ix.dim, a.query, a.qstride, a.m, locate_shift, p.roots,
+        p.qcloud, keys.p);
+      L(k_sort_hist<0>, dim3(unsigned(tiles)), dim3(kHistBlock), 0, keys.p, a.query, a.qstride,
+        box, static_cast<const int32_t*>(nullptr), static_cast<const float*>(nullptr),
+        uint32_t(0), a.m, fine_bits, coarse_bins, tiles, counts.p, counters.p);
     }
-    NH_LAUNCH_CHECK();
-    size_t bytes = scan_bytes;
-    NH_CHECK(rocprim::exclusive_scan(scan_tmp.p, bytes, counts.p, starts.p, uint32_t(0),
-                                     size_t(ncounts), rocprim::plus<uint32_t>(), s));
-    k_sort_scatter<<<unsigned(tiles), kHistBlock, 0, s>>>(keys.p, a.m, fine_bits, coarse_bins, tiles,
-                                                       starts.p, fkeys.p, fidx.p);
-    k_sort_fine<<<unsigned(coarse_bins), kHistBlock, 0, s>>>(fkeys.p, fidx.p, starts.p, tiles, a.m,
-                                                         fine_bits, coarse_bins, order.p);
-    NH_LAUNCH_CHECK();
+    exclusive_scan<uint32_t>(L, counts.p, starts.p, ScanCount{nullptr, ncounts, nullptr},
+                             partials.p, static_cast<uint32_t*>(nullptr));
+    L(k_sort_scatter, dim3(unsigned(tiles)), dim3(kHistBlock), 0,
+      static_cast<const uint32_t*>(keys.p), a.m, fine_bits, coarse_bins, tiles,
+      static_cast<const uint32_t*>(starts.p), fkeys.p, fidx.p);
+    L(k_sort_fine, dim3(unsigned(coarse_bins)), dim3(kHistBlock), 0,
+      static_cast<const uint32_t*>(fkeys.p), static_cast<const int32_t*>(fidx.p),
+      static_cast<const uint32_t*>(starts.p), tiles, a.m, fine_bits, coarse_bins, order.p);
     p.order = order.p;
   }
 
@@ -1122,12 +1165,11 @@ void kd_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   const int depth = std::max(1, ix.depth);
   DevBuf<uint2> stack;
   if (ix.dim == 3)
-    dispatch_k<3>(p, stl, cus, dev, depth, stack, a.scratch, s);
+    dispatch_k<3>(p, stl, cus, dev, depth, stack, a.scratch, L);
   else if (ix.dim == 2)
-    dispatch_k<2>(p, stl, cus, dev, depth, stack, a.scratch, s);
+    dispatch_k<2>(p, stl, cus, dev, depth, stack, a.scratch, L);
   else
-    dispatch_k<0>(p, stl, cus, dev, depth, stack, a.scratch, s);
-  NH_LAUNCH_CHECK();
+    dispatch_k<0>(p, stl, cus, dev, depth, stack, a.scratch, L);
 }
 
 }  // namespace nabo_hip

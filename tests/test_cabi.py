@@ -77,3 +77,24 @@ def test_heap_param_and_unknown_heap_rejected_before_any_device_call():
                              int(nabo.SearchType.KDTREE_HIP), 0, ctypes.byref(p), ctypes.byref(h))
     assert rc == 1 and b"Unknown heap type" in lib.nabo_hip_last_error()
     assert not h.value
+
+
+def test_schedule_params_checked_before_any_device_call():
+    import numpy as np
+
+    lib = nabo.load_library()
+    cloud = np.random.default_rng(0).random((50, 3)).astype(np.float32)
+    for field, bad in (("wave_queries", 96), ("stack_place", 5), ("stl_log", 8)):
+        p = nabo.make_params()
+        setattr(p, field, bad)
+        h = ctypes.c_void_p()
+        rc = lib.nabo_hip_create(cloud.ctypes.data_as(ctypes.c_void_p), 50, 3, 3, 3, 6, 0,
+                                 ctypes.byref(p), ctypes.byref(h))
+        assert rc == 1, field
+        assert field.split("_")[0] in lib.nabo_hip_last_error().decode()
+    try:
+        nabo.make_params(bogus=1)
+    except nabo.NaboError as e:
+        assert "bogus" in str(e)
+    else:
+        raise AssertionError("unknown parameter accepted")

@@ -184,3 +184,26 @@ def test_knn_no_query_sort(nabo, oracle):
     gi, gd = nns.knn(query, 10, 0.0, 0)
     oi, od = otree.knn(query, 10, 0.0, 0)
     assert_bit_exact(gi, gd, oi, od, "no sort")
+
+
+@pytest.mark.parametrize("params", [
+    {"waveQueries": 128}, {"waveQueries": 512}, {"stackPlace": "hbm"},
+    {"waveQueries": 256, "stackPlace": "hbm"},
+    {"heap": "stl", "stlLog": 16}, {"heap": "stl", "stlLog": 1024, "waveQueries": 128},
+])
+def test_knn_schedule_params(nabo, oracle, params):
+    """The search's schedule (queries per wavefront claim, stack placement,
+    admission-log size) changes neither results nor touched counts."""
+    cloud, query = datasets.lidar_scene(40_000, seed=12)
+    query = query[:12_345]  # a partial last wavefront and claim
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3,
+                                      dict({"bucketSize": 8}, **params),
+                                      creationOptionFlags=nabo.CreationOptionFlags.TOUCH_STATISTICS)
+    otree = oracle.kdtree(cloud, bucket_size=8)
+    heap = "stl" if params.get("heap") == "stl" else "linear"
+    print(f"case knn schedule {params}", flush=True)
+    for k, eps, r in ((16, 0.0, 1.0), (5, 0.1, np.inf)):
+        gi, gd = nns.knn(query, k, eps, 0, r)
+        oi, od, touched, _ = otree.knn(query, k, eps, 0, r, heap=heap, stats=True)
+        assert_bit_exact(gi, gd, oi, od, f"{params} k={k}")
+        assert nns.last_touched == int(touched.sum())
