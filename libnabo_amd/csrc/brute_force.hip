@@ -22,6 +22,7 @@
 // equal distances are kept.
 
 #include <cfloat>
+#include <type_traits>
 
 #include "device_common.h"
 #include "internal.h"
@@ -747,6 +748,42 @@ void shortlist_l(const ShortlistParams& s, const BfParams& p, hipStream_t st, in
     shortlist_k<F16, 64>(s, p, st, oi, od);
 }
 
+// Brute force beyond the tiled kernels' sizes (k above 64, dim above 128,
+// or a shortlist longer than 64): one query per thread scans the cloud in
+// index order (brute_force_cpu.cpp:81-98), its k-best in its own output
+// row, either sorted (BRUTE_FORCE_HIP: ascending, a new entry after equal
+// ones, so ties keep the lower index; the shortlist precisions take this
+// exact path too) or as IndexHeapSTL (BRUTE_FORCE: heap order, sort_heap
+// with SORT_RESULTS). Distances are the sequential sum the tiled kernels
+// compute. No tiling: it is the unbounded form, not the fast one.
+template <bool STL>
+__global__ __launch_bounds__(kBlock) void k_bf_rows(BfParams a, int sort, int32_t* out_idx,
+                                                    float* out_d2) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= a.m) return;
+  const float* qp = a.query + q * a.qstride;
+  float r2 = a.max_r2;
+  if (a.radii) {
+    const float r = a.radii[q];
+    r2 = r * r;
+  }
+  const RowHeapStore row{out_idx + q * a.k, out_d2 + q * a.k};
+  typename std::conditional<STL, StlHeap<RowHeapStore>, RowSortedHeap>::type heap;
+  heap.reset(row, a.k);
+  for (int64_t i = 0; i < a.n; ++i) {
+    const float* pp = a.cloud + i * a.cstride;
+    float dist = 0.f;
+    for (int d = 0; d < a.dim; ++d) {
+      const float diff = pp[d] - qp[d];
+      dist = dist + diff * diff;
+    }
+    if (dist <= r2 && dist < heap.head && (a.allow_self || dist > FLT_EPSILON))
+      heap.replace_head(int32_t(i), dist);
+  }
+  if (sort) heap.sort();
+  heap.write(row.idx, row.val);
+}
+
 }  // namespace
 
 constexpr int kBfMaxK = 64;
@@ -755,10 +792,6 @@ constexpr int kBfMaxDim = 128;
 constexpr int kLpMaxDim = 62;  // dim + 2 norm columns within 4 MFMA K-steps
 
 void bf_prepare(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s) {
-  if (ix.dim > kBfMaxDim)
-    throw Error(NABO_HIP_ERR_UNSUPPORTED, "dim = " + std::to_string(ix.dim) +
-                                              " exceeds the HIP brute-force limit of " +
-                                              std::to_string(kBfMaxDim));
   const int prec = ix.params.bf_precision;
   if (prec != NABO_HIP_BF_FP32 && prec != NABO_HIP_BF_BF16 && prec != NABO_HIP_BF_FP16)
     throw Error(NABO_HIP_ERR_ARGUMENT, "unknown brute-force precision");
@@ -804,10 +837,6 @@ void bf_prepare(Index& ix, const float* d_cloud, int64_t stride, hipStream_t s) 
 
 void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   if (a.m == 0) return;
-  if (a.k > kBfMaxK)
-    throw Error(NABO_HIP_ERR_UNSUPPORTED, "k = " + std::to_string(a.k) +
-                                              " exceeds the HIP brute-force limit of " +
-                                              std::to_string(kBfMaxK));
   BfParams p{};
   p.cloud = ix.d_cloud;
   p.cstride = ix.bf_stride;
@@ -821,6 +850,18 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   p.radii = a.radii;
   p.allow_self = (a.flags & NABO_HIP_ALLOW_SELF_MATCH) ? 1 : 0;
   if (p.cstride != p.dim) throw Error(NABO_HIP_ERR_ARGUMENT, "brute-force cloud is not packed");
+  const bool lp = ix.params.bf_precision != NABO_HIP_BF_FP32;
+  if (a.k > kBfMaxK || ix.dim > kBfMaxDim || (lp && a.k + ix.params.bf_extra > 64)) {
+    const int sort = (a.flags & NABO_HIP_SORT_RESULTS) ? 1 : 0;
+    const unsigned g = unsigned(ceil_div(a.m, kBlock));
+    KernelTimer timer(s);
+    if (ix.params.heap == NABO_HIP_HEAP_STL)
+      k_bf_rows<true><<<g, kBlock, 0, s>>>(p, sort, a.out_idx, a.out_d2);
+    else
+      k_bf_rows<false><<<g, kBlock, 0, s>>>(p, sort, a.out_idx, a.out_d2);
+    NH_LAUNCH_CHECK();
+    return;
+  }
   const int64_t qblocks = ceil_div(a.m, kBlock);
   int64_t chunks = std::max<int64_t>(1, ceil_div(2048, qblocks));
   chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, ceil_div(ix.n, 1024)));
@@ -829,8 +870,6 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
   p.chunks = int(ceil_div(ix.n, p.chunk));
   if (ix.params.bf_precision != NABO_HIP_BF_FP32) {
     const int l = a.k + ix.params.bf_extra;
-    if (l > 64)
-      throw Error(NABO_HIP_ERR_UNSUPPORTED, "k + bfExtra must not exceed 64 for the shortlist");
     const bool f16 = ix.params.bf_precision == NABO_HIP_BF_FP16;
     ShortlistParams sp{};
     sp.kp = ix.lp_kp;
@@ -876,8 +915,16 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     // full scan). Queries go in batches that keep the logs within
     // kStlLogBytes.
     const int cap = std::min(1024, std::max(64, 16 * a.k));
-    const size_t per_query =
-        size_t(p.chunks) * (size_t(cap) * sizeof(uint2) + sizeof(uint32_t)) + sizeof(int32_t);
+    // At least kBlock queries per batch within kStlLogBytes: fewer, longer
+    // chunks when the logs of kBlock queries would not fit.
+    const size_t per_chunk = size_t(cap) * sizeof(uint2) + sizeof(uint32_t);
+    const int64_t max_chunks =
+        std::max<int64_t>(1, int64_t(kStlLogBytes / (size_t(kBlock) * per_chunk)));
+    if (p.chunks > max_chunks) {
+      p.chunk = ceil_div(ceil_div(ix.n, max_chunks), kTile) * kTile;
+      p.chunks = int(ceil_div(ix.n, p.chunk));
+    }
+    const size_t per_query = size_t(p.chunks) * per_chunk + sizeof(int32_t);
     int64_t mb = std::max<int64_t>(kBlock, int64_t(kStlLogBytes / per_query) / kBlock * kBlock);
     mb = std::min(mb, a.m);
     Scratch scratch(s);

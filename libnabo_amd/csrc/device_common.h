@@ -75,6 +75,41 @@ struct RowHeapStore {  // entry j in the query's output row: idx[j], val[j]
   }
 };
 
+// IndexHeapBruteForceVector (nabo/index_heap.h:263-369) as a sorted array
+// in the query's output row, for k beyond the register-resident sizes: an
+// admission shifts the larger entries up by one and lands after the equal
+// ones (replaceHead, :322-334); the row is the output as it stands.
+struct RowSortedHeap {
+  int32_t* idx;
+  float* val;
+  int k;
+  float head;
+  __device__ void reset(RowHeapStore st, int kk) {
+    idx = st.idx;
+    val = st.val;
+    k = kk;
+    for (int j = 0; j < k; ++j) {
+      idx[j] = -1;
+      val[j] = INFINITY;
+    }
+    head = INFINITY;
+  }
+  __device__ void replace_head(int32_t index, float v) {
+    int j = k - 1;
+    for (; j > 0; --j) {
+      const float prev = val[j - 1];
+      if (!(prev > v)) break;
+      val[j] = prev;
+      idx[j] = idx[j - 1];
+    }
+    val[j] = v;
+    idx[j] = index;
+    head = val[k - 1];
+  }
+  __device__ void sort() {}
+  __device__ void write(int32_t*, float*) const {}
+};
+
 template <class Store>
 struct StlHeap {
   Store st;

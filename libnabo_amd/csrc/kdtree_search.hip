@@ -173,15 +173,16 @@ __device__ __forceinline__ void heap_insert(float (&hd)[K], int32_t (&hi)[K], fl
 
 // The k-best of a query: for K > 0, IndexHeapBruteForceVector as K sorted
 // entries in VGPRs (hd, hi); for K = -1, IndexHeapSTL as a heap in the
-// query's output row, updated at each admission (k above 96). Both admit the
-// same distances (each drops one largest entry for the new one, so their
-// multisets of values, and their heads, are always equal): the descent and
-// its pruning are the same; only the entry kept among equal distances and
-// the output order differ.
+// query's output row, updated at each admission (k above 96); for K = -2,
+// IndexHeapBruteForceVector as a sorted array in the output row (k above
+// 128). All admit the same distances (each drops one largest entry for the
+// new one, so their multisets of values, and their heads, are always
+// equal): the descent and its pruning are the same; only the entry kept
+// among equal distances and the output order differ between the two heaps.
 template <int K>
 constexpr int kArr = K > 0 ? K : 1;
 template <int K>
-using HeapOf = StlHeap<RowHeapStore>;
+using HeapOf = typename std::conditional<K == -2, RowSortedHeap, StlHeap<RowHeapStore>>::type;
 
 template <int K>
 __device__ __forceinline__ void best_insert(float (&hd)[kArr<K>], int32_t (&hi)[kArr<K>],
@@ -954,8 +955,10 @@ void dispatch_sorted(const SearchParams& p, int cus, int dev, int depth, DevBuf<
     launch_search<DIM, 96, LOG>(p, cus, dev, depth, stack, ext, L);
   else if constexpr (LOG)
     throw Error(NABO_HIP_ERR_UNSUPPORTED, "the logged IndexHeapSTL search runs up to k = 96");
-  else
+  else if (k <= kMaxK)
     launch_search<DIM, 128, LOG>(p, cus, dev, depth, stack, ext, L);
+  else
+    launch_search<DIM, -2, LOG>(p, cus, dev, depth, stack, ext, L);
 }
 
 template <int DIM>
@@ -1019,10 +1022,6 @@ void kd_knn_launch(const Index& ix, const KnnArgs& a, Launcher& L) {
                       " levels, and indices made with HBM stacks, keep their search stacks in "
                       "HBM, which a caller-owned scratch does not hold");
   }
-  if (a.k > kMaxK)
-    throw Error(NABO_HIP_ERR_UNSUPPORTED,
-                "k = " + std::to_string(a.k) + " exceeds the HIP kd-tree limit of " +
-                    std::to_string(kMaxK));
   if (ix.dim > kMaxGenericDim && ix.dim != 2 && ix.dim != 3)
     throw Error(NABO_HIP_ERR_UNSUPPORTED,
                 "dim = " + std::to_string(ix.dim) + " exceeds the HIP kd-tree limit of " +
