@@ -586,7 +586,7 @@ def main():
         lib = nabo.load_library()
         h = C.c_void_p()
         p = nabo.make_params(bucket_size=args.bucket)
-        rc = lib.nabo_hip_create_device(C.c_void_p(d_cloud.data_ptr()), m, 3, 3, 3,
+        rc = lib.nabo_hip_create_device(C.c_void_p(d_cloud.data_ptr()), len(cloud), 3, 3, 3,
                                         int(nabo.SearchType.KDTREE_HIP), 0, C.byref(p),
                                         C.c_void_p(torch.cuda.current_stream().cuda_stream),
                                         C.byref(h))

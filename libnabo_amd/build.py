@@ -18,6 +18,10 @@ LIB = os.path.join(PKG, "libnabo_hip.so")
 SOURCES = ["capi.hip", "kdtree_build.hip", "kdtree_search.hip", "brute_force.hip", "server.hip",
            "noalloc.hip", "plan.hip"]
 ARCH = os.environ.get("NABO_HIP_ARCH", "gfx950")
+# Per-source flags. brute_force.hip holds the only MFMA kernel (the bf16 /
+# fp16 shortlist), whose accumulators are tested value by value after each
+# tile: kept in VGPRs they need no accumulator-register reads.
+SOURCE_FLAGS = {"brute_force.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -42,7 +46,8 @@ def hipcc() -> str:
 
 def _compile(src: str) -> str:
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", out]
+    cmd = [hipcc(), *HIPCC_FLAGS, *SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src),
+           "-o", out]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{res.stdout}\n{res.stderr}")

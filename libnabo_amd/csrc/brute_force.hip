@@ -21,6 +21,7 @@
 // are bit-identical to the reference's, including the order and which of
 // equal distances are kept.
 
+#include <atomic>
 #include <cfloat>
 #include <type_traits>
 
@@ -416,7 +417,8 @@ __global__ void k_copy_cloud(int64_t n, int dim, const float* __restrict__ src, 
 // rows of half l/32 of each 32-point tile.
 //
 // Each lane keeps the L = k + m smallest ranking values of its half of a
-// chunk of points; a second kernel takes the L smallest of a query's lists
+// chunk of points (candidates buffered in LDS, k_bf_shortlist); a second
+// kernel takes the L smallest of a query's lists
 // (the shortlist), recomputes their squared distances exactly in fp32 with
 // the sequential sum, applies the reference's admission rule and keeps the
 // k best by (distance, index).
@@ -426,7 +428,8 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kMfmaTile = 32;  // points (rows) and queries (columns) per MFMA
-constexpr int kShortBuf = 8;   // candidates buffered per lane before insertion (long lists)
+constexpr int kShortBuf = 20;  // candidates buffered per lane (LDS) before insertion
+static_assert(kShortBuf >= 16, "a tile can bring 16 candidates to an emptied buffer");
 
 template <bool F16>
 struct Lp;
@@ -512,13 +515,6 @@ __device__ __forceinline__ void list_insert(float (&ld)[L], int32_t (&li)[L], fl
   ld[0] = fminf(ld[0], value);
 }
 
-__device__ __forceinline__ float pick16(const f32x16& v, uint32_t r) {
-  float out = v[0];
-#pragma unroll
-  for (int e = 1; e < 16; ++e) out = (r == uint32_t(e)) ? v[e] : out;
-  return out;
-}
-
 struct ShortlistParams {
   const void* points;   // n_pad x kp 16-bit
   const void* queries;  // m_pad x kp 16-bit
@@ -532,22 +528,36 @@ struct ShortlistParams {
 };
 
 // grid: (m_pad / 128, chunks); 4 waves, each 32 queries.
-template <bool F16, int LCAP>
-__global__ __launch_bounds__(256) void k_bf_shortlist(ShortlistParams a) {
+//
+// Candidate upkeep: each of a tile's 16 values per lane that is below the
+// lane's list head at the tile's start is appended to the lane's buffer in
+// LDS ([slot][thread]; the test is one compare per value and a branch that
+// skips the append for the whole wave when no lane has that value as a
+// candidate, which late in a chunk is most values). When some lane's buffer
+// could not take a whole tile more, every lane moves its buffer into its
+// sorted list (each entry still below the head is inserted), so the O(L)
+// insertions run for all lanes at once, about once per kShortBuf - 15
+// candidates of the busiest lane, instead of once per candidate of any
+// lane. Buffered values may have been passed by the head since; they are
+// tested again at the move, so the list ends as the L smallest values, as
+// with one insertion per value in the same order. The next tile's point
+// operands are loaded while a tile's values are tested.
+template <bool F16, int LCAP, int KS>
+__device__ __forceinline__ void shortlist_scan(const ShortlistParams& a, float* __restrict__ bv,
+                                               int32_t* __restrict__ bi) {
   typedef typename Lp<F16>::T T;
   typedef typename Lp<F16>::V V;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
   const int half = lane >> 5;
   const int64_t q = int64_t(blockIdx.x) * 128 + wave * 32 + (lane & 31);
   const int64_t p0 = int64_t(blockIdx.y) * a.chunk;
   const int64_t p1 = min(p0 + a.chunk, (a.n + kMfmaTile - 1) / kMfmaTile * kMfmaTile);
-  const int ks_count = a.kp / 16;
   const T* qrow = static_cast<const T*>(a.queries) + q * a.kp + 8 * half;
-  V b[4];
+  V b[KS];
 #pragma unroll
-  for (int ks = 0; ks < 4; ++ks)
-    if (ks < ks_count) b[ks] = *reinterpret_cast<const V*>(qrow + 16 * ks);
+  for (int ks = 0; ks < KS; ++ks) b[ks] = *reinterpret_cast<const V*>(qrow + 16 * ks);
 
   float ld[LCAP];
   int32_t li[LCAP];
@@ -556,70 +566,54 @@ __global__ __launch_bounds__(256) void k_bf_shortlist(ShortlistParams a) {
     ld[j] = j < LCAP - a.l ? -INFINITY : INFINITY;
     li[j] = -1;
   }
-  // Candidate buffer of the long-list path (bv, bi; nb entries).
-  float bv[kShortBuf];
-  int32_t bi[kShortBuf];
-  int nb = 0;
+  int nb = 0;  // buffered candidates of this lane
   auto flush = [&]() {
-#pragma unroll
-    for (int e = 0; e < kShortBuf; ++e)
-      if (e < nb && bv[e] < ld[LCAP - 1]) list_insert<LCAP>(ld, li, bv[e], bi[e]);
+#pragma unroll 1
+    for (int e = 0; e < kShortBuf; ++e) {
+      if (!__builtin_amdgcn_ballot_w64(e < nb)) break;
+      if (e < nb) {
+        const float v = bv[e * 256 + tid];
+        if (v < ld[LCAP - 1]) list_insert<LCAP>(ld, li, v, bi[e * 256 + tid]);
+      }
+    }
     nb = 0;
   };
   const T* pbase = static_cast<const T*>(a.points) + (lane & 31) * a.kp + 8 * half;
+  V pa[KS];
+  if (p0 < p1) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) pa[ks] = *reinterpret_cast<const V*>(pbase + p0 * a.kp + 16 * ks);
+  }
   for (int64_t t = p0; t < p1; t += kMfmaTile) {
-    const T* prow = pbase + t * a.kp;
-    f32x16 acc = {};
+    V cur[KS];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
-      if (ks < ks_count) acc = Lp<F16>::mfma(*reinterpret_cast<const V*>(prow + 16 * ks), b[ks], acc);
-    // Candidates of the tile: the values under the head at the tile's start.
-    uint32_t pending = 0;
+    for (int ks = 0; ks < KS; ++ks) cur[ks] = pa[ks];
+    // the next tile's rows (the last tile reloads its own: in bounds)
+    const int64_t tn = t + kMfmaTile < p1 ? t + kMfmaTile : t;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) pa[ks] = *reinterpret_cast<const V*>(pbase + tn * a.kp + 16 * ks);
+    f32x16 acc = Lp<F16>::mfma(cur[0], b[0], f32x16{});
+#pragma unroll
+    for (int ks = 1; ks < KS; ++ks) acc = Lp<F16>::mfma(cur[ks], b[ks], acc);
+    if (__builtin_amdgcn_ballot_w64(nb > kShortBuf - 16)) flush();
     const float head = ld[LCAP - 1];
+    const int32_t base = int32_t(t) + 4 * half;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) pending |= (acc[r] < head ? 1u : 0u) << r;
-    if constexpr (LCAP <= 16) {
-      // Short lists: each lane takes its own next candidate (a select over
-      // the 16 values); the wave loops as often as its busiest lane.
-      while (__ballot(pending != 0)) {
-        if (pending) {
-          const uint32_t r = __builtin_ctz(pending);
-          pending &= pending - 1;
-          const float v = pick16(acc, r);
-          if (v < ld[LCAP - 1]) {
-            const int32_t row = int32_t((r & 3) + 8 * (r >> 2) + 4 * half);
-            list_insert<LCAP>(ld, li, v, int32_t(t + row));
-          }
-        }
-      }
-    } else if (__any(pending != 0)) {
-      // Long lists: value r of every lane in row order, r uniform across
-      // the wave (a relative register read: the select and its
-      // temporaries would not fit beside the lists), r skipped when no lane
-      // has a candidate there. Candidates wait in a small unsorted buffer
-      // and are inserted when some lane's buffer is full, all lanes at
-      // once: an O(LCAP) insertion then serves every lane that has one
-      // instead of the few whose candidate came up in that row.
-#pragma unroll 1
-      for (int r = 0; r < 16; ++r) {
-        const bool mine = (pending >> r) & 1u;
-        if (__any(mine)) {
-          const float v = acc[r];
-          if (mine && v < ld[LCAP - 1]) {
-            const int32_t row = int32_t((r & 3) + 8 * (r >> 2) + 4 * half);
-#pragma unroll
-            for (int e = 0; e < kShortBuf; ++e) {
-              bv[e] = e == nb ? v : bv[e];
-              bi[e] = e == nb ? int32_t(t + row) : bi[e];
-            }
-            ++nb;
-          }
-          if (__any(nb == kShortBuf)) flush();
+    for (int r = 0; r < 16; ++r) {
+      const bool mine = acc[r] < head;
+      if (__builtin_amdgcn_ballot_w64(mine)) {
+        // keeps the wave-wide test a branch of its own (not folded into the
+        // lane mask of the append, which would then be issued for every value)
+        asm volatile("");
+        if (mine) {
+          bv[nb * 256 + tid] = acc[r];
+          bi[nb * 256 + tid] = base + (r & 3) + 8 * (r >> 2);
+          ++nb;
         }
       }
     }
   }
-  if constexpr (LCAP > 16) flush();
+  flush();
   float* ov = a.part_v + ((int64_t(blockIdx.y) * a.m_pad + q) * 2 + half) * a.l;
   int32_t* oi = a.part_i + ((int64_t(blockIdx.y) * a.m_pad + q) * 2 + half) * a.l;
 #pragma unroll
@@ -631,32 +625,29 @@ __global__ __launch_bounds__(256) void k_bf_shortlist(ShortlistParams a) {
   }
 }
 
-// Lexicographic (distance, index) insertion: the brute-force order.
-template <int K>
-__device__ __forceinline__ void lex_insert(float (&hd)[K], int32_t (&hi)[K], float d,
-                                           int32_t idx) {
-  bool gt[K];
-#pragma unroll
-  for (int j = 0; j < K; ++j) gt[j] = hd[j] > d || (hd[j] == d && hi[j] > idx);
-#pragma unroll
-  for (int j = K - 1; j > 0; --j) {
-    const float nd = gt[j - 1] ? hd[j - 1] : (gt[j] ? d : hd[j]);
-    const int32_t ni = gt[j - 1] ? hi[j - 1] : (gt[j] ? idx : hi[j]);
-    hd[j] = nd;
-    hi[j] = ni;
+template <bool F16, int LCAP>
+__global__ __launch_bounds__(256) void k_bf_shortlist(ShortlistParams a) {
+  __shared__ float bv[kShortBuf * 256];
+  __shared__ int32_t bi[kShortBuf * 256];
+  switch (a.kp / 16) {
+    case 1: shortlist_scan<F16, LCAP, 1>(a, bv, bi); break;
+    case 2: shortlist_scan<F16, LCAP, 2>(a, bv, bi); break;
+    case 3: shortlist_scan<F16, LCAP, 3>(a, bv, bi); break;
+    default: shortlist_scan<F16, LCAP, 4>(a, bv, bi); break;
   }
-  hd[0] = gt[0] ? d : hd[0];
-  hi[0] = gt[0] ? idx : hi[0];
 }
 
-// Shortlist (L smallest ranking values over the 2 x chunks lists of a query),
-// then exact fp32 rescoring and the k best.
-template <int LCAP, int K>
-__global__ __launch_bounds__(256) void k_bf_rescore(ShortlistParams s, BfParams a,
-                                                    int32_t* __restrict__ out_idx,
-                                                    float* __restrict__ out_d2) {
+// Shortlist of each query: the l smallest ranking values over its
+// 2 x chunks lists, taken in list order (a value equal to the list's head
+// does not enter, so among equal values the earlier list's entry is kept).
+// One thread per query; each list is read four entries at a time (lists are
+// ascending: the first entry not below the head ends the list). The
+// shortlist's indices are written over the query's first list, which only
+// this thread reads.
+template <int LCAP>
+__global__ __launch_bounds__(64) void k_bf_lp_merge(ShortlistParams s, int chunks, int64_t m) {
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q >= a.m) return;
+  if (q >= m) return;
   float ld[LCAP];
   int32_t li[LCAP];
 #pragma unroll
@@ -664,88 +655,177 @@ __global__ __launch_bounds__(256) void k_bf_rescore(ShortlistParams s, BfParams 
     ld[j] = j < LCAP - s.l ? -INFINITY : INFINITY;
     li[j] = -1;
   }
-  for (int c = 0; c < a.chunks * 2; ++c) {
+  for (int c = 0; c < chunks * 2; ++c) {
     const int64_t o = ((int64_t(c >> 1) * s.m_pad + q) * 2 + (c & 1)) * s.l;
-    for (int j = 0; j < s.l; ++j) {
-      const float v = s.part_v[o + j];
-      if (!(v < ld[LCAP - 1])) break;  // lists are ascending
-      list_insert<LCAP>(ld, li, v, s.part_i[o + j]);
-    }
-  }
-  float r2 = a.max_r2;
-  if (a.radii) {
-    const float r = a.radii[q];
-    r2 = r * r;
-  }
-  float hd[K];
-  int32_t hi[K];
+    bool done = false;
+    for (int j = 0; j < s.l && !done; j += 4) {
+      float v[4];
+      int32_t ix[4];
 #pragma unroll
-  for (int j = 0; j < K; ++j) {
-    hd[j] = j < K - a.k ? -INFINITY : INFINITY;
-    hi[j] = j < K - a.k ? INT32_MIN : -1;
-  }
-  const float* qp = a.query + q * a.qstride;
-#pragma unroll
-  for (int j = 0; j < LCAP; ++j) {
-    const int32_t idx = li[j];
-    if (j >= LCAP - s.l && idx >= 0 && idx < a.n) {
-      const float* pp = a.cloud + int64_t(idx) * a.cstride;
-      float dist = 0.f;
-      for (int d = 0; d < a.dim; ++d) {
-        const float diff = pp[d] - qp[d];
-        dist = dist + diff * diff;
+      for (int u = 0; u < 4; ++u) {
+        const bool in = j + u < s.l;
+        v[u] = in ? s.part_v[o + j + u] : INFINITY;
+        ix[u] = in ? s.part_i[o + j + u] : -1;
       }
-      const bool better = dist < hd[K - 1] || (dist == hd[K - 1] && idx < hi[K - 1]);
-      if (dist <= r2 && better && (a.allow_self || dist > FLT_EPSILON))
-        lex_insert<K>(hd, hi, dist, idx);
-    }
-  }
 #pragma unroll
-  for (int j = 0; j < K; ++j) {
-    if (j >= K - a.k) {
-      out_idx[q * a.k + j - (K - a.k)] = hd[j] == INFINITY ? -1 : hi[j];
-      out_d2[q * a.k + j - (K - a.k)] = hd[j];
+      for (int u = 0; u < 4; ++u) {
+        if (done || !(v[u] < ld[LCAP - 1])) {
+          done = true;
+        } else {
+          list_insert<LCAP>(ld, li, v[u], ix[u]);
+        }
+      }
     }
   }
+  int32_t* out = s.part_i + q * 2 * s.l;
+#pragma unroll
+  for (int j = 0; j < LCAP; ++j)
+    if (j >= LCAP - s.l) out[j - (LCAP - s.l)] = li[j];
 }
 
-template <bool F16, int LCAP, int K>
-void launch_shortlist(const ShortlistParams& s, const BfParams& p, hipStream_t st,
-                      int32_t* oi, float* od) {
-  k_bf_shortlist<F16, LCAP><<<dim3(unsigned(s.m_pad / 128), unsigned(p.chunks)), 256, 0, st>>>(s);
-  k_bf_rescore<LCAP, K><<<unsigned(ceil_div(p.m, 256)), 256, 0, st>>>(s, p, oi, od);
+// Exact rescoring of the shortlists: G lanes per query (G >= l), lane j
+// takes the shortlist's j-th index, computes its squared distance in fp32
+// with the sequential sum (the exact path's arithmetic), and applies the
+// reference's admission rule (within the radius, finite, not a self match
+// unless allowed). The k best by (distance, index) are the admitted
+// candidates of rank < k, the rank being the number of admitted candidates
+// of the query before it in that order (indices in a shortlist are
+// distinct, so ranks are too); output positions past the admitted count
+// get (-1, inf). Equal to inserting the candidates one by one into a
+// sorted k-best.
+template <int G>
+__global__ __launch_bounds__(256) void k_bf_rescore(ShortlistParams s, BfParams a,
+                                                    int32_t* __restrict__ out_idx,
+                                                    float* __restrict__ out_d2) {
+  const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t q = gid / G;
+  const int j = int(gid % G);
+  const bool live = q < a.m;
+  int32_t idx = -1;
+  if (live && j < s.l) idx = s.part_i[q * 2 * s.l + j];
+  float dist = INFINITY;
+  bool admit = false;
+  if (live && idx >= 0 && idx < a.n) {
+    const float* pp = a.cloud + int64_t(idx) * a.cstride;
+    const float* qp = a.query + q * a.qstride;
+    float sum = 0.f;
+    int d = 0;
+    for (; d + 8 <= a.dim; d += 8) {
+      float pv[8], qv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        pv[u] = pp[d + u];
+        qv[u] = qp[d + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const float diff = pv[u] - qv[u];
+        sum = sum + diff * diff;
+      }
+    }
+    for (; d < a.dim; ++d) {
+      const float diff = pp[d] - qp[d];
+      sum = sum + diff * diff;
+    }
+    float r2 = a.max_r2;
+    if (a.radii) {
+      const float r = a.radii[q];
+      r2 = r * r;
+    }
+    dist = sum;
+    admit = sum <= r2 && sum < INFINITY && (a.allow_self || sum > FLT_EPSILON);
+  }
+  const float key_d = admit ? dist : INFINITY;
+  const int32_t key_i = admit ? idx : INT32_MAX;
+  int rank = 0;
+  int admitted = admit ? 1 : 0;
+#pragma unroll 4
+  for (int o = 1; o < G; ++o) {
+    const int src = (j + o) % G;
+    const float od = __shfl(key_d, src, G);
+    const int32_t oi = __shfl(key_i, src, G);
+    const int oadmit = __shfl(admit ? 1 : 0, src, G);
+    admitted += oadmit;
+    rank += (oadmit && (od < key_d || (od == key_d && oi < key_i))) ? 1 : 0;
+  }
+  if (!live) return;
+  if (admit && rank < a.k) {
+    out_idx[q * a.k + rank] = idx;
+    out_d2[q * a.k + rank] = dist;
+  }
+  if (j >= admitted && j < a.k) {
+    out_idx[q * a.k + j] = -1;
+    out_d2[q * a.k + j] = INFINITY;
+  }
 }
 
 template <bool F16, int LCAP>
-void shortlist_k(const ShortlistParams& s, const BfParams& p, hipStream_t st, int32_t* oi,
-                 float* od) {
-  if (p.k <= 1)
-    launch_shortlist<F16, LCAP, 1>(s, p, st, oi, od);
-  else if (p.k <= 4)
-    launch_shortlist<F16, LCAP, 4>(s, p, st, oi, od);
-  else if (p.k <= 8)
-    launch_shortlist<F16, LCAP, 8>(s, p, st, oi, od);
-  else if (p.k <= 16)
-    launch_shortlist<F16, LCAP, 16>(s, p, st, oi, od);
-  else
-    launch_shortlist<F16, LCAP, 32>(s, p, st, oi, od);
+void launch_shortlist(const ShortlistParams& s, const BfParams& p, hipStream_t st,
+                      int32_t* oi, float* od) {
+  k_bf_shortlist<F16, LCAP><<<dim3(unsigned(s.m_pad / 128), unsigned(p.chunks)), 256, 0, st>>>(s);
+  k_bf_lp_merge<LCAP><<<unsigned(ceil_div(p.m, 64)), 64, 0, st>>>(s, p.chunks, p.m);
+  constexpr int G = LCAP <= 16 ? 16 : (LCAP <= 32 ? 32 : 64);
+  k_bf_rescore<G><<<unsigned(ceil_div(p.m * G, 256)), 256, 0, st>>>(s, p, oi, od);
 }
 
 template <bool F16>
 void shortlist_l(const ShortlistParams& s, const BfParams& p, hipStream_t st, int32_t* oi,
                  float* od) {
   if (s.l <= 16)
-    shortlist_k<F16, 16>(s, p, st, oi, od);
+    launch_shortlist<F16, 16>(s, p, st, oi, od);
   else if (s.l <= 24)
-    shortlist_k<F16, 24>(s, p, st, oi, od);
+    launch_shortlist<F16, 24>(s, p, st, oi, od);
   else if (s.l <= 32)
-    shortlist_k<F16, 32>(s, p, st, oi, od);
+    launch_shortlist<F16, 32>(s, p, st, oi, od);
   else if (s.l <= 40)
-    shortlist_k<F16, 40>(s, p, st, oi, od);
+    launch_shortlist<F16, 40>(s, p, st, oi, od);
   else if (s.l <= 48)
-    shortlist_k<F16, 48>(s, p, st, oi, od);
+    launch_shortlist<F16, 48>(s, p, st, oi, od);
   else
-    shortlist_k<F16, 64>(s, p, st, oi, od);
+    launch_shortlist<F16, 64>(s, p, st, oi, od);
+}
+
+// Workgroups of k_bf_shortlist one CU holds at once (registers grow with
+// the list length; the LDS buffer is fixed), cached per instantiation.
+template <bool F16, int LCAP>
+int shortlist_per_cu_of() {
+  static std::atomic<int> cache{0};
+  int v = cache.load();
+  if (v < 1) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_bf_shortlist<F16, LCAP>, 256, 0) !=
+            hipSuccess ||
+        v < 1)
+      v = 1;
+    cache.store(v);
+  }
+  return v;
+}
+
+template <bool F16>
+int shortlist_per_cu(int l) {
+  if (l <= 16) return shortlist_per_cu_of<F16, 16>();
+  if (l <= 24) return shortlist_per_cu_of<F16, 24>();
+  if (l <= 32) return shortlist_per_cu_of<F16, 32>();
+  if (l <= 40) return shortlist_per_cu_of<F16, 40>();
+  if (l <= 48) return shortlist_per_cu_of<F16, 48>();
+  return shortlist_per_cu_of<F16, 64>();
+}
+
+// Chunks of points for the shortlist: at least enough workgroups to fill
+// the device once (each chunk fills its lists from empty, about
+// l * (1 + ln(chunk / l)) candidates, so fewer, longer chunks are cheaper
+// per point), then, up to twice that, the count whose last round of
+// workgroups is fullest (a grid of 1.15 device-fulls takes as long as 2).
+int64_t shortlist_chunks(int64_t qb, int64_t cap, int64_t max_chunks) {
+  auto fill = [&](int64_t c) {
+    const int64_t blocks = qb * c;
+    return double(blocks) / double(ceil_div(blocks, cap) * cap);
+  };
+  const int64_t c0 = std::min(max_chunks, std::max<int64_t>(1, ceil_div(cap, qb)));
+  int64_t best = c0;
+  for (int64_t c = c0 + 1; c <= std::min(max_chunks, 2 * c0); ++c)
+    if (fill(c) > fill(best) + 0.05) best = c;
+  return best;
 }
 
 // Brute force beyond the tiled kernels' sizes (k above 64, dim above 128,
@@ -886,13 +966,13 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
       k_lp_queries<false><<<gq, kBlock, 0, s>>>(a.m, sp.m_pad, ix.dim, sp.kp, a.query, a.qstride,
                                                  reinterpret_cast<__bf16*>(qlp.p));
     sp.queries = qlp.p;
-    // Chunks of points: each chunk fills its own lists from empty, which
-    // costs about l * (1 + ln(chunk / l)) insertions of O(l) each, so longer
-    // lists take fewer, longer chunks (measured on 200k x 50k, dim 33:
-    // l = 16 best at ~1k workgroups, l = 40 at one chunk).
     const int64_t qb = sp.m_pad / 128;
-    int64_t chunks_lp = std::max<int64_t>(1, ceil_div(l <= 16 ? 1024 : 256, qb));
-    chunks_lp = std::min<int64_t>(chunks_lp, std::max<int64_t>(1, ix.lp_n_pad / 1024));
+    int dev = 0, cus = 256;
+    NH_CHECK(hipGetDevice(&dev));
+    NH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int64_t cap = int64_t(cus) * (f16 ? shortlist_per_cu<true>(l) : shortlist_per_cu<false>(l));
+    const int64_t chunks_lp =
+        shortlist_chunks(qb, cap, std::min<int64_t>(65535, std::max<int64_t>(1, ix.lp_n_pad / 1024)));
     sp.chunk = ceil_div(ceil_div(ix.lp_n_pad, chunks_lp), kMfmaTile) * kMfmaTile;
     p.chunks = int(ceil_div(ix.lp_n_pad, sp.chunk));
     DevBuf<float> pv(size_t(p.chunks) * sp.m_pad * 2 * l, s);

@@ -80,3 +80,44 @@ def test_bf_shortlist(nabo, oracle, prec, extra, max_miss):
     miss = np.mean(np.any(gd.view(np.uint32) != od.view(np.uint32), axis=1))
     print(f"{prec} m={extra}: shortlist miss rate {miss:.4f}")
     assert miss <= max_miss
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("k,extra,flags,radius", [(20, 24, 2, np.inf), (8, 40, 0, 0.6),
+                                                  (30, 20, 3, np.inf), (5, 11, 2, 0.4)])
+def test_bf_shortlist_all_points_exact(nabo, oracle, prec, k, extra, flags, radius):
+    """A cloud no larger than the shortlist (n <= k + bfExtra): every point is
+    a candidate, so the rescored result is the exact brute force's, ties and
+    admission rules (radius, self match) included. Covers shortlists of 16,
+    32 and 64 lanes per query and k above 16."""
+    rng = np.random.default_rng(7)
+    n = min(k + extra, 44)
+    cloud = rng.random((n, 5), dtype=np.float32)
+    query = np.concatenate([cloud[::2], rng.random((300, 5), dtype=np.float32)])
+    print(f"case bf {prec} all points k={k} m={extra} flags={flags} r={radius}", flush=True)
+    bf = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, 5,
+                                     {"bfPrecision": prec, "bfExtra": extra})
+    gi, gd = bf.knn(query, k, 0.0, flags, radius)
+    oi, od = oracle.bf_knn(cloud, query, k, flags=flags | 2, max_radius=radius)
+    same_up_to_ties(gi, gd, oi, od)
+    assert np.array_equal(gi == -1, oi == -1)
+
+
+@pytest.mark.parametrize("prec,k,extra,max_miss", [("fp16", 40, 8, 0.10), ("bf16", 24, 16, 0.10)])
+def test_bf_shortlist_large_k(nabo, oracle, prec, k, extra, max_miss):
+    """k above 16 on 33-D descriptors: exact distances, sorted rows, and a
+    miss rate against the exact fp32 k-best under the bound written here."""
+    cloud, query = CASES["dirichlet33"]()
+    print(f"case bf {prec} k={k} m={extra}", flush=True)
+    bf = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.BRUTE_FORCE_HIP, 33,
+                                     {"bfPrecision": prec, "bfExtra": extra})
+    gi, gd = bf.knn(query, k)
+    exact = oracle.dist2_f64(cloud, query, gi)
+    np.testing.assert_allclose(gd, exact, rtol=1e-5)
+    assert np.all(np.diff(gd, axis=1) >= 0)
+    assert np.all(gi >= 0)
+    assert all(len(set(r)) == k for r in gi)
+    oi, od = oracle.bf_knn(cloud, query, k, flags=2, threads=8)
+    miss = np.mean(np.any(gd.view(np.uint32) != od.view(np.uint32), axis=1))
+    print(f"{prec} k={k} m={extra}: shortlist miss rate {miss:.4f}")
+    assert miss <= max_miss
