@@ -428,8 +428,8 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kMfmaTile = 32;  // points (rows) and queries (columns) per MFMA
-constexpr int kShortBuf = 20;  // candidates buffered per lane (LDS) before insertion
-static_assert(kShortBuf >= 16, "a tile can bring 16 candidates to an emptied buffer");
+constexpr int kShortBuf = 16;  // candidates buffered per lane (LDS) before insertion
+static_assert(kShortBuf >= 8, "8 registers can bring 8 candidates to an emptied buffer");
 
 template <bool F16>
 struct Lp;
@@ -451,7 +451,19 @@ struct Lp<true> {
 };
 
 // Point rows: coordinates, then the two halves of |p|^2 (fp32 sequential
-// sum); padding rows get an infinite norm so that they rank last.
+// sum); padding rows get an infinite norm so that they rank last. Stored in
+// the MFMA's A-operand order, so that each K-step of a 32-row tile is one
+// contiguous 1 KiB load (lane l reads 16 bytes at 16 l): column c of row p
+// sits at ((p / 32) * KS + c / 16) * 512 + (p % 32 + 32 * (c % 16 / 8)) * 8
+// + c % 8, KS = kp / 16.
+__device__ __forceinline__ int64_t lp_offset(int64_t p, int c, int kp) {
+  const int64_t tile = p / kMfmaTile;
+  const int row = int(p % kMfmaTile);
+  const int ks = c / 16;
+  const int lane = row + 32 * ((c % 16) / 8);
+  return (tile * (kp / 16) + ks) * 512 + lane * 8 + c % 8;
+}
+
 template <bool F16>
 __global__ void k_lp_points(int64_t n, int64_t n_pad, int dim, int kp,
                             const float* __restrict__ cloud, int64_t stride,
@@ -459,29 +471,29 @@ __global__ void k_lp_points(int64_t n, int64_t n_pad, int dim, int kp,
   typedef typename Lp<F16>::T T;
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= n_pad) return;
-  T* row = out + p * kp;
   if (p >= n) {
-    for (int c = 0; c < kp; ++c) row[c] = T(c == dim ? INFINITY : 0.f);
+    for (int c = 0; c < kp; ++c) out[lp_offset(p, c, kp)] = T(c == dim ? INFINITY : 0.f);
     return;
   }
   const float* src = cloud + p * stride;
   float norm = 0.f;
   for (int d = 0; d < dim; ++d) {
     const float x = src[d];
-    row[d] = T(x);
+    out[lp_offset(p, d, kp)] = T(x);
     norm = norm + x * x;
   }
   const T hi = T(norm);
-  row[dim] = hi;
-  row[dim + 1] = T(norm - float(hi));
-  for (int c = dim + 2; c < kp; ++c) row[c] = T(0.f);
+  out[lp_offset(p, dim, kp)] = hi;
+  out[lp_offset(p, dim + 1, kp)] = T(norm - float(hi));
+  for (int c = dim + 2; c < kp; ++c) out[lp_offset(p, c, kp)] = T(0.f);
 }
 
-// Query columns: -2 q, then 1, 1 for the norm halves.
+// Query columns: -2 q, then 1, 1 for the norm halves. Also resets the
+// query's shared threshold (order_key of +inf).
 template <bool F16>
 __global__ void k_lp_queries(int64_t m, int64_t m_pad, int dim, int kp,
                              const float* __restrict__ query, int64_t qstride,
-                             typename Lp<F16>::T* __restrict__ out) {
+                             typename Lp<F16>::T* __restrict__ out, int32_t* __restrict__ thr) {
   typedef typename Lp<F16>::T T;
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= m_pad) return;
@@ -491,7 +503,20 @@ __global__ void k_lp_queries(int64_t m, int64_t m_pad, int dim, int kp,
     if (q < m) v = c < dim ? -2.f * query[q * qstride + c] : (c < dim + 2 ? 1.f : 0.f);
     row[c] = T(v);
   }
+  thr[q] = 0x7F800000;
 }
+
+// Signed-integer keys in float order (both zeros map to 0), for atomicMin
+// on floats; order_value inverts them. The key after k is that of the next
+// float up.
+__device__ __forceinline__ int32_t order_key(float f) {
+  const int32_t b = __float_as_int(f);
+  return f == 0.f ? 0 : (b >= 0 ? b : b ^ 0x7FFFFFFF);
+}
+__device__ __forceinline__ float order_value(int32_t k) {
+  return __int_as_float(k >= 0 ? k : k ^ 0x7FFFFFFF);
+}
+constexpr int32_t kKeyInf = 0x7F800000;
 
 // Sorted insertion of a ranking value (ties keep the earlier entry first),
 // in place from the top: entry j takes entry j-1 when that is greater than
@@ -525,23 +550,26 @@ struct ShortlistParams {
   int l;                // k + m
   float* part_v;        // chunks x m_pad x 2 x LCAP
   int32_t* part_i;
+  int32_t* thr;         // m_pad: lowest list head of each query so far (order_key)
 };
 
 // grid: (m_pad / 128, chunks); 4 waves, each 32 queries.
 //
 // Candidate upkeep: each of a tile's 16 values per lane that is below the
-// lane's list head at the tile's start is appended to the lane's buffer in
-// LDS ([slot][thread]; the test is one compare per value and a branch that
-// skips the append for the whole wave when no lane has that value as a
-// candidate, which late in a chunk is most values). When some lane's buffer
-// could not take a whole tile more, every lane moves its buffer into its
-// sorted list (each entry still below the head is inserted), so the O(L)
-// insertions run for all lanes at once, about once per kShortBuf - 15
+// query's threshold (the lower of the heads of its two lists, read at the
+// tile's start) is appended to the lane's buffer in LDS ([slot][thread];
+// the test is one compare per value and a branch that skips the append for
+// the whole wave when no lane has that value as a candidate, which late in
+// a chunk is most values). Before each half tile (8 values), if some lane's
+// buffer could not take 8 more, every lane moves its buffer into its sorted
+// list (each entry still below the head is inserted), so the O(L)
+// insertions run for all lanes at once, about once per kShortBuf - 7
 // candidates of the busiest lane, instead of once per candidate of any
 // lane. Buffered values may have been passed by the head since; they are
-// tested again at the move, so the list ends as the L smallest values, as
-// with one insertion per value in the same order. The next tile's point
-// operands are loaded while a tile's values are tested.
+// tested again at the move, so each list ends as the L smallest values of
+// its rows that were under the threshold, as with one insertion per value
+// in the same order. The next tile's point operands are loaded while a
+// tile's values are tested.
 template <bool F16, int LCAP, int KS>
 __device__ __forceinline__ void shortlist_scan(const ShortlistParams& a, float* __restrict__ bv,
                                                int32_t* __restrict__ bi) {
@@ -567,7 +595,30 @@ __device__ __forceinline__ void shortlist_scan(const ShortlistParams& a, float* 
     li[j] = -1;
   }
   int nb = 0;  // buffered candidates of this lane
+  // The candidate threshold. A value v enters this lane's buffer when
+  // v < own head (the list's own rule) and v <= h for h the head of any
+  // other list of the query: the partner lane's (lane ^ 32: the other rows
+  // of the tile) and those published by the query's other chunks (the
+  // lowest in thr). Such a list holds l values at or below h, so a value
+  // above h is not among the query's l smallest: the lists' merge is the
+  // same as without the threshold, ties included. Heads only fall, so a
+  // stale h is still valid: the threshold is refreshed at each flush only
+  // (measured: refreshing it every 64 to 1024 tiles as well costs more than
+  // it saves).
+  int32_t* gq = a.thr + q;
+  float head = INFINITY;
+  auto refresh = [&](int32_t published) {
+    const float own = ld[LCAP - 1];
+    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(own), __float_as_uint(own),
+                                                     false, false);
+    const int32_t other = min(min(published, order_key(own)),
+                              min(order_key(__uint_as_float(sw[0])),
+                                  order_key(__uint_as_float(sw[1]))));
+    const float th = other >= kKeyInf ? INFINITY : order_value(other + 1);
+    head = __builtin_amdgcn_fmed3f(own, th, -INFINITY);  // the lower (no NaN quieting)
+  };
   auto flush = [&]() {
+    const int32_t published = atomicMin(gq, order_key(ld[LCAP - 1]));
 #pragma unroll 1
     for (int e = 0; e < kShortBuf; ++e) {
       if (!__builtin_amdgcn_ballot_w64(e < nb)) break;
@@ -577,29 +628,29 @@ __device__ __forceinline__ void shortlist_scan(const ShortlistParams& a, float* 
       }
     }
     nb = 0;
+    atomicMin(gq, order_key(ld[LCAP - 1]));
+    refresh(published);
   };
-  const T* pbase = static_cast<const T*>(a.points) + (lane & 31) * a.kp + 8 * half;
-  V pa[KS];
-  if (p0 < p1) {
+  // tile t's K-step ks: 512 elements at (t / 32 * KS + ks) * 512, lane's 8 at 8 * lane
+  const T* pbase = static_cast<const T*>(a.points) + lane * 8;
+  auto load = [&](V (&dst)[KS], int64_t t) {
+    const T* tp = pbase + (t / kMfmaTile) * (KS * 512);
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) pa[ks] = *reinterpret_cast<const V*>(pbase + p0 * a.kp + 16 * ks);
-  }
-  for (int64_t t = p0; t < p1; t += kMfmaTile) {
-    V cur[KS];
+    for (int ks = 0; ks < KS; ++ks) dst[ks] = *reinterpret_cast<const V*>(tp + ks * 512);
+  };
+  // One tile: the MFMA, then the candidate test. The threshold is the lower
+  // of this lane's head and its partner's (lane ^ 32: the same query, the
+  // other rows): the partner's list already holds l values at or below its
+  // head, so a value at or above it is not among the query's l smallest.
+  auto tile = [&](const V (&op)[KS], int64_t t) {
+    f32x16 acc = Lp<F16>::mfma(op[0], b[0], f32x16{});
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) cur[ks] = pa[ks];
-    // the next tile's rows (the last tile reloads its own: in bounds)
-    const int64_t tn = t + kMfmaTile < p1 ? t + kMfmaTile : t;
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) pa[ks] = *reinterpret_cast<const V*>(pbase + tn * a.kp + 16 * ks);
-    f32x16 acc = Lp<F16>::mfma(cur[0], b[0], f32x16{});
-#pragma unroll
-    for (int ks = 1; ks < KS; ++ks) acc = Lp<F16>::mfma(cur[ks], b[ks], acc);
-    if (__builtin_amdgcn_ballot_w64(nb > kShortBuf - 16)) flush();
-    const float head = ld[LCAP - 1];
+    for (int ks = 1; ks < KS; ++ks) acc = Lp<F16>::mfma(op[ks], b[ks], acc);
     const int32_t base = int32_t(t) + 4 * half;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
+      // room for the next 8 registers' candidates in every lane's buffer
+      if (r % 8 == 0 && __builtin_amdgcn_ballot_w64(nb > kShortBuf - 8)) flush();
       const bool mine = acc[r] < head;
       if (__builtin_amdgcn_ballot_w64(mine)) {
         // keeps the wave-wide test a branch of its own (not folded into the
@@ -612,6 +663,18 @@ __device__ __forceinline__ void shortlist_scan(const ShortlistParams& a, float* 
         }
       }
     }
+  };
+  // Two operand sets in turn: one tile's rows load while the other's are
+  // tested (the last tile reloads its own rows: in bounds).
+  V pa[KS], pb[KS];
+  if (p0 < p1) load(pa, p0);
+  for (int64_t t = p0; t < p1; t += 2 * kMfmaTile) {
+    const int64_t t1 = t + kMfmaTile;
+    load(pb, t1 < p1 ? t1 : t);
+    tile(pa, t);
+    if (t1 >= p1) break;
+    load(pa, t1 + kMfmaTile < p1 ? t1 + kMfmaTile : t1);
+    tile(pb, t1);
   }
   flush();
   float* ov = a.part_v + ((int64_t(blockIdx.y) * a.m_pad + q) * 2 + half) * a.l;
@@ -958,14 +1021,16 @@ void bf_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
     sp.l = l;
     sp.points = ix.d_cloud_lp;
     DevBuf<uint16_t> qlp(size_t(sp.m_pad) * sp.kp, s);
+    DevBuf<int32_t> thr(size_t(sp.m_pad), s);
     const unsigned gq = unsigned(ceil_div(sp.m_pad, kBlock));
     if (f16)
       k_lp_queries<true><<<gq, kBlock, 0, s>>>(a.m, sp.m_pad, ix.dim, sp.kp, a.query, a.qstride,
-                                                reinterpret_cast<_Float16*>(qlp.p));
+                                                reinterpret_cast<_Float16*>(qlp.p), thr.p);
     else
       k_lp_queries<false><<<gq, kBlock, 0, s>>>(a.m, sp.m_pad, ix.dim, sp.kp, a.query, a.qstride,
-                                                 reinterpret_cast<__bf16*>(qlp.p));
+                                                 reinterpret_cast<__bf16*>(qlp.p), thr.p);
     sp.queries = qlp.p;
+    sp.thr = thr.p;
     const int64_t qb = sp.m_pad / 128;
     int dev = 0, cus = 256;
     NH_CHECK(hipGetDevice(&dev));

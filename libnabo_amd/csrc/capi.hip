@@ -102,7 +102,7 @@ void build_index(nabo_hip_index* h, const float* d_cloud, int64_t n, int32_t row
 // Serialized index: a fixed header, then the device arrays and the host
 // vectors, each at a 256-byte aligned offset of the buffer.
 constexpr char kSerialMagic[8] = {'N', 'A', 'B', 'O', 'H', 'I', 'P', '1'};
-constexpr uint32_t kSerialVersion = 1;
+constexpr uint32_t kSerialVersion = 2;  // 2: shortlist rows in MFMA operand order
 
 struct SerialHeader {
   char magic[8];
