@@ -29,9 +29,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per search launch on the default workload (lidar 1M x 1M, k=16,
 # bucket 8), from two rocprofv3 counter passes scaled by a 256 MiB copy:
-# FETCH_SIZE 55,908 KiB x 2.000 + WRITE_SIZE 130,507 KiB x 1.000
+# FETCH_SIZE 55,681 KiB x 2.000 + WRITE_SIZE 130,497 KiB x 1.000
 # (profiles/kd_search_traffic_lidar1M_k16.md).
-MEASURED_TRAFFIC_BYTES = (55_908 * 2.000 + 130_507 * 1.000) * 1024
+MEASURED_TRAFFIC_BYTES = (55_681 * 2.000 + 130_497 * 1.000) * 1024
 TRAFFIC_SOURCE = ("not measured in this run: rocprofv3 FETCH_SIZE / WRITE_SIZE passes of the "
                   "same command, profiles/kd_search_traffic_lidar1M_k16.md")
 METRIC = "exact kNN queries/s, 1M 3D pts k=16; batched+served vs plain GPU; p99 ms"
@@ -54,6 +54,9 @@ def parse():
                     help="also run a 256 MiB device-to-device copy (counter calibration)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline line (no brute-force / batched / served sections)")
+    ap.add_argument("--bf-forms", default="fp32,fp32_stl,bf16_m8,fp16_m8,bf16_m32,fp16_m32",
+                    help="brute-force forms to run (fp32 is always run: the others are "
+                         "compared with it)")
     ap.add_argument("--extras", default="brute_force,batched,served",
                     help="comma list of the sections to run after the headline")
     return ap.parse_args()
@@ -185,7 +188,7 @@ def bench_lidar_variants(nns, cloud, query, d_query, radius, bucket, use_oracle:
     return out
 
 
-def bench_brute_force(log_cpu: bool):
+def bench_brute_force(log_cpu: bool, forms=None):
     """Brute force, 200k x 33-D Dirichlet descriptors, 50k queries, k=8: exact
     fp32 path and bf16 / fp16 MFMA shortlists (m = 8, 32) with exact
     rescoring. Miss rate = share of queries whose k-best differs from the
@@ -205,6 +208,9 @@ def bench_brute_force(log_cpu: bool):
     stl = None
     for prec, extra in (("fp32", 0), ("fp32_stl", 0), ("bf16", 8), ("fp16", 8), ("bf16", 32),
                         ("fp16", 32)):
+        key = prec if prec.startswith("fp32") else f"{prec}_m{extra}"
+        if forms is not None and key != "fp32" and key not in forms:
+            continue
         # fp32_stl: BRUTE_FORCE, the reference's own IndexHeapSTL order
         nns = nabo.NearestNeighbourSearch(
             cloud, nabo.SearchType.BRUTE_FORCE if prec == "fp32_stl" else nabo.SearchType.BRUTE_FORCE_HIP,
@@ -224,7 +230,6 @@ def bench_brute_force(log_cpu: bool):
         kms, launches = nabo.kernel_timing_collect()
         nabo.kernel_timing(False)
         gi, gd = di.cpu().numpy(), dd.cpu().numpy()
-        key = prec if prec.startswith("fp32") else f"{prec}_m{extra}"
         entry = {"ms": round(ms, 3), "queries_per_s": round(m / (ms / 1e3), 1)}
         if prec == "fp32_stl":
             stl = (gi, gd)
@@ -272,11 +277,12 @@ def bench_brute_force(log_cpu: bool):
                                              "sample": f"{nq} queries, OpenMP over queries"}}
         out["fp32_parity_sample"] = bool(np.array_equal(exact[1][:nq].view(np.uint32),
                                                         od.view(np.uint32)))
-        # the STL-heap form returns the reference's own order: bit-identical rows
-        si, sd = o.bf_knn(cloud, query[:nq], k, flags=0, threads=threads)
-        out["fp32_stl_parity_sample"] = bool(
-            np.array_equal(stl[0][:nq], si) and
-            np.array_equal(stl[1][:nq].view(np.uint32), sd.view(np.uint32)))
+        if stl is not None:
+            # the STL-heap form returns the reference's own order: bit-identical rows
+            si, sd = o.bf_knn(cloud, query[:nq], k, flags=0, threads=threads)
+            out["fp32_stl_parity_sample"] = bool(
+                np.array_equal(stl[0][:nq], si) and
+                np.array_equal(stl[1][:nq].view(np.uint32), sd.view(np.uint32)))
     return out
 
 
@@ -303,17 +309,25 @@ def bench_batched():
     best, table = autotune.tune_batch(pts, off, qs, qoff, k)
     log(f"autotune best {best}")
     params = autotune.params_of(best)
+    # Host CLOCK_MONOTONIC at the start and end of each timed phase, so that
+    # a kernel trace of the run (rocprofv3 stamps dispatches on the same
+    # clock) can be split by phase.
+    marks = {}
     # batched: build + search
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    m0 = time.monotonic_ns()
     b = nabo.BatchSearch(pts, off, 3, 0, params)
     torch.cuda.synchronize()
+    marks["batched_build"] = [m0, time.monotonic_ns()]
     build_ms = (time.perf_counter() - t0) * 1e3
     bi, bd = b.knn(qs, dqoff, k)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    m0 = time.monotonic_ns()
     bi, bd = b.knn(qs, dqoff, k, indices=bi, dists2=bd)
     torch.cuda.synchronize()
+    marks["batched_knn"] = [m0, time.monotonic_ns()]
     knn_ms = (time.perf_counter() - t0) * 1e3
     # plain: one index and one knn call per cloud, same parameters
     lib = nabo.load_library()
@@ -324,6 +338,7 @@ def bench_batched():
     pd = torch.empty_like(bd)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    m0 = time.monotonic_ns()
     handles = []
     for c in range(len(clouds)):
         h = C.c_void_p()
@@ -333,8 +348,10 @@ def bench_batched():
         assert rc == 0, lib.nabo_hip_last_error()
         handles.append(h)
     torch.cuda.synchronize()
+    marks["plain_build"] = [m0, time.monotonic_ns()]
     plain_build_ms = (time.perf_counter() - t0) * 1e3
     t0 = time.perf_counter()
+    m0 = time.monotonic_ns()
     for c, h in enumerate(handles):
         q0, q1 = int(qoff[c]), int(qoff[c + 1])
         rc = lib.nabo_hip_knn_device(h, C.c_void_p(qs[q0:].data_ptr()), 3, 3, q1 - q0, k, 0.0, 0,
@@ -342,6 +359,7 @@ def bench_batched():
                                      C.c_void_p(pd[q0:].data_ptr()), None, stream)
         assert rc == 0, lib.nabo_hip_last_error()
     torch.cuda.synchronize()
+    marks["plain_knn"] = [m0, time.monotonic_ns()]
     plain_knn_ms = (time.perf_counter() - t0) * 1e3
     for h in handles:
         lib.nabo_hip_destroy(h)
@@ -358,9 +376,11 @@ def bench_batched():
     reps = 5
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    m0 = time.monotonic_ns()
     for _ in range(reps):
         plan.launch()
     torch.cuda.synchronize()
+    marks["graph_launches"] = [m0, time.monotonic_ns()]
     graph_ms = (time.perf_counter() - t0) * 1e3 / reps
     complete, _ = plan.status()
     graph_same = bool(complete and torch.equal(plan.indices, bi) and torch.equal(plan.dists2, bd))
@@ -379,7 +399,8 @@ def bench_batched():
             "speedup_total": round(plain_total / batched_total, 2),
             "speedup_knn": round(plain_knn_ms / knn_ms, 2),
             "batched_queries_per_s": round(total_q / (knn_ms / 1e3), 1),
-            "results_equal_plain": same}
+            "results_equal_plain": same,
+            "phase_marks_monotonic_ns": marks}
 
 
 def _serial_queue(arrivals, service):
@@ -718,7 +739,7 @@ def main():
         torch.cuda.empty_cache()
     if "brute_force" in extras:
         log("brute force: 200k x 33-D points, 50k queries, k=8")
-        out["brute_force"] = bench_brute_force(not args.no_cpu)
+        out["brute_force"] = bench_brute_force(not args.no_cpu, set(args.bf_forms.split(",")))
         torch.cuda.empty_cache()
     if "batched" in extras:
         log("batched: 512 uneven clouds, k=8")
