@@ -59,7 +59,8 @@ constexpr int kPopBatch = 4;
 constexpr int kMaxCachedDevices = 64;        // stack entries read per round trip
 
 // Coordinates / offsets of one query in VGPRs, indexed by a runtime cut
-// dimension through selects (DIM > 0), or a local array (DIM == 0).
+// dimension through selects (DIM > 0), a local array (DIM == 0, up to
+// kMaxGenericDim) or memory (DIM == -1, any dimension).
 template <int DIM>
 struct Vec {
   float v[DIM];
@@ -80,6 +81,7 @@ struct Vec {
       return __uint_as_float(r);
     }
   }
+  __device__ __forceinline__ float at(int d) const { return v[d]; }
   __device__ __forceinline__ void set(uint32_t i, float x) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -92,7 +94,19 @@ template <>
 struct Vec<0> {
   float v[kMaxGenericDim];
   __device__ __forceinline__ float get(uint32_t i) const { return v[i]; }
+  __device__ __forceinline__ float at(int d) const { return v[d]; }
   __device__ __forceinline__ void set(uint32_t i, float x) { v[i] = x; }
+};
+// Any dimension (above kMaxGenericDim): the values live in memory, element
+// i at p[i * stride] (the query's own row, or this lane's slots of a
+// slot-major array in HBM).
+template <>
+struct Vec<-1> {
+  float* p;
+  int64_t stride;
+  __device__ __forceinline__ float get(uint32_t i) const { return p[int64_t(i) * stride]; }
+  __device__ __forceinline__ float at(int d) const { return p[int64_t(d) * stride]; }
+  __device__ __forceinline__ void set(uint32_t i, float x) { p[int64_t(i) * stride] = x; }
 };
 
 struct SearchParams {
@@ -125,6 +139,8 @@ struct SearchParams {
   uint2* log_heap;     // LOG kernels: heaps of lanes whose log filled, [entry][grid lane]
   uint32_t log_cap;    // entries per lane
   uint32_t log_lanes;  // lanes of the grid
+  float* offs;         // DIM == -1: per-lane offsets, [dimension][grid lane]
+  int64_t offs_lanes;
   int rounds;          // sub-chunks of kWave queries per work claim
   int hbm_stack;       // host side: stacks in HBM even when LDS would hold them
   int log_entries;     // host side: admission-log entries per lane (0: by k)
@@ -284,7 +300,7 @@ __device__ __forceinline__ float point_dist(const SearchParams& a, const Vec<DIM
     const int dim = DIM > 0 ? DIM : a.dim;
     float dist = 0.f;
     for (int d = 0; d < dim; ++d) {
-      const float diff = q.v[d] - p[d];
+      const float diff = q.at(d) - p[d];
       dist = dist + diff * diff;
     }
     *index = __float_as_int(p[dim]);
@@ -430,9 +446,17 @@ __global__ __launch_bounds__(kSearchBlock, (K <= 32 ? 3 : K <= 64 ? 2 : 1)) void
     // reference only ever uses off[cd] squared, and -(a * a) is (-a) * a.
     Vec<DIM> q, off2;
     const float* qp = a.query + qi * a.qstride;
-    for (int d = 0; d < dim; ++d) {
-      q.v[d] = qp[d];
-      off2.v[d] = 0.f;
+    if constexpr (DIM == -1) {
+      q.p = const_cast<float*>(qp);
+      q.stride = 1;
+      off2.p = a.offs + lane_id;
+      off2.stride = a.offs_lanes;
+      for (int d = 0; d < dim; ++d) off2.set(uint32_t(d), 0.f);
+    } else {
+      for (int d = 0; d < dim; ++d) {
+        q.v[d] = qp[d];
+        off2.v[d] = 0.f;
+      }
     }
     float hd[kArr<K>];
     int32_t hi[kArr<K>];
@@ -894,16 +918,28 @@ void launch_search_as(SearchParams p, int cus, int dev, int depth, DevBuf<uint2>
   const unsigned grid = lds_stack ? resident_grid<DIM, K, true, EXACT, LOG>(lds, cus, dev)
                                   : resident_grid<DIM, K, false, EXACT, LOG>(lds, cus, dev);
   const int64_t lanes = int64_t(grid) * kSearchBlock;
-  Scratch scratch(L.stream());  // the LOG kernels' admission logs and spilled heaps
+  if (ext && DIM == -1)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                "searches above " + std::to_string(kMaxGenericDim) +
+                    " dimensions keep per-lane offsets in HBM, which a caller-owned scratch "
+                    "does not hold");
+  // the LOG kernels' admission logs and spilled heaps; DIM == -1: offsets
+  Scratch scratch(L.stream());
   Span<uint2> log, log_heap;
+  Span<float> offs;
   if constexpr (LOG) {
     p.log_cap = search_log_cap(p.k, p.log_entries);
     p.log_lanes = uint32_t(lanes);
     scratch.want(log, size_t(p.log_cap) * size_t(lanes));
     scratch.want(log_heap, size_t(p.k) * size_t(lanes));
+  }
+  if constexpr (DIM == -1) scratch.want(offs, size_t(p.dim) * size_t(lanes));
+  if constexpr (LOG || DIM == -1) {
     scratch.commit(dev);
     p.log = log.p;
     p.log_heap = log_heap.p;
+    p.offs = offs.p;
+    p.offs_lanes = lanes;
   }
   // measurement events bracket eager launches only
   std::optional<KernelTimer> timer;
@@ -1022,10 +1058,6 @@ void kd_knn_launch(const Index& ix, const KnnArgs& a, Launcher& L) {
                       " levels, and indices made with HBM stacks, keep their search stacks in "
                       "HBM, which a caller-owned scratch does not hold");
   }
-  if (ix.dim > kMaxGenericDim && ix.dim != 2 && ix.dim != 3)
-    throw Error(NABO_HIP_ERR_UNSUPPORTED,
-                "dim = " + std::to_string(ix.dim) + " exceeds the HIP kd-tree limit of " +
-                    std::to_string(kMaxGenericDim));
 
   SearchParams p{};
   p.nodes = ix.d_nodes;
@@ -1167,8 +1199,10 @@ void kd_knn_launch(const Index& ix, const KnnArgs& a, Launcher& L) {
     dispatch_k<3>(p, stl, cus, dev, depth, stack, a.scratch, L);
   else if (ix.dim == 2)
     dispatch_k<2>(p, stl, cus, dev, depth, stack, a.scratch, L);
-  else
+  else if (ix.dim <= kMaxGenericDim)
     dispatch_k<0>(p, stl, cus, dev, depth, stack, a.scratch, L);
+  else
+    dispatch_k<-1>(p, stl, cus, dev, depth, stack, a.scratch, L);
 }
 
 }  // namespace nabo_hip

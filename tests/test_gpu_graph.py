@@ -3,9 +3,14 @@
 Replays of captured knn calls made illegal memory accesses on the GPU,
 intermittently: once for a batched search, once for a single-tree search
 after other captured graphs had run in the same process (eager calls never
-did). The cause was not found, so every knn entry point refuses stream
-capture with NABO_HIP_ERR_UNSUPPORTED before it launches anything; the replay
-test below is kept, skipped, for when capture is supported again.
+did). The captured calls then allocated and freed their temporaries
+stream-ordered inside the capture (graph memory nodes) and, later, took
+them from an event-ordered pool; which of these faulted was not shown. The
+plain knn entry points therefore refuse stream capture with
+NABO_HIP_ERR_UNSUPPORTED before they launch anything, and the replay test
+below stays skipped. The graph form is the batch plan
+(nabo_hip_batch_plan_*, tests/test_gpu_plan.py): kernel nodes only, recorded
+explicitly, with every buffer owned by the plan.
 """
 import numpy as np
 import pytest

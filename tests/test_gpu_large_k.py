@@ -3,8 +3,9 @@
 The reference bounds k only by the cloud size (nabo/nabo.cpp:105-116) and
 recommends KDTREE_TREE_HEAP for large k (nabo/nabo.h:293-294). The kd-tree
 keeps its k-best in the query's output row above k = 128 (linear heap) and
-96 (IndexHeapSTL); the brute force scans in index order with the k-best in
-the output row above k = 64 or dim = 128. Each is checked against the oracle.
+96 (IndexHeapSTL), and its per-query offsets in HBM above 64 dimensions; the
+brute force scans in index order with the k-best in the output row above
+k = 64 or dim = 128. Each is checked against the oracle.
 """
 import numpy as np
 import pytest
@@ -75,3 +76,23 @@ def test_brute_force_high_dim(nabo, oracle):
             assert_bit_exact(gi, gd, oi, od, "dim 150 BRUTE_FORCE")
         else:
             assert_same_up_to_ties(gi, gd, oi, od, "dim 150 BRUTE_FORCE_HIP")
+
+
+@pytest.mark.parametrize("heap", ["linear", "stl"])
+def test_kdtree_high_dim(nabo, oracle, heap):
+    """kd-tree above 64 dimensions (the search's offsets then live in HBM):
+    tree and results bit-exact against the oracle."""
+    rng = np.random.default_rng(55)
+    dim = 100
+    cloud = rng.random((4_000, dim)).astype(np.float32)
+    query = np.concatenate([cloud[:50], rng.random((250, dim)).astype(np.float32)])
+    st = nabo.SearchType.KDTREE_HIP if heap == "linear" else nabo.SearchType.KDTREE_TREE_HEAP
+    nns = nabo.NearestNeighbourSearch(cloud, st, dim, {"bucketSize": 8},
+                                      creationOptionFlags=nabo.CreationOptionFlags.TOUCH_STATISTICS)
+    otree = oracle.kdtree(cloud, bucket_size=8)
+    print(f"case kd dim {dim} heap={heap}", flush=True)
+    for k, flags, radius, eps in ((5, 0, np.inf, 0.0), (12, 3, 1.4, 0.0), (5, 0, np.inf, 0.2)):
+        gi, gd = nns.knn(query, k, eps, flags, radius)
+        oi, od, touched, _ = otree.knn(query, k, eps, flags, radius, heap=heap, stats=True)
+        assert_bit_exact(gi, gd, oi, od, f"dim {dim} {heap} k={k} flags={flags} r={radius}")
+        assert nns.last_touched == int(touched.sum())
