@@ -7,11 +7,15 @@
 // code and keeps the message for nabo_hip_last_error().
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -400,6 +404,102 @@ namespace {
 // kStageKeepBytes kept); every call that uses one synchronizes its stream
 // before giving it back.
 constexpr size_t kStageChunk = size_t(8) << 20;
+
+// Host copies between the caller's pageable memory and the pinned stage run
+// on a small persistent pool of threads (at most 16, the calling thread
+// included): a copy into freshly allocated memory is bound by its
+// first-touch page faults, which the operating system serves in parallel
+// (1M queries x k = 16: 128 MB of results). Several calls may copy at once;
+// each waits for its own pieces only.
+class CopyPool {
+ public:
+  static CopyPool& get() {
+    static CopyPool* pool = new CopyPool();  // never destroyed: idle workers wait
+    return *pool;
+  }
+  void copy(void* dst, const void* src, size_t len) {
+    if (workers_ == 0 || len < kParallelMin) {
+      std::memcpy(dst, src, len);
+      return;
+    }
+    Job j;
+    j.dst = static_cast<char*>(dst);
+    j.src = static_cast<const char*>(src);
+    j.len = len;
+    j.piece = std::max(kPieceMin, (len + 2 * (workers_ + 1) - 1) / (2 * (workers_ + 1)));
+    j.count = (len + j.piece - 1) / j.piece;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      jobs_.push_back(&j);
+    }
+    cv_.notify_all();
+    while (run_piece(j)) {
+    }
+    {
+      // after this no worker can join the job; wait for those that did
+      std::lock_guard<std::mutex> lock(mu_);
+      auto it = std::find(jobs_.begin(), jobs_.end(), &j);
+      if (it != jobs_.end()) jobs_.erase(it);
+    }
+    while (j.done.load(std::memory_order_acquire) < j.count ||
+           j.users.load(std::memory_order_acquire) > 0)
+      std::this_thread::yield();
+  }
+
+ private:
+  static constexpr size_t kParallelMin = size_t(2) << 20;
+  static constexpr size_t kPieceMin = size_t(256) << 10;
+  struct Job {
+    char* dst = nullptr;
+    const char* src = nullptr;
+    size_t len = 0, piece = 0, count = 0;
+    std::atomic<size_t> next{0}, done{0};
+    std::atomic<int> users{0};
+  };
+  CopyPool() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    workers_ = hw > 1 ? std::min(hw, 16u) - 1 : 0;
+    for (unsigned i = 0; i < workers_; ++i) {
+      try {
+        std::thread(&CopyPool::work, this).detach();
+      } catch (const std::exception&) {
+        workers_ = i;  // run with the workers that started
+        break;
+      }
+    }
+  }
+  static bool run_piece(Job& j) {
+    const size_t i = j.next.fetch_add(1, std::memory_order_relaxed);
+    if (i >= j.count) return false;
+    const size_t off = i * j.piece;
+    std::memcpy(j.dst + off, j.src + off, std::min(j.piece, j.len - off));
+    j.done.fetch_add(1, std::memory_order_release);
+    return true;
+  }
+  void work() {
+    for (;;) {
+      Job* j = nullptr;
+      {
+        std::unique_lock<std::mutex> lock(mu_);
+        cv_.wait(lock, [&] {
+          while (!jobs_.empty() &&
+                 jobs_.front()->next.load(std::memory_order_relaxed) >= jobs_.front()->count)
+            jobs_.pop_front();
+          return !jobs_.empty();
+        });
+        j = jobs_.front();
+        j->users.fetch_add(1, std::memory_order_relaxed);
+      }
+      while (run_piece(*j)) {
+      }
+      j->users.fetch_sub(1, std::memory_order_release);
+    }
+  }
+  unsigned workers_ = 0;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<Job*> jobs_;
+};
 constexpr int kStageEvents = 32;  // chunks per direction and call, at most
 constexpr int kStageKeep = 2;
 constexpr size_t kStageKeepBytes = size_t(256) << 20;
@@ -473,7 +573,7 @@ class HostStage {
   void to_device(void* dst, const void* src, size_t bytes, size_t stage_off) {
     for (size_t off = 0; off < bytes; off += kStageChunk) {
       const size_t len = std::min(kStageChunk, bytes - off);
-      std::memcpy(b_.p + stage_off + off, static_cast<const char*>(src) + off, len);
+      CopyPool::get().copy(b_.p + stage_off + off, static_cast<const char*>(src) + off, len);
       NH_CHECK(hipMemcpyAsync(static_cast<char*>(dst) + off, b_.p + stage_off + off, len,
                               hipMemcpyHostToDevice, s_));
     }
@@ -512,7 +612,7 @@ class HostStage {
     }
     for (int j = 0; j < n; ++j) {
       NH_CHECK(hipEventSynchronize(b_.ev[j]));
-      std::memcpy(pieces[j].dst, b_.p + pieces[j].stage_off, pieces[j].len);
+      CopyPool::get().copy(pieces[j].dst, b_.p + pieces[j].stage_off, pieces[j].len);
     }
   }
 
