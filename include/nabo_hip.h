@@ -137,7 +137,9 @@ int nabo_hip_knn(const nabo_hip_index* index, const float* query, int32_t qrows,
  * returns once enqueued. d_touched (nullable) is a device uint64 that
  * receives the touched count. Stream-ordered scratch only: concurrent calls
  * on one index from different streams are safe (knn is const and
- * reentrant, nabo/nabo.h:179). */
+ * reentrant, nabo/nabo.h:179). A stream being captured into a HIP graph is
+ * refused (NABO_HIP_ERR_UNSUPPORTED, nothing launched): the temporaries come
+ * from a pool that reuses its blocks between calls. */
 int nabo_hip_knn_device(const nabo_hip_index* index, const float* d_query, int32_t qrows,
                         int64_t qstride, int64_t m, int32_t k, float epsilon,
                         uint32_t option_flags, float max_radius, const float* d_max_radii,
@@ -156,7 +158,11 @@ int nabo_hip_knn_device(const nabo_hip_index* index, const float* d_query, int32
  * calls whose temporaries exceed the array (about 24 bytes per query with
  * query ordering: up to about 2.6 million queries). All calls on one GPU
  * share the array: calls that overlap on a GPU race and must be serialized
- * by the caller. Stream capture is refused as for nabo_hip_knn_device. */
+ * by the caller. Unlike nabo_hip_knn_device, it may be captured in a HIP
+ * graph (stream capture): the captured work is kernel launches only, on the
+ * index's buffers, the caller's and the static array, which outlives every
+ * graph; replays share the array with every other no-allocation call on that
+ * GPU and must not overlap them. */
 int nabo_hip_knn_device_noalloc(const nabo_hip_index* index, const float* d_query,
                                 int32_t qrows, int64_t qstride, int64_t m, int32_t k,
                                 float epsilon, uint32_t option_flags, float max_radius,

@@ -176,6 +176,10 @@ __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
 // cause was not found.
 // knn calls allocate from the scratch pool and query its events, which a
 // stream capture forbids: they refuse capture before doing anything.
+// Calls that take temporaries from the per-device pool refuse stream
+// capture: a graph would keep a pool block's address after the pool hands
+// the block to the next call or frees it. The no-allocation entry point
+// (temporaries in a static device array, kernels only) may be captured.
 void refuse_capture(hipStream_t s) {
   hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
   NH_CHECK(hipStreamIsCapturing(s, &capture));
@@ -184,7 +188,7 @@ void refuse_capture(hipStream_t s) {
 }
 
 void run_knn(const Index& ix, const KnnArgs& a, hipStream_t s) {
-  refuse_capture(s);
+  if (!a.scratch) refuse_capture(s);
   if (a.touched) {
     k_set_u64<<<1, 1, 0, s>>>(a.touched, 0);
     NH_LAUNCH_CHECK();
@@ -653,6 +657,11 @@ bool kernel_timing_enabled() {
 
 KernelTimer::KernelTimer(hipStream_t stream) : s(stream) {
   if (!kernel_timing_enabled()) return;
+  // a graph being captured gets no event nodes: the pair would never be
+  // recorded outside the graph
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &capture) != hipSuccess || capture != hipStreamCaptureStatusNone)
+    return;
   NH_CHECK(hipEventCreate(&start));
   NH_CHECK(hipEventCreate(&stop));
   NH_CHECK(hipEventRecord(start, s));

@@ -5,20 +5,25 @@ intermittently: once for a batched search, once for a single-tree search
 after other captured graphs had run in the same process (eager calls never
 did). The captured calls then allocated and freed their temporaries
 stream-ordered inside the capture (graph memory nodes) and, later, took
-them from an event-ordered pool; which of these faulted was not shown. The
-plain knn entry points therefore refuse stream capture with
-NABO_HIP_ERR_UNSUPPORTED before they launch anything, and the replay test
-below stays skipped. The graph form is the batch plan
-(nabo_hip_batch_plan_*, tests/test_gpu_plan.py): kernel nodes only, recorded
-explicitly, with every buffer owned by the plan.
+them from an event-ordered pool that hands a block to the next call while a
+graph still holds its address. The pooled knn entry points therefore refuse
+stream capture with NABO_HIP_ERR_UNSUPPORTED before they launch anything,
+and their replay test below stays skipped. Capture is supported where no
+such construct exists: nabo_hip_knn_device_noalloc (kernels only, on a
+static device array that outlives every graph; replayed here, checked
+against the eager call at every replay) and the batch plan
+(nabo_hip_batch_plan_*, tests/test_gpu_plan.py: kernel nodes recorded
+explicitly, every buffer owned by the plan).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-REPLAY_SKIP = ("replays of captured knn calls faulted the GPU intermittently (cause not found); "
-               "knn refuses stream capture")
+REPLAY_SKIP = ("nabo_hip_knn_device refuses stream capture (pooled temporaries); captured "
+               "replays are tested through nabo_hip_knn_device_noalloc")
 UNSUPPORTED = 4  # NABO_HIP_ERR_UNSUPPORTED
 
 
@@ -127,3 +132,67 @@ def test_batched_capture_refused(nabo):
     torch.cuda.synchronize()
     assert bool((i[:, 0] == torch.from_numpy(np.r_[np.arange(300), np.arange(500)]).cuda()).all())
     b.close()
+
+
+def _noalloc(nabo, nns, dq, k, di, dd, radius):
+    """Enqueues nabo_hip_knn_device_noalloc on torch's current stream."""
+    import torch
+
+    lib = nabo.load_library()
+    rc = lib.nabo_hip_knn_device_noalloc(
+        nns._h, C.c_void_p(dq.data_ptr()), 3, dq.stride(0), dq.shape[0], k, 0.0, 0,
+        float(radius), None, C.c_void_p(di.data_ptr()), C.c_void_p(dd.data_ptr()), None,
+        C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    if rc != 0:
+        raise nabo.NaboError(rc, lib.nabo_hip_last_error().decode())
+
+
+def test_noalloc_knn_graph_replay(nabo, oracle):
+    """Two captured no-allocation searches (k = 16 and k = 5) on one index,
+    replayed alternately three times each with eager work between replays
+    (an eager no-allocation call on the same static array, other torch
+    kernels); every replay equals the eager call, which equals
+    nabo_hip_knn_device and, on a sample, the oracle."""
+    import torch
+
+    from libnabo_amd import datasets
+    from tests.helpers import assert_bit_exact
+
+    cloud, query = datasets.lidar_scene(60_000, seed=11)
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3, {"bucketSize": 8})
+    dq = torch.from_numpy(query).cuda()
+    m = len(query)
+    tree = oracle.kdtree(cloud, bucket_size=8)
+    graphs, outs, wants = [], [], []
+    for k in (16, 5):
+        print(f"case noalloc capture k={k} m={m}", flush=True)
+        ei = torch.empty((m, k), dtype=torch.int32, device="cuda")
+        ed = torch.empty((m, k), dtype=torch.float32, device="cuda")
+        _noalloc(nabo, nns, dq, k, ei, ed, 1.0)
+        pi, pd = torch.empty_like(ei), torch.empty_like(ed)
+        nns.knn_device(dq, k, pi, pd, 0.0, 0, 1.0)
+        torch.cuda.synchronize()
+        assert torch.equal(ei, pi) and torch.equal(ed, pd), "noalloc differs from knn_device"
+        oi, od = tree.knn(query[:3000], k, max_radius=1.0)
+        assert_bit_exact(ei[:3000].cpu().numpy(), ed[:3000].cpu().numpy(), oi, od,
+                         f"noalloc k={k} vs oracle")
+        gi, gd = torch.empty_like(ei), torch.empty_like(ed)
+        graphs.append(_capture(lambda k=k, gi=gi, gd=gd: _noalloc(nabo, nns, dq, k, gi, gd, 1.0)))
+        outs.append([gi, gd])
+        wants.append([ei, ed])
+    xi = torch.empty((m, 8), dtype=torch.int32, device="cuda")
+    xd = torch.empty((m, 8), dtype=torch.float32, device="cuda")
+    for r in range(3):
+        for g, o, w in zip(graphs, outs, wants):
+            print(f"replay {r} k={w[0].shape[1]}", flush=True)
+            for t in o:
+                t.fill_(-7)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            for t, e in zip(o, w):
+                assert torch.equal(t, e), f"replay {r} differs from the eager call"
+            _noalloc(nabo, nns, dq, 8, xi, xd, np.inf)  # eager, same static array
+            torch.randn(4096, device="cuda").sum().item()
+    torch.cuda.synchronize()
+    nns.close()
