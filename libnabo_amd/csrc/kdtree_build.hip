@@ -31,6 +31,13 @@
 // records (the finalize kernels, and the search that follows in a graph,
 // return at once).
 //
+// Segments of at most a few thousand points leave the level loop: one
+// workgroup finishes such a segment's whole subtree in LDS (k_local_finish),
+// with the same steps in the same arithmetic (min/max, cut clamp, counts,
+// both Hoare passes by ranks from a workgroup scan, leftCount rule, clipped
+// cells), so the tree is the same; its nodes take ids from a range of their
+// own (2n + 2 * the segment's first point onwards), after the level loop's.
+//
 // Node numbering: the reference emits nodes in pre-order with the left
 // child right after its parent (:215-223). With a full binary tree, the
 // nodes before node v in pre-order are its level(v) ancestors and the
@@ -91,6 +98,33 @@ struct Bfs {  // all nodes in BFS order
   int32_t* rc;      // right-child edges on the path
   uint32_t* pre;    // pre-order id
 };
+
+// A segment finished in LDS (k_local_finish): its points, level, right-child
+// edges, node id and cell.
+struct LocalSeg {
+  int32_t start, count, level, rc, node;
+  float lo[3], hi[3];
+};
+// Build-wide counters in device memory, zeroed at the build's start.
+struct BuildCounters {
+  uint32_t local_count;  // LocalSeg entries
+  uint32_t claim;        // entries taken by k_local_finish workgroups
+  uint32_t local_nodes;  // nodes made by k_local_finish (the segments' own nodes excluded)
+  int32_t max_level;     // deepest level holding a node
+};
+constexpr int kLocalT = 2048;     // points of a segment finished in LDS, at most
+constexpr int kLocalSegs = 228;   // splitting segments of one LDS level, at most
+constexpr int kLocalBlock = 256;
+constexpr int kLocalPerCu = 2;    // k_local_finish workgroups resident per CU (LDS)
+constexpr int kLocalItems = kLocalT / kLocalBlock;  // consecutive points per thread
+
+// Largest segment finished in LDS: 2-D and 3-D clouds (16-byte records);
+// a level of a segment of T points has at most T / (bucket + 1) splitting
+// segments, which must fit the LDS tables.
+inline int local_points(int dim, uint32_t bucket) {
+  if (dim < 1 || dim > 3) return 0;
+  return int(std::min<int64_t>(kLocalT, int64_t(kLocalSegs) * (int64_t(bucket) + 1)));
+}
 
 constexpr int kTile = 2048;              // points per workgroup in the tiled passes
 constexpr int kItems = kTile / kBlock;   // consecutive points per thread
@@ -331,9 +365,11 @@ __global__ void k_init_points(int64_t n, int32_t clouds, const int64_t* __restri
 }
 
 __global__ void k_seg_classify(const BuildLevel* __restrict__ cur, int32_t level, Seg seg,
-                               SegStat st, Bfs bfs, uint32_t bucket, int dim) {
+                               SegStat st, Bfs bfs, uint32_t bucket, int dim, int32_t local_t,
+                               LocalSeg* __restrict__ local, BuildCounters* __restrict__ ctr) {
   const int32_t S = cur->S;
   const int64_t base = cur->base;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && S > 0) atomicMax(&ctr->max_level, level);
   for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
     const int32_t cnt = seg.count[s];
     const int64_t node = base + s;
@@ -345,6 +381,21 @@ __global__ void k_seg_classify(const BuildLevel* __restrict__ cur, int32_t level
       bfs.cd[node] = uint32_t(dim);
       bfs.aux[node] = uint32_t(seg.start[s]);
       bfs.count[node] = uint32_t(cnt);
+      continue;
+    }
+    if (cnt <= local_t) {
+      // finished in LDS after the level loop (its node words are written there)
+      st.split[s] = 0;
+      LocalSeg& e = local[atomicAdd(&ctr->local_count, 1u)];
+      e.start = seg.start[s];
+      e.count = cnt;
+      e.level = level;
+      e.rc = seg.rc[s];
+      e.node = int32_t(node);
+      for (int d = 0; d < dim; ++d) {
+        e.lo[d] = seg.lo[int64_t(s) * dim + d];
+        e.hi[d] = seg.hi[int64_t(s) * dim + d];
+      }
       continue;
     }
     st.split[s] = 1;
@@ -372,6 +423,7 @@ __global__ void k_seg_classify(const BuildLevel* __restrict__ cur, int32_t level
 // count, a system-scope fence, then the sequence number the host waits for.
 struct Mailbox {
   int32_t total;
+  int32_t extra;
   uint32_t seq;
 };
 
@@ -643,6 +695,338 @@ __global__ void k_pt_reseg(int64_t n, const int32_t* __restrict__ gate,
   pseg[p] = int32_t(2 * st.childbase[s]) + (pos < st.left[s] ? 0 : 1);
 }
 
+// Exclusive prefix sum over the workgroup's kLocalBlock threads (wave
+// shuffles, then the waves' totals through LDS).
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wave_tot) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  uint32_t incl = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t up = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += up;
+  }
+  if (lane == kWave - 1) wave_tot[w] = incl;
+  __syncthreads();
+  uint32_t add = 0;
+  for (int i = 0; i < w; ++i) add += wave_tot[i];
+  __syncthreads();
+  return add + incl - v;
+}
+
+__device__ __forceinline__ float rec_coord(const float4& r, uint32_t cd) {
+  const float lo = cd == 1u ? r.y : r.x;
+  return cd == 2u ? r.z : lo;
+}
+
+// Finishes segments of at most kLocalT points in LDS, one workgroup per
+// segment (taken from the LocalSeg list through a counter): level by level,
+// every splitting segment of the subtree at once, the steps of the level
+// loop above on LDS copies of the records (thread t owns points
+// [kLocalItems t, kLocalItems (t + 1)); per-segment min/max and counts by
+// runs of equal segments, one LDS atomic per run; each Hoare pass's ranks
+// from a workgroup scan of the packed out-of-place flags). The subtree's
+// nodes take ids r0 + 2 * start + j (j < 2 * count - 2, in creation order);
+// the segment's own node keeps its level-loop id.
+template <int DIM>
+__global__ __launch_bounds__(kLocalBlock) void k_local_finish(
+    const LocalSeg* __restrict__ list, BuildCounters* __restrict__ ctr, float4* __restrict__ recs,
+    Bfs bfs, int64_t r0, uint32_t bucket) {
+  __shared__ float4 rec[kLocalT];
+  __shared__ int16_t pseg[kLocalT];   // segment of the point in the level's table, -1 done
+  __shared__ int16_t slot_r[kLocalT];
+  __shared__ uint32_t excl[kLocalT + 1];
+  // the level's splitting segments
+  __shared__ int32_t c_start[kLocalSegs], c_count[kLocalSegs], c_rc[kLocalSegs], c_node[kLocalSegs];
+  __shared__ float c_lo[kLocalSegs * DIM], c_hi[kLocalSegs * DIM];
+  __shared__ uint32_t c_cd[kLocalSegs], c_min[kLocalSegs], c_max[kLocalSegs], c_lt[kLocalSegs],
+      c_eq[kLocalSegs];
+  __shared__ float c_ideal[kLocalSegs], c_cut[kLocalSegs];
+  __shared__ int32_t c_left[kLocalSegs];
+  __shared__ int16_t c_child[2 * kLocalSegs];
+  // the next level's
+  __shared__ int32_t n_start[kLocalSegs], n_count[kLocalSegs], n_rc[kLocalSegs], n_node[kLocalSegs];
+  __shared__ float n_lo[kLocalSegs * DIM], n_hi[kLocalSegs * DIM];
+  __shared__ uint32_t wave_tot[kLocalBlock / kWave];
+  __shared__ int32_t s_entry, s_next, s_nodes, s_lmax;
+  const int t = threadIdx.x;
+  for (;;) {
+    if (t == 0) s_entry = int32_t(atomicAdd(&ctr->claim, 1u));
+    __syncthreads();
+    const int32_t e = s_entry;
+    if (e >= int32_t(__hip_atomic_load(&ctr->local_count, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT)))
+      return;
+    const int32_t C = list[e].count;
+    const int64_t S0 = list[e].start;
+    int32_t level = list[e].level;
+    for (int p = t; p < kLocalT; p += kLocalBlock) {
+      if (p < C) rec[p] = recs[S0 + p];
+      pseg[p] = int16_t(p < C ? 0 : -1);
+    }
+    if (t == 0) {
+      c_start[0] = 0;
+      c_count[0] = C;
+      c_rc[0] = list[e].rc;
+      c_node[0] = list[e].node;
+      for (int d = 0; d < DIM; ++d) {
+        c_lo[d] = list[e].lo[d];
+        c_hi[d] = list[e].hi[d];
+      }
+      s_nodes = 0;
+      s_lmax = level;
+    }
+    __syncthreads();
+    int ns = 1;
+    while (ns > 0) {
+      // cut dimension: the first widest extent; ideal cut: the midpoint
+      for (int g = t; g < ns; g += kLocalBlock) {
+        float best = 0.f;
+        uint32_t axis = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const float extent = c_hi[g * DIM + d] - c_lo[g * DIM + d];
+          if (extent > best) {
+            best = extent;
+            axis = uint32_t(d);
+          }
+        }
+        c_cd[g] = axis;
+        c_ideal[g] = (c_hi[g * DIM + axis] + c_lo[g * DIM + axis]) * 0.5f;
+        c_min[g] = 0xffffffffu;
+        c_max[g] = 0u;
+        c_lt[g] = 0u;
+        c_eq[g] = 0u;
+      }
+      if (t == 0) s_next = 0;
+      __syncthreads();
+      // min / max of the cut coordinate
+      {
+        int run = -1;
+        uint32_t mn = 0xffffffffu, mx = 0u;
+        for (int j = 0; j < kLocalItems; ++j) {
+          const int p = t * kLocalItems + j;
+          const int g = pseg[p];
+          if (g != run) {
+            if (run >= 0) {
+              atomicMin(&c_min[run], mn);
+              atomicMax(&c_max[run], mx);
+            }
+            run = g;
+            mn = 0xffffffffu;
+            mx = 0u;
+          }
+          if (g >= 0) {
+            const uint32_t key = float_key(rec_coord(rec[p], c_cd[g]));
+            mn = min(mn, key);
+            mx = max(mx, key);
+          }
+        }
+        if (run >= 0) {
+          atomicMin(&c_min[run], mn);
+          atomicMax(&c_max[run], mx);
+        }
+      }
+      __syncthreads();
+      for (int g = t; g < ns; g += kLocalBlock) {
+        const float pmin = key_float(c_min[g]), pmax = key_float(c_max[g]);
+        const float ideal = c_ideal[g];
+        c_cut[g] = ideal < pmin ? pmin : (ideal > pmax ? pmax : ideal);
+      }
+      __syncthreads();
+      // points below / equal to the cut
+      {
+        int run = -1;
+        uint32_t lt = 0u, eq = 0u;
+        for (int j = 0; j < kLocalItems; ++j) {
+          const int p = t * kLocalItems + j;
+          const int g = pseg[p];
+          if (g != run) {
+            if (run >= 0) {
+              atomicAdd(&c_lt[run], lt);
+              atomicAdd(&c_eq[run], eq);
+            }
+            run = g;
+            lt = 0u;
+            eq = 0u;
+          }
+          if (g >= 0) {
+            const float x = rec_coord(rec[p], c_cd[g]);
+            lt += x < c_cut[g] ? 1u : 0u;
+            eq += x == c_cut[g] ? 1u : 0u;
+          }
+        }
+        if (run >= 0) {
+          atomicAdd(&c_lt[run], lt);
+          atomicAdd(&c_eq[run], eq);
+        }
+      }
+      __syncthreads();
+      // the two Hoare passes: flags (low half: belongs right but sits left,
+      // high half: belongs left but sits right), ranks, pairs, swaps
+      for (int pass = 1; pass <= 2; ++pass) {
+        uint32_t f[kLocalItems];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < kLocalItems; ++j) {
+          const int p = t * kLocalItems + j;
+          const int g = pseg[p];
+          f[j] = 0u;
+          if (g >= 0) {
+            const int32_t pos = p - c_start[g];
+            const float x = rec_coord(rec[p], c_cd[g]);
+            const float cut = c_cut[g];
+            const int32_t br1 = int32_t(c_lt[g]);
+            bool fl, fr;
+            if (pass == 1) {
+              fl = pos < br1 && !(x < cut);
+              fr = pos >= br1 && x < cut;
+            } else {
+              const int32_t br2 = br1 + int32_t(c_eq[g]);
+              fl = pos >= br1 && pos < br2 && !(x == cut);
+              fr = pos >= br2 && x == cut;
+            }
+            f[j] = uint32_t(fl) | (uint32_t(fr) << 16);
+          }
+          sum += f[j];
+        }
+        uint32_t run = block_excl_sum(sum, wave_tot);
+#pragma unroll
+        for (int j = 0; j < kLocalItems; ++j) {
+          excl[t * kLocalItems + j] = run;
+          run += f[j];
+        }
+        if (t == kLocalBlock - 1) excl[kLocalT] = run;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kLocalItems; ++j) {
+          if ((f[j] >> 16) == 0u) continue;
+          const int p = t * kLocalItems + j;
+          const int g = pseg[p];
+          const int32_t st = c_start[g];
+          const uint32_t base = excl[st] >> 16;
+          const uint32_t total_r = (excl[st + c_count[g]] >> 16) - base;
+          const uint32_t from_left = (excl[p] >> 16) - base;
+          slot_r[st + int32_t(total_r - 1u - from_left)] = int16_t(p);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kLocalItems; ++j) {
+          if ((f[j] & 0xffffu) == 0u) continue;
+          const int p = t * kLocalItems + j;
+          const int32_t st = c_start[pseg[p]];
+          const int q = slot_r[st + int32_t((excl[p] & 0xffffu) - (excl[st] & 0xffffu))];
+          const float4 a = rec[p];
+          rec[p] = rec[q];
+          rec[q] = a;
+        }
+        __syncthreads();
+      }
+      // leftCount rule, node words, children
+      for (int g = t; g < ns; g += kLocalBlock) {
+        const int32_t cnt = c_count[g];
+        const float ideal = c_ideal[g];
+        const float pmin = key_float(c_min[g]), pmax = key_float(c_max[g]);
+        const int32_t br1 = int32_t(c_lt[g]);
+        const int32_t br2 = br1 + int32_t(c_eq[g]);
+        int32_t left;
+        if (ideal < pmin)
+          left = 1;
+        else if (ideal > pmax)
+          left = cnt - 1;
+        else if (br1 > cnt / 2)
+          left = br1;
+        else if (br2 < cnt / 2)
+          left = br2;
+        else
+          left = cnt / 2;
+        left = min(max(left, 1), cnt - 1);
+        c_left[g] = left;
+        const uint32_t cd = c_cd[g];
+        const float cut = c_cut[g];
+        const int64_t id0 = r0 + 2 * S0 + atomicAdd(&s_nodes, 2);
+        const int32_t node = c_node[g];
+        bfs.cd[node] = cd;
+        bfs.aux[node] = __float_as_uint(cut);
+        bfs.left[node] = int32_t(id0);
+        bfs.right[node] = int32_t(id0 + 1);
+        for (int side = 0; side < 2; ++side) {
+          const int32_t cs = c_start[g] + (side ? left : 0);
+          const int32_t cc = side ? cnt - left : left;
+          const int32_t rc = c_rc[g] + side;
+          const int64_t id = id0 + side;
+          bfs.start[id] = int32_t(S0 + cs);
+          bfs.level[id] = level + 1;
+          bfs.rc[id] = rc;
+          if (cc <= int32_t(bucket)) {
+            bfs.cd[id] = uint32_t(DIM);
+            bfs.aux[id] = uint32_t(S0 + cs);
+            bfs.count[id] = uint32_t(cc);
+            c_child[2 * g + side] = -1;
+            atomicMax(&s_lmax, level + 1);
+          } else {
+            const int k = atomicAdd(&s_next, 1);
+            n_start[k] = cs;
+            n_count[k] = cc;
+            n_rc[k] = rc;
+            n_node[k] = int32_t(id);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+              n_lo[k * DIM + d] = (side && uint32_t(d) == cd) ? cut : c_lo[g * DIM + d];
+              n_hi[k * DIM + d] = (!side && uint32_t(d) == cd) ? cut : c_hi[g * DIM + d];
+            }
+            c_child[2 * g + side] = int16_t(k);
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kLocalItems; ++j) {
+        const int p = t * kLocalItems + j;
+        const int g = pseg[p];
+        if (g >= 0) pseg[p] = c_child[2 * g + (p - c_start[g] < c_left[g] ? 0 : 1)];
+      }
+      ns = s_next;
+      __syncthreads();
+      for (int g = t; g < ns; g += kLocalBlock) {
+        c_start[g] = n_start[g];
+        c_count[g] = n_count[g];
+        c_rc[g] = n_rc[g];
+        c_node[g] = n_node[g];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          c_lo[g * DIM + d] = n_lo[g * DIM + d];
+          c_hi[g * DIM + d] = n_hi[g * DIM + d];
+        }
+      }
+      ++level;
+      __syncthreads();
+    }
+    for (int p = t; p < C; p += kLocalBlock) recs[S0 + p] = rec[p];
+    if (t == 0) {
+      atomicAdd(&ctr->local_nodes, uint32_t(s_nodes));
+      atomicMax(&ctr->max_level, s_lmax);
+    }
+    __syncthreads();
+  }
+}
+
+// Node count and depth after the LDS subtrees: tot->base = all nodes,
+// tot->S = the deepest level; posted to the host for eager builds.
+__global__ void k_build_totals(const BuildLevel* __restrict__ fin,
+                               const BuildCounters* __restrict__ ctr, BuildLevel* __restrict__ tot,
+                               Mailbox* box, uint32_t seq) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int64_t total = fin->base + int64_t(ctr->local_nodes);
+  tot->base = total;
+  tot->S = ctr->max_level;
+  if (box) {
+    __hip_atomic_store(&box->total, int32_t(total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&box->extra, ctr->max_level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __threadfence_system();
+    __hip_atomic_store(&box->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 // The finalize kernels below return at once when *ok is 0 (a recorded
 // build whose levels did not finish the tree: its records are partial);
 // eager builds pass null.
@@ -654,35 +1038,54 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t count, uint32_t v,
     p[i] = v;
 }
 
-// Nodes: fin->base (the BFS id after the last level) in all.
-__global__ void k_mark_leaves(const BuildLevel* __restrict__ fin, Bfs bfs, int dim,
+// Node ids: the level loop's [0, fin->base), then the LDS subtrees' range
+// [r0, r0 + local) whose unused ids have level -1. node_id maps a position
+// in that sequence to an id, or -1.
+struct NodeIds {
+  int64_t r0, local;
+  __device__ __forceinline__ int64_t count(const BuildLevel* fin) const { return fin->base + local; }
+  __device__ __forceinline__ int64_t id(const BuildLevel* fin, const Bfs& bfs, int64_t v) const {
+    if (v < fin->base) return v;
+    const int64_t u = r0 + (v - fin->base);
+    return bfs.level[u] >= 0 ? u : -1;
+  }
+};
+
+__global__ void k_mark_leaves(const BuildLevel* __restrict__ fin, Bfs bfs, int dim, NodeIds ids,
                               uint32_t* __restrict__ leaf_start, const int32_t* __restrict__ ok) {
   if (ok && *ok == 0) return;
-  const int64_t count = fin->base;
-  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < count;
-       v += int64_t(gridDim.x) * blockDim.x)
-    if (bfs.cd[v] == uint32_t(dim)) leaf_start[bfs.start[v]] = 1u;
+  const int64_t count = ids.count(fin);
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t v = ids.id(fin, bfs, i);
+    if (v >= 0 && bfs.cd[v] == uint32_t(dim)) leaf_start[bfs.start[v]] = 1u;
+  }
 }
 
-__global__ void k_preorder(const BuildLevel* __restrict__ fin, Bfs bfs,
+__global__ void k_preorder(const BuildLevel* __restrict__ fin, Bfs bfs, NodeIds ids,
                            const uint32_t* __restrict__ leaves_before,
                            const int32_t* __restrict__ cid, const int32_t* __restrict__ ok) {
   if (ok && *ok == 0) return;
-  const int64_t count = fin->base;
-  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < count;
-       v += int64_t(gridDim.x) * blockDim.x) {
+  const int64_t count = ids.count(fin);
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t v = ids.id(fin, bfs, i);
+    if (v < 0) continue;
     const int32_t s = bfs.start[v];
     bfs.pre[v] = 2u * leaves_before[s] + uint32_t(bfs.level[v]) - uint32_t(bfs.rc[v]) -
                  uint32_t(cid[s]);
   }
 }
 
-__global__ void k_encode(const BuildLevel* __restrict__ fin, Bfs bfs, uint32_t dim_bits, int dim,
-                         uint2* __restrict__ nodes, const int32_t* __restrict__ ok) {
+__global__ void k_encode(const BuildLevel* __restrict__ fin, Bfs bfs, NodeIds ids,
+                         uint32_t dim_bits, int dim, uint2* __restrict__ nodes,
+                         const int32_t* __restrict__ ok) {
   if (ok && *ok == 0) return;
-  const int64_t count = fin->base;
-  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < count;
-       v += int64_t(gridDim.x) * blockDim.x) {
+  const int64_t count = ids.count(fin);
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t v = ids.id(fin, bfs, i);
+    if (v < 0) continue;
     const uint32_t cd = bfs.cd[v];
     uint2 out;
     if (cd == uint32_t(dim)) {
@@ -701,12 +1104,13 @@ __global__ void k_encode(const BuildLevel* __restrict__ fin, Bfs bfs, uint32_t d
 // node and both its children; links[c].zw is {cut dim, cut bits} of the
 // split of c's parent, so a far child is entered without reading its parent
 // again. nodes[count] is a padding leaf, so nodes[p + 1] exists for every p.
-__global__ void k_links_self(const BuildLevel* __restrict__ fin, const uint2* __restrict__ nodes,
+// tot->base: the node count.
+__global__ void k_links_self(const BuildLevel* __restrict__ tot, const uint2* __restrict__ nodes,
                              uint32_t dim_bits, uint32_t dim_mask, int dim,
                              uint4* __restrict__ links, uint2* __restrict__ all_nodes,
                              const int32_t* __restrict__ ok) {
   if (ok && *ok == 0) return;
-  const int64_t count = fin->base;
+  const int64_t count = tot->base;
   for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < count;
        p += int64_t(gridDim.x) * blockDim.x) {
     const uint2 nd = nodes[p];
@@ -721,12 +1125,12 @@ __global__ void k_links_self(const BuildLevel* __restrict__ fin, const uint2* __
   }
 }
 
-__global__ void k_links_parent(const BuildLevel* __restrict__ fin,
+__global__ void k_links_parent(const BuildLevel* __restrict__ tot,
                                const uint2* __restrict__ nodes, uint32_t dim_bits,
                                uint32_t dim_mask, int dim, uint4* __restrict__ links,
                                const int32_t* __restrict__ ok) {
   if (ok && *ok == 0) return;
-  const int64_t count = fin->base;
+  const int64_t count = tot->base;
   for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < count;
        p += int64_t(gridDim.x) * blockDim.x) {
     const uint2 nd = nodes[p];
@@ -778,9 +1182,11 @@ __global__ void k_boxes(int32_t clouds, int dim, const int64_t* __restrict__ off
 }
 
 // 1 when the recorded levels finished the tree (the level after the last
-// one has no segments), else 0.
-__global__ void k_build_ok(const BuildLevel* __restrict__ fin, int32_t* __restrict__ ok) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *ok = fin->S == 0 ? 1 : 0;
+// one has no segments) and its deepest level is at most max_level (the
+// depth the search's stacks are sized for), else 0.
+__global__ void k_build_ok(const BuildLevel* __restrict__ fin, const BuildLevel* __restrict__ tot,
+                           int32_t max_level, int32_t* __restrict__ ok) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *ok = (fin->S == 0 && tot->S <= max_level) ? 1 : 0;
 }
 
 inline unsigned grid_for(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, kBlock))); }
@@ -812,6 +1218,7 @@ class LevelMailbox {
       NH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_.box), sizeof(Mailbox),
                              hipHostMallocCoherent));
       m_.box->total = 0;
+      m_.box->extra = 0;
       m_.box->seq = 0;
       m_.seq = 0;
     }
@@ -825,6 +1232,8 @@ class LevelMailbox {
 
   Mailbox* box() const { return m_.box; }
   uint32_t next() { return ++m_.seq; }
+  // The second value posted with the last `seq` waited for.
+  int32_t extra() const { return __atomic_load_n(&m_.box->extra, __ATOMIC_RELAXED); }
   // Waits until the kernel has posted `seq`; after ~20 ms of polling it
   // falls back to synchronizing the stream, which reports a failed launch.
   int32_t wait(uint32_t seq, hipStream_t s) {
@@ -863,11 +1272,16 @@ struct BuildScratch {
   Span<int64_t> d_offsets;
   Span<int32_t> part_seg;
   Span<uint32_t> part_a, part_b;
-  Span<BuildLevel> levels;  // two alternating level states
+  Span<BuildLevel> levels;  // two alternating level states, then the totals
+  Span<LocalSeg> local;     // segments finished in LDS
+  Span<BuildCounters> counters;
+  int32_t local_t = 0;      // largest segment finished in LDS (0: none)
 
-  void want(Scratch& sc, int64_t n, int dim, int ps, int32_t clouds) {
+  void want(Scratch& sc, int64_t n, int dim, int ps, int32_t clouds, uint32_t bucket) {
     const int64_t seg_cap = std::max<int64_t>(n, clouds);
-    const int64_t max_nodes = 2 * n;
+    local_t = local_points(dim, bucket);
+    // node ids: the level loop's below 2n, the LDS subtrees' in [2n, 4n)
+    const int64_t max_nodes = local_t > 0 ? 4 * n : 2 * n;
     for (int b = 0; b < 2; ++b) {
       sc.want(seg_start[b], size_t(seg_cap));
       sc.want(seg_count[b], size_t(seg_cap));
@@ -900,7 +1314,9 @@ struct BuildScratch {
     sc.want(part_seg, size_t(nparts) * dim);
     sc.want(part_a, size_t(nparts) * dim);
     sc.want(part_b, size_t(nparts) * dim);
-    sc.want(levels, 2);
+    sc.want(levels, 3);
+    sc.want(local, local_t > 0 ? size_t(n / (int64_t(bucket) + 1) + 1) : 1);
+    sc.want(counters, 1);
   }
 };
 
@@ -933,6 +1349,16 @@ void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t str
   const unsigned gpart = unsigned(ceil_div(nparts, kTile));
   float* recs = out.pts;
   BuildLevel* lvl = sc.levels.p;
+  BuildLevel* tot = lvl + 2;
+  const int64_t r0 = 2 * n;  // first id of the LDS subtrees' nodes
+  const NodeIds ids{r0, sc.local_t > 0 ? 2 * n : 0};
+  BuildCounters* ctr = sc.counters.p;
+  L(k_fill_u32, dim3(1), dim3(64), 0, reinterpret_cast<uint32_t*>(ctr),
+    int64_t(sizeof(BuildCounters) / sizeof(uint32_t)), 0u, static_cast<const int32_t*>(nullptr));
+  if (sc.local_t > 0)
+    L(k_fill_u32, dim3(unsigned(std::min<int64_t>(kGridStride, grid_for(2 * n)))), dim3(kBlock), 0,
+      reinterpret_cast<uint32_t*>(bfs.level + r0), 2 * n, 0xffffffffu,
+      static_cast<const int32_t*>(nullptr));
 
   // Bounds pass (kdtree_cpu.cpp:256-267) and the root segments.
   const int64_t cd_elems = int64_t(clouds) * dim;
@@ -961,7 +1387,7 @@ void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t str
     if (eager && node_base + S > 2 * n)
       throw Error(NABO_HIP_ERR_HIP, "kd-tree build overflow");
     L(k_seg_classify, dim3(gs), dim3(kBlock), 0, static_cast<const BuildLevel*>(cur), level, sg,
-      st, bfs, sh.bucket, dim);
+      st, bfs, sh.bucket, dim, sc.local_t, sc.local.p, ctr);
     exclusive_scan<uint32_t>(L, st.split, st.childbase, ScanCount{&cur->S, 0, nullptr},
                              sc.partials32.p, sc.splits.p);
     const uint32_t seq = eager ? mailbox.next() : 0u;
@@ -1007,7 +1433,34 @@ void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t str
   }
   const int levels_run = level;
   BuildLevel* fin = lvl + (levels_run & 1);
-  if (levels_out) *levels_out = levels_run;
+
+  // The segments left to LDS, then the node count and depth.
+  if (sc.local_t > 0) {
+    int dev = 0, cus = 256;
+    NH_CHECK(hipGetDevice(&dev));
+    NH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const unsigned g = unsigned(std::min<int64_t>(int64_t(sc.local.count),
+                                                  int64_t(std::max(1, cus)) * kLocalPerCu));
+    if (dim == 3)
+      L(k_local_finish<3>, dim3(g), dim3(kLocalBlock), 0, static_cast<const LocalSeg*>(sc.local.p),
+        ctr, reinterpret_cast<float4*>(recs), bfs, r0, sh.bucket);
+    else if (dim == 2)
+      L(k_local_finish<2>, dim3(g), dim3(kLocalBlock), 0, static_cast<const LocalSeg*>(sc.local.p),
+        ctr, reinterpret_cast<float4*>(recs), bfs, r0, sh.bucket);
+    else
+      L(k_local_finish<1>, dim3(g), dim3(kLocalBlock), 0, static_cast<const LocalSeg*>(sc.local.p),
+        ctr, reinterpret_cast<float4*>(recs), bfs, r0, sh.bucket);
+  }
+  {
+    const uint32_t seq = eager ? mailbox.next() : 0u;
+    L(k_build_totals, dim3(1), dim3(64), 0, static_cast<const BuildLevel*>(fin),
+      static_cast<const BuildCounters*>(ctr), tot, eager ? mailbox.box() : nullptr, seq);
+    if (eager) {
+      node_base = mailbox.wait(seq, L.stream());
+      if (levels_out) *levels_out = mailbox.extra() + 1;
+    }
+  }
+  if (!eager && levels_out) *levels_out = levels;
   if (nodes_out) *nodes_out = eager ? node_base : -1;
 
   // Pre-order ids from the leaf-start scan, then the node words and links.
@@ -1016,20 +1469,22 @@ void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t str
     NH_CHECK(hipMalloc(reinterpret_cast<void**>(&out.links), sizeof(uint4) * node_base));
   }
   const int32_t* ok = out.ok;
-  if (out.ok) L(k_build_ok, dim3(1), dim3(64), 0, static_cast<const BuildLevel*>(fin), out.ok);
+  if (out.ok)
+    L(k_build_ok, dim3(1), dim3(64), 0, static_cast<const BuildLevel*>(fin),
+      static_cast<const BuildLevel*>(tot), int32_t(levels - 1), out.ok);
   const unsigned gn = unsigned(std::min<int64_t>(kGridStride, grid_for(2 * n)));
   L(k_fill_u32, dim3(gn), dim3(kBlock), 0, sc.leaf_start.p, n + 1, 0u, ok);
-  L(k_mark_leaves, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs, dim,
+  L(k_mark_leaves, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs, dim, ids,
     sc.leaf_start.p, ok);
   exclusive_scan<uint32_t>(L, sc.leaf_start.p, sc.leaves_before.p, ScanCount{nullptr, n + 1, ok},
                            sc.partials32.p, static_cast<uint32_t*>(nullptr));
-  L(k_preorder, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs,
+  L(k_preorder, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs, ids,
     static_cast<const uint32_t*>(sc.leaves_before.p), static_cast<const int32_t*>(sc.cid.p), ok);
-  L(k_encode, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs, sh.dim_bits,
-    dim, out.nodes, ok);
-  L(k_links_self, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin),
+  L(k_encode, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs, ids,
+    sh.dim_bits, dim, out.nodes, ok);
+  L(k_links_self, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(tot),
     static_cast<const uint2*>(out.nodes), sh.dim_bits, sh.dim_mask, dim, out.links, out.nodes, ok);
-  L(k_links_parent, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin),
+  L(k_links_parent, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(tot),
     static_cast<const uint2*>(out.nodes), sh.dim_bits, sh.dim_mask, dim, out.links, ok);
   L(k_pad_points, dim3(grid_for(int64_t(kPtsPad) * ps)), dim3(kBlock), 0, n, ps, out.pts);
   if (out.roots)
@@ -1047,7 +1502,7 @@ void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t str
 size_t kd_build_record_bytes(const KdBuildShape& sh) {
   Scratch sc(nullptr);
   BuildScratch b;
-  b.want(sc, sh.n, sh.dim, sh.pts_stride, sh.clouds);
+  b.want(sc, sh.n, sh.dim, sh.pts_stride, sh.clouds, sh.bucket);
   return sc.bytes();
 }
 
@@ -1057,7 +1512,7 @@ void kd_build_record(const KdBuildShape& sh, const float* d_cloud, int64_t strid
   if (levels < 1) throw Error(NABO_HIP_ERR_ARGUMENT, "a recorded build needs at least one level");
   Scratch sc(nullptr, scratch, scratch_bytes);
   BuildScratch b;
-  b.want(sc, sh.n, sh.dim, sh.pts_stride, sh.clouds);
+  b.want(sc, sh.n, sh.dim, sh.pts_stride, sh.clouds, sh.bucket);
   sc.commit(0);
   NH_CHECK(hipMemcpy(b.d_offsets.p, h_offsets, sizeof(int64_t) * (size_t(sh.clouds) + 1),
                      hipMemcpyHostToDevice));
@@ -1078,7 +1533,7 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
                      sizeof(float) * size_t(n + kPtsPad) * ix.pts_stride));
   Scratch scratch(s);
   BuildScratch sc;
-  sc.want(scratch, n, dim, ix.pts_stride, clouds);
+  sc.want(scratch, n, dim, ix.pts_stride, clouds, ix.bucket_size);
   int dev = 0;
   NH_CHECK(hipGetDevice(&dev));
   scratch.commit(dev);
