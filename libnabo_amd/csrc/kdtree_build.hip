@@ -114,7 +114,7 @@ struct BuildCounters {
 };
 constexpr int kLocalT = 2048;     // points of a segment finished in LDS, at most
 constexpr int kLocalSegs = 228;   // splitting segments of one LDS level, at most
-constexpr int kLocalBlock = 256;
+constexpr int kLocalBlock = 512;
 constexpr int kLocalPerCu = 2;    // k_local_finish workgroups resident per CU (LDS)
 constexpr int kLocalItems = kLocalT / kLocalBlock;  // consecutive points per thread
 
