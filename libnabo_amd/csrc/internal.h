@@ -144,6 +144,7 @@ struct Index {
   uint32_t dim_bits = 0;
   uint32_t dim_mask = 0;
   int depth = 0;
+  int build_levels = 0;  // levels the build's level loop ran (deeper parts finished in LDS)
   int64_t node_count = 0;
   uint2* d_nodes = nullptr;   // {dim | child << dim_bits, cut bits or bucket start},
                               // node_count entries + one padding leaf
@@ -287,12 +288,14 @@ class Launcher;
 
 // Scratch bytes of a recorded build of this shape.
 size_t kd_build_record_bytes(const KdBuildShape& sh);
-// Records the build's launches (with `levels` levels) through L, with its
-// scratch in [scratch, scratch + scratch_bytes), which must live as long as
-// the recording is replayed; copies the host offsets into it first.
+// Records the build's launches (`levels` levels of the level loop) through
+// L, with its scratch in [scratch, scratch + scratch_bytes), which must live
+// as long as the recording is replayed; copies the host offsets into it
+// first. A replay is complete (out.ok) when the levels finished the loop and
+// the tree is at most max_depth levels deep.
 void kd_build_record(const KdBuildShape& sh, const float* d_cloud, int64_t stride,
                      const int64_t* h_offsets, void* scratch, size_t scratch_bytes,
-                     KdBuildOut& out, Launcher& L, int levels);
+                     KdBuildOut& out, Launcher& L, int levels, int max_depth);
 
 // Builds the kd-tree of d_cloud on the GPU (kdtree_build.hip).
 // offsets: clouds + 1 point offsets into d_cloud ({0, n} for one cloud).

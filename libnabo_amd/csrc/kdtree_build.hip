@@ -1323,14 +1323,16 @@ struct BuildScratch {
 // The build's launch sequence. The offsets must already be in
 // scratch d_offsets. Eager (levels == 0): levels are launched until one
 // splits nothing, reading each level's split count from the mailbox;
-// *levels_out and *nodes_out receive the level count and node count, and the
-// node arrays are allocated once the count is known (out.nodes / out.links
-// null on entry). Recorded (levels > 0): exactly `levels` levels, node
-// arrays preallocated for 2n nodes; out.ok receives 0 if the tree needed
-// more levels, in which case the node arrays are left unwritten.
+// *levels_out and *nodes_out receive the tree's depth + 1 and node count,
+// *loop_out the levels the loop ran, and the node arrays are allocated once
+// the count is known (out.nodes / out.links null on entry). Recorded
+// (levels > 0): exactly `levels` levels of the loop, node arrays
+// preallocated for 2n nodes; out.ok receives 0 if the tree needed more
+// levels or is deeper than max_depth, in which case the node arrays are left
+// unwritten.
 void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t stride,
                        BuildScratch& sc, KdBuildOut& out, Launcher& L, int levels,
-                       int* levels_out, int64_t* nodes_out) {
+                       int max_depth, int* levels_out, int* loop_out, int64_t* nodes_out) {
   const int64_t n = sh.n;
   const int dim = sh.dim;
   const int ps = sh.pts_stride;
@@ -1461,6 +1463,7 @@ void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t str
     }
   }
   if (!eager && levels_out) *levels_out = levels;
+  if (loop_out) *loop_out = levels_run;
   if (nodes_out) *nodes_out = eager ? node_base : -1;
 
   // Pre-order ids from the leaf-start scan, then the node words and links.
@@ -1471,7 +1474,7 @@ void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t str
   const int32_t* ok = out.ok;
   if (out.ok)
     L(k_build_ok, dim3(1), dim3(64), 0, static_cast<const BuildLevel*>(fin),
-      static_cast<const BuildLevel*>(tot), int32_t(levels - 1), out.ok);
+      static_cast<const BuildLevel*>(tot), int32_t(max_depth), out.ok);
   const unsigned gn = unsigned(std::min<int64_t>(kGridStride, grid_for(2 * n)));
   L(k_fill_u32, dim3(gn), dim3(kBlock), 0, sc.leaf_start.p, n + 1, 0u, ok);
   L(k_mark_leaves, dim3(gn), dim3(kBlock), 0, static_cast<const BuildLevel*>(fin), bfs, dim, ids,
@@ -1508,7 +1511,7 @@ size_t kd_build_record_bytes(const KdBuildShape& sh) {
 
 void kd_build_record(const KdBuildShape& sh, const float* d_cloud, int64_t stride,
                      const int64_t* h_offsets, void* scratch, size_t scratch_bytes,
-                     KdBuildOut& out, Launcher& L, int levels) {
+                     KdBuildOut& out, Launcher& L, int levels, int max_depth) {
   if (levels < 1) throw Error(NABO_HIP_ERR_ARGUMENT, "a recorded build needs at least one level");
   Scratch sc(nullptr, scratch, scratch_bytes);
   BuildScratch b;
@@ -1516,7 +1519,7 @@ void kd_build_record(const KdBuildShape& sh, const float* d_cloud, int64_t strid
   sc.commit(0);
   NH_CHECK(hipMemcpy(b.d_offsets.p, h_offsets, sizeof(int64_t) * (size_t(sh.clouds) + 1),
                      hipMemcpyHostToDevice));
-  kd_build_sequence(sh, d_cloud, stride, b, out, L, levels, nullptr, nullptr);
+  kd_build_sequence(sh, d_cloud, stride, b, out, L, levels, max_depth, nullptr, nullptr, nullptr);
 }
 
 void kd_build(Index& ix, const float* d_cloud, int64_t stride,
@@ -1550,9 +1553,9 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
     }
   }
   Launcher L(s);
-  int levels = 0;
+  int levels = 0, loop = 0;
   int64_t count = 0;
-  kd_build_sequence(sh, d_cloud, stride, sc, out, L, 0, &levels, &count);
+  kd_build_sequence(sh, d_cloud, stride, sc, out, L, 0, 0, &levels, &loop, &count);
   ix.d_nodes = out.nodes;
   ix.d_links = out.links;
   std::vector<float> hb(size_t(2) * clouds * dim);
@@ -1572,6 +1575,7 @@ void kd_build(Index& ix, const float* d_cloud, int64_t stride,
   }
   ix.node_count = count;
   ix.depth = levels - 1;
+  ix.build_levels = loop;
 }
 
 void kd_make_boxes(Index& ix, hipStream_t s) {

@@ -17,9 +17,11 @@
 //
 // The level budget: a tree needs one level per split depth, unknown before
 // the data is seen. With levels == 0 the plan builds the data once at
-// creation (eagerly) and records its depth plus kLevelMargin; a launch
-// whose data needs more levels finishes no tree and runs no search (both
-// check a completeness word the build writes), which
+// creation (eagerly) and takes its depth plus kLevelMargin as the budget,
+// recording as many levels of the build's level loop as that build ran plus
+// kLevelMargin (the deeper levels are finished in LDS); a launch whose data
+// needs more loop levels, or is deeper than the budget, finishes no tree and
+// runs no search (both check a completeness word the build writes), which
 // nabo_hip_batch_plan_status reports.
 
 #include <hip/hip_runtime.h>
@@ -141,6 +143,7 @@ int nabo_hip_batch_plan_create(const float* d_points, const int64_t* offsets, in
       check_knn(ix, rows, qstride, m, k, option_flags);
 
       // The level budget: learned from one eager build of the data.
+      int loop_levels = levels;
       if (levels == 0) {
         nabo_hip_index* probe = nullptr;
         hipStream_t s = nullptr;
@@ -148,11 +151,16 @@ int nabo_hip_batch_plan_create(const float* d_points, const int64_t* offsets, in
         const int rc = nabo_hip_batch_create(d_points, off.data(), clouds, rows, stride, dim, 0,
                                              &p, s, &probe);
         nabo_hip_info info{};
-        if (rc == NABO_HIP_OK) nabo_hip_get_info(probe, &info);
+        int probe_loop = 0;
+        if (rc == NABO_HIP_OK) {
+          nabo_hip_get_info(probe, &info);
+          probe_loop = probe->ix.build_levels;
+        }
         nabo_hip_destroy(probe);
         (void)hipStreamDestroy(s);
         if (rc != NABO_HIP_OK) throw Error(rc, nabo_hip_last_error());
         levels = std::min(kMaxPlanLevels, info.depth + 1 + kLevelMargin);
+        loop_levels = std::min(levels, probe_loop + kLevelMargin);
         if (info.depth + 1 > kMaxPlanLevels)
           throw Error(NABO_HIP_ERR_UNSUPPORTED,
                       "batch plans serve trees of at most " + std::to_string(kMaxPlanLevels - 1) +
@@ -220,7 +228,7 @@ int nabo_hip_batch_plan_create(const float* d_points, const int64_t* offsets, in
       bo.boxes = ix.d_boxes;
       bo.ok = plan->ok;
       kd_build_record(sh, d_points, stride, off.data(), plan->build_scratch, build_bytes, bo, L,
-                      levels);
+                      loop_levels, levels - 1);
       if (m > 0 && k > 0) {
         L(k_query_clouds, dim3(unsigned(ceil_div(m, 256))), dim3(256), 0, m, clouds,
           static_cast<const int64_t*>(plan->d_query_offsets), plan->qcloud);
