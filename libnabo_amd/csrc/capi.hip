@@ -426,9 +426,39 @@ class CopyPool {
       std::memcpy(dst, src, len);
       return;
     }
+    run(static_cast<char*>(dst), static_cast<const char*>(src), len);
+  }
+  // Writes a zero into every page of [dst, dst + len): pages not yet backed
+  // by memory take their first-touch faults now, on all workers, instead of
+  // one piece at a time during the copies into them.
+  void touch(void* dst, size_t len) {
+    if (len == 0) return;
+    if (workers_ == 0 || len < kParallelMin) {
+      touch_range(static_cast<char*>(dst), len);
+      return;
+    }
+    run(static_cast<char*>(dst), nullptr, len);
+  }
+
+ private:
+  static constexpr size_t kParallelMin = size_t(2) << 20;
+  static constexpr size_t kPieceMin = size_t(256) << 10;
+  static constexpr size_t kPage = 4096;
+  struct Job {
+    char* dst = nullptr;
+    const char* src = nullptr;  // null: touch the pages of dst
+    size_t len = 0, piece = 0, count = 0;
+    std::atomic<size_t> next{0}, done{0};
+    std::atomic<int> users{0};
+  };
+  static void touch_range(char* p, size_t len) {
+    for (size_t off = 0; off < len; off += kPage) *reinterpret_cast<volatile char*>(p + off) = 0;
+    *reinterpret_cast<volatile char*>(p + len - 1) = 0;
+  }
+  void run(char* dst, const char* src, size_t len) {
     Job j;
-    j.dst = static_cast<char*>(dst);
-    j.src = static_cast<const char*>(src);
+    j.dst = dst;
+    j.src = src;
     j.len = len;
     j.piece = std::max(kPieceMin, (len + 2 * (workers_ + 1) - 1) / (2 * (workers_ + 1)));
     j.count = (len + j.piece - 1) / j.piece;
@@ -449,17 +479,6 @@ class CopyPool {
            j.users.load(std::memory_order_acquire) > 0)
       std::this_thread::yield();
   }
-
- private:
-  static constexpr size_t kParallelMin = size_t(2) << 20;
-  static constexpr size_t kPieceMin = size_t(256) << 10;
-  struct Job {
-    char* dst = nullptr;
-    const char* src = nullptr;
-    size_t len = 0, piece = 0, count = 0;
-    std::atomic<size_t> next{0}, done{0};
-    std::atomic<int> users{0};
-  };
   CopyPool() {
     const unsigned hw = std::thread::hardware_concurrency();
     workers_ = hw > 1 ? std::min(hw, 16u) - 1 : 0;
@@ -476,7 +495,10 @@ class CopyPool {
     const size_t i = j.next.fetch_add(1, std::memory_order_relaxed);
     if (i >= j.count) return false;
     const size_t off = i * j.piece;
-    std::memcpy(j.dst + off, j.src + off, std::min(j.piece, j.len - off));
+    if (j.src)
+      std::memcpy(j.dst + off, j.src + off, std::min(j.piece, j.len - off));
+    else
+      touch_range(j.dst + off, std::min(j.piece, j.len - off));
     j.done.fetch_add(1, std::memory_order_release);
     return true;
   }
@@ -583,7 +605,8 @@ class HostStage {
     }
   }
   // Device to host, for several arrays: all DMAs are enqueued (one event
-  // per chunk), then each chunk is copied out as soon as it has landed.
+  // per chunk), the destination pages are faulted in while the GPU works,
+  // then each chunk is copied out as soon as it has landed.
   struct Out {
     void* dst;
     const void* src;
@@ -614,6 +637,7 @@ class HostStage {
         stage_off += len;
       }
     }
+    for (int i = 0; i < count; ++i) CopyPool::get().touch(outs[i].dst, outs[i].bytes);
     for (int j = 0; j < n; ++j) {
       NH_CHECK(hipEventSynchronize(b_.ev[j]));
       CopyPool::get().copy(pieces[j].dst, b_.p + pieces[j].stage_off, pieces[j].len);
