@@ -662,6 +662,11 @@ class HostStage {
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
+void host_copy(void* dst, const void* src, size_t bytes) {
+  if (bytes) CopyPool::get().copy(dst, src, bytes);
+}
+void host_touch(void* dst, size_t bytes) { CopyPool::get().touch(dst, bytes); }
+
 void throw_hip(hipError_t e, const char* what, const char* file, int line) {
   const int code = (e == hipErrorOutOfMemory) ? NABO_HIP_ERR_MEMORY : NABO_HIP_ERR_HIP;
   throw Error(code, std::string(what) + " failed: " + hipGetErrorString(e) + " (" + file + ":" +

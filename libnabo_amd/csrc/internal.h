@@ -207,6 +207,11 @@ struct KnnArgs {
   size_t* measure = nullptr;    // set: kd_knn_launch only computes the call's scratch bytes
 };
 
+// Host memory copies on the library's pool of copying threads (up to 16),
+// and the first-touch faults of a destination taken ahead of a copy into it.
+void host_copy(void* dst, const void* src, size_t bytes);
+void host_touch(void* dst, size_t bytes);
+
 // Makes `dev` the current device for the scope, and restores the previous
 // one after.
 struct DeviceScope {

@@ -616,8 +616,11 @@ int nabo_hip_server_submit(nabo_hip_server* s, int32_t cloud_id, const float* qu
         delete r;
         throw;
       }
-      for (int64_t i = 0; i < m; ++i)
-        std::memcpy(r->h_q + i * dim, query + i * qstride, sizeof(float) * dim);
+      if (qstride == dim)
+        nabo_hip::host_copy(r->h_q, query, sizeof(float) * size_t(m) * size_t(dim));
+      else
+        for (int64_t i = 0; i < m; ++i)
+          std::memcpy(r->h_q + i * dim, query + i * qstride, sizeof(float) * dim);
     }
     {
       std::lock_guard<std::mutex> lock(s->mu);
@@ -684,15 +687,25 @@ int nabo_hip_server_wait(nabo_hip_server* s, int64_t request_id, double* latency
     auto it = s->requests.find(request_id);
     if (it == s->requests.end()) throw Error(NABO_HIP_ERR_ARGUMENT, "unknown request id");
     Request* r = it->second;
+    if (!r->done && r->m > 0 && r->out_idx && r->out_d2) {
+      // the output pages take their first-touch faults while the device works
+      const size_t bytes = sizeof(float) * size_t(r->m) * size_t(r->k);
+      int32_t* oi = r->out_idx;
+      float* od = r->out_d2;
+      lock.unlock();
+      nabo_hip::host_touch(oi, bytes);
+      nabo_hip::host_touch(od, bytes);
+      lock.lock();
+    }
     s->finished.wait(lock, [&] { return r->done; });
     if (latency_ms) *latency_ms = r->latency;
     status = r->status;
     const std::string err = r->error;
-    s->requests.erase(it);
+    s->requests.erase(request_id);  // by key: `it` may predate a rehash
     lock.unlock();
     if (status == NABO_HIP_OK && r->m > 0) {
-      std::memcpy(r->out_idx, r->h_i, sizeof(int32_t) * r->m * r->k);
-      std::memcpy(r->out_d2, r->h_d, sizeof(float) * r->m * r->k);
+      nabo_hip::host_copy(r->out_idx, r->h_i, sizeof(int32_t) * r->m * r->k);
+      nabo_hip::host_copy(r->out_d2, r->h_d, sizeof(float) * r->m * r->k);
     }
     s->release(r);
     delete r;
