@@ -12,7 +12,9 @@ outside the box), then checks one search type:
   and the TOUCH_STATISTICS count;
 - BRUTE_FORCE (IndexHeapSTL, brute_force_cpu.cpp:45-103): bit for bit;
 - BRUTE_FORCE_HIP: dists2 bit for bit, indices equal up to ties;
-- batched trees: every cloud's results bit for bit.
+- batched trees: every cloud's results bit for bit;
+- large k (49-200) and high dimensions (up to 100), for the kd-tree's
+  larger k-best forms and the brute force's index-order scan.
 The case's parameters are printed before it runs.
 """
 import numpy as np
@@ -163,3 +165,57 @@ def test_batched_sweep(nabo, oracle, seed):
         oi, od = oracle.kdtree(cl, bucket_size=bucket).knn(q, k, flags=flags, threads=8)
         assert_bit_exact(gi[qoff[c]:qoff[c + 1]], gd[qoff[c]:qoff[c + 1]], oi, od, f"cloud {c}")
     b.close()
+
+
+def _draw_large(seed):
+    """Large k (the 64/96/128-entry, output-row and sorted-row k-best forms)
+    and high dimensions (offsets in HBM above 64)."""
+    rng = np.random.default_rng(3000 + seed)
+    shape = SHAPES[seed % len(SHAPES)]
+    rows = int(rng.choice([2, 3, 3, 8, 70, 100]))
+    n = int(rng.integers(300, 6_000 if rows > 8 else 20_000))
+    flags = int(rng.integers(0, 4))
+    k = int(min(n - 1, rng.choice([49, 64, 65, 96, 97, 128, 129, 200])))
+    eps = float(rng.choice([0.0, 0.0, 0.2]))
+    cloud = _cloud(rng, shape, n, rows)
+    query = _queries(rng, cloud, int(rng.integers(8, 600)))
+    radius = np.inf
+    if rng.random() < 0.3:
+        s = cloud[rng.integers(0, n, 64)]
+        radius = float(np.float32(np.median(np.sqrt(((s[:, None] - s[None]) ** 2).sum(-1)))))
+    return dict(shape=shape, n=n, rows=rows, k=k, eps=eps, flags=flags, m=len(query),
+                radius=radius), cloud, query
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_large_k_sweep(nabo, oracle, seed):
+    p, cloud, query = _draw_large(seed)
+    heap = "stl" if _coin(7000 + seed, 0.5) else "linear"
+    print(f"case sweep large kd seed={seed} heap={heap} {_label(p)}", flush=True)
+    st = nabo.SearchType.KDTREE_TREE_HEAP if heap == "stl" else nabo.SearchType.KDTREE_LINEAR_HEAP
+    nns = nabo.NearestNeighbourSearch(cloud, st, p["rows"], {"bucketSize": 8},
+                                      creationOptionFlags=nabo.CreationOptionFlags.TOUCH_STATISTICS)
+    otree = oracle.kdtree(cloud, bucket_size=8)
+    gi, gd = nns.knn(query, p["k"], p["eps"], p["flags"], p["radius"])
+    oi, od, otouch, _ = otree.knn(query, p["k"], p["eps"], p["flags"], p["radius"], heap=heap,
+                                  threads=8, stats=True)
+    assert_bit_exact(gi, gd, oi, od, f"large kd sweep {seed}")
+    assert nns.last_touched == int(otouch.sum()), "touched count differs"
+    nns.close()
+
+
+@pytest.mark.parametrize("seed", range(24, 36))
+def test_large_k_brute_force_sweep(nabo, oracle, seed):
+    p, cloud, query = _draw_large(seed)
+    exact = _coin(7000 + seed, 0.5)
+    print(f"case sweep large bf seed={seed} stl={exact} {_label(p)}", flush=True)
+    st = nabo.SearchType.BRUTE_FORCE if exact else nabo.SearchType.BRUTE_FORCE_HIP
+    bf = nabo.NearestNeighbourSearch(cloud, st, p["rows"])
+    flags = p["flags"] if exact else p["flags"] | SORT
+    gi, gd = bf.knn(query, p["k"], 0.0, flags, p["radius"])
+    oi, od = oracle.bf_knn(cloud, query, p["k"], flags=flags, max_radius=p["radius"], threads=8)
+    if exact:
+        assert_bit_exact(gi, gd, oi, od, f"large bf sweep {seed}")
+    else:
+        assert_same_up_to_ties(gi, gd, oi, od, f"large bf sweep {seed}")
+    bf.close()
