@@ -107,7 +107,9 @@ void nabo_hip_params_default(nabo_hip_params* p);
  * kd-tree / brute-force constructors (nabo/kdtree_cpu.cpp:227-272,
  * nabo/brute_force_cpu.cpp:45-60). dim is clamped to rows (nabo/nabo.cpp:71).
  * The tree is built on the GPU; the cloud is copied to HBM, so the host
- * buffer need not outlive the call. */
+ * buffer need not outlive the call. Kd-trees take fewer than 2^29 points,
+ * brute force fewer than 2^30, and a knn call fewer than 2^31 queries
+ * (NABO_HIP_ERR_UNSUPPORTED otherwise). */
 int nabo_hip_create(const float* cloud, int64_t n, int32_t rows, int64_t stride, int32_t dim,
                     int32_t search_type, uint32_t creation_flags, const nabo_hip_params* params,
                     nabo_hip_index** out);

@@ -222,6 +222,9 @@ void check_create(int64_t n, int32_t rows, int64_t stride, int32_t dim, int32_t 
     arg_error("Unknown search type");
   if (n >= (int64_t(1) << 30))
     throw Error(NABO_HIP_ERR_UNSUPPORTED, "clouds of 2^30 points or more are not supported");
+  if (type == NABO_HIP_SEARCH_KDTREE && n >= kKdPointsLimit)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED,
+                "kd-tree clouds of 2^29 points or more are not supported");
   if (p.heap != NABO_HIP_HEAP_LINEAR && p.heap != NABO_HIP_HEAP_STL)
     arg_error("Unknown heap type " + std::to_string(p.heap));
   if (p.wave_queries != 0 && p.wave_queries != 64 && p.wave_queries != 128 &&
@@ -274,6 +277,8 @@ void check_knn(const Index& ix, int32_t qrows, int64_t qstride, int64_t m, int32
               ") is larger than maximal valid value (3)");
   if (k < 0) arg_error("k must not be negative");
   if (m < 0) arg_error("Query has a negative number of columns");
+  if (m >= kQueryLimit)
+    throw Error(NABO_HIP_ERR_UNSUPPORTED, "2^31 queries or more in one call are not supported");
   if (m > 0 && qstride < qrows) arg_error("Query column stride is smaller than its number of rows");
 }
 
@@ -973,8 +978,8 @@ int nabo_hip_batch_create(const float* d_points, const int64_t* offsets, int32_t
       check_create(off[c + 1] - off[c], rows, stride, dim, NABO_HIP_SEARCH_KDTREE, p);
     }
     const int64_t n = off[clouds];
-    if (n >= (int64_t(1) << 30))
-      throw Error(NABO_HIP_ERR_UNSUPPORTED, "batches of 2^30 points or more are not supported");
+    if (n >= kKdPointsLimit)
+      throw Error(NABO_HIP_ERR_UNSUPPORTED, "batches of 2^29 points or more are not supported");
     if (!d_points) arg_error("points pointer is null");
     int dev = p.device;
     if (dev < 0) NH_CHECK(hipGetDevice(&dev));

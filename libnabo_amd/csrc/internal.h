@@ -330,6 +330,12 @@ struct KernelTimer {
 };
 bool kernel_timing_enabled();
 
+// kd-tree clouds (and batches) hold fewer points: the build numbers the
+// nodes of its LDS-finished subtrees from 2n up, below 4n, in int32 words.
+constexpr int64_t kKdPointsLimit = int64_t(1) << 29;
+// Query counts stay below this: the query order holds int32 positions.
+constexpr int64_t kQueryLimit = int64_t(1) << 31;
+
 // Largest k the register-resident k-best arrays support.
 constexpr int kMaxK = 128;
 // Zeroed entries after the leaf-ordered points, so a leaf scan can load a

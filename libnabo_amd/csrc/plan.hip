@@ -108,8 +108,8 @@ int nabo_hip_batch_plan_create(const float* d_points, const int64_t* offsets, in
       check_create(off[c + 1] - off[c], rows, stride, dim, NABO_HIP_SEARCH_KDTREE, p);
     }
     const int64_t n = off[clouds], m = qoff[clouds];
-    if (n >= (int64_t(1) << 30))
-      throw Error(NABO_HIP_ERR_UNSUPPORTED, "batches of 2^30 points or more are not supported");
+    if (n >= kKdPointsLimit)
+      throw Error(NABO_HIP_ERR_UNSUPPORTED, "batches of 2^29 points or more are not supported");
     const int used_dim = std::min(dim, rows);
     if (used_dim != 2 && used_dim != 3)
       throw Error(NABO_HIP_ERR_UNSUPPORTED, "batch plans serve 2-D and 3-D clouds");

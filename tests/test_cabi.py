@@ -98,3 +98,20 @@ def test_schedule_params_checked_before_any_device_call():
         assert "bogus" in str(e)
     else:
         raise AssertionError("unknown parameter accepted")
+
+
+def test_cloud_size_limits_checked_before_any_device_call():
+    # The point count is checked before the cloud is read: a 10-point buffer
+    # stands for clouds of 2^29 and 2^30 points.
+    import numpy as np
+
+    lib = nabo.load_library()
+    p = nabo.make_params()
+    cloud = np.zeros((10, 3), np.float32)
+    for n, stype, text in ((1 << 29, int(nabo.SearchType.KDTREE_HIP), b"kd-tree clouds of 2^29"),
+                           (1 << 30, int(nabo.SearchType.BRUTE_FORCE_HIP), b"clouds of 2^30")):
+        h = ctypes.c_void_p()
+        rc = lib.nabo_hip_create(cloud.ctypes.data_as(ctypes.c_void_p), n, 3, 3, 3, stype, 0,
+                                 ctypes.byref(p), ctypes.byref(h))
+        assert rc == 4 and text in lib.nabo_hip_last_error(), (n, lib.nabo_hip_last_error())
+        assert not h.value
