@@ -454,7 +454,7 @@ def _served_run(nabo, pool, reqs, arrivals, budget, slots, warm, host_cache=0):
     return lat, makespan, st, handles
 
 
-def bench_served(requests: int = 120, slots: int = 4):
+def bench_served(requests: int = 400, slots: int = 4):
     """Served stream over 64 clouds (10k..2M points): Poisson arrivals,
     batches of 1k..500k queries, k in {1, 8, 32}, maxRadius 0.5.
 
