@@ -527,12 +527,9 @@ __global__ __launch_bounds__(kBlock) void k_pt_count(int64_t n, const int32_t* _
 
 // Out-of-place flags of one Hoare pass: low word = belongs right but sits
 // left, high word = belongs left but sits right.
-__global__ void k_pt_flags(int64_t n, const int32_t* __restrict__ gate, int pass,
-                           const int32_t* __restrict__ pseg, const float* __restrict__ xs,
-                           Seg seg, SegStat st, uint64_t* __restrict__ flags) {
-  if (*gate == 0) return;
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+__device__ __forceinline__ uint64_t pt_flag(int64_t p, int pass, const int32_t* __restrict__ pseg,
+                                            const float* __restrict__ xs, const Seg& seg,
+                                            const SegStat& st) {
   const int s = pseg[p];
   uint64_t f = 0;
   if (s >= 0) {
@@ -554,7 +551,29 @@ __global__ void k_pt_flags(int64_t n, const int32_t* __restrict__ gate, int pass
     }
     f = uint64_t(fl) | (uint64_t(fr) << 32);
   }
-  flags[p] = f;
+  return f;
+}
+
+// The flags of every point, laid out as the scan splits them (workgroup b
+// of kScanGrid writes chunk b of scan_chunk(n)), with each workgroup's sum
+// in partials[b]: the scan's first pass, without reading the flags again.
+__global__ __launch_bounds__(kScanThreads) void k_pt_flags(
+    int64_t n, const int32_t* __restrict__ gate, int pass, const int32_t* __restrict__ pseg,
+    const float* __restrict__ xs, Seg seg, SegStat st, uint64_t* __restrict__ flags,
+    uint64_t* __restrict__ partials) {
+  if (*gate == 0) return;
+  const int64_t chunk = scan_chunk(n);
+  const int64_t begin = int64_t(blockIdx.x) * chunk;
+  const int64_t end = begin + chunk < n ? begin + chunk : n;
+  uint64_t sum = 0;
+  for (int64_t p = begin + threadIdx.x; p < end; p += kScanThreads) {
+    const uint64_t f = pt_flag(p, pass, pseg, xs, seg, st);
+    flags[p] = f;
+    sum += f;
+  }
+  uint64_t total;
+  block_exclusive(sum, &total);
+  if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
 __device__ __forceinline__ void pass_ranks(int64_t p, int s, Seg seg,
@@ -1417,11 +1436,12 @@ void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t str
     L(k_tile_partials<2, 2>, dim3(gpart), dim3(kBlock), 0, nparts, parts, st.lt, st.eq,
       int64_t(1), int64_t(0), gate);
     for (int pass = 1; pass <= 2; ++pass) {
-      L(k_pt_flags, dim3(gp), dim3(kBlock), 0, n, gate, pass,
+      L(k_pt_flags, dim3(kScanGrid), dim3(kScanThreads), 0, n, gate, pass,
         static_cast<const int32_t*>(sc.pseg.p), static_cast<const float*>(sc.xs.p), sg, st,
-        sc.flags.p);
-      exclusive_scan<uint64_t>(L, sc.flags.p, sc.excl.p, ScanCount{nullptr, n, gate},
-                               sc.partials64.p, static_cast<uint64_t*>(nullptr));
+        sc.flags.p, sc.partials64.p);
+      exclusive_scan_from_partials<uint64_t>(L, sc.flags.p, sc.excl.p,
+                                             ScanCount{nullptr, n, gate}, sc.partials64.p,
+                                             static_cast<uint64_t*>(nullptr));
       L(k_pt_slots, dim3(gp), dim3(kBlock), 0, n, gate, static_cast<const int32_t*>(sc.pseg.p),
         sg, static_cast<const uint64_t*>(sc.flags.p), static_cast<const uint64_t*>(sc.excl.p),
         sc.slot_r.p);

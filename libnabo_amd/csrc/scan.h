@@ -172,4 +172,15 @@ void exclusive_scan(Launcher& L, const T* in, T* out, ScanCount cnt, T* partials
     static_cast<const T*>(partials));
 }
 
+// The same scan when the kernel that wrote `in` also wrote partials[b], the
+// sum of chunk b of the split above (workgroup b of kScanGrid covering
+// [b * scan_chunk(count), ...)): the spine and the apply only.
+template <class T>
+void exclusive_scan_from_partials(Launcher& L, const T* in, T* out, ScanCount cnt, T* partials,
+                                  T* total) {
+  L(k_scan_spine<T>, dim3(1), dim3(kScanGrid), 0, cnt, partials, total);
+  L(k_scan_apply<T>, dim3(kScanGrid), dim3(kScanThreads), 0, in, out, cnt,
+    static_cast<const T*>(partials));
+}
+
 }  // namespace nabo_hip
