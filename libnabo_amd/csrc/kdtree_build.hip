@@ -557,23 +557,32 @@ __device__ __forceinline__ uint64_t pt_flag(int64_t p, int pass, const int32_t* 
 // The flags of every point, laid out as the scan splits them (workgroup b
 // of kScanGrid writes chunk b of scan_chunk(n)), with each workgroup's sum
 // in partials[b]: the scan's first pass, without reading the flags again.
-__global__ __launch_bounds__(kScanThreads) void k_pt_flags(
+constexpr int kFlagThreads = 1024;  // per workgroup: more points of a chunk in flight
+__global__ __launch_bounds__(kFlagThreads) void k_pt_flags(
     int64_t n, const int32_t* __restrict__ gate, int pass, const int32_t* __restrict__ pseg,
     const float* __restrict__ xs, Seg seg, SegStat st, uint64_t* __restrict__ flags,
     uint64_t* __restrict__ partials) {
   if (*gate == 0) return;
+  __shared__ uint64_t wave_sum[kFlagThreads / kWave];
   const int64_t chunk = scan_chunk(n);
   const int64_t begin = int64_t(blockIdx.x) * chunk;
   const int64_t end = begin + chunk < n ? begin + chunk : n;
   uint64_t sum = 0;
-  for (int64_t p = begin + threadIdx.x; p < end; p += kScanThreads) {
+  for (int64_t p = begin + threadIdx.x; p < end; p += kFlagThreads) {
     const uint64_t f = pt_flag(p, pass, pseg, xs, seg, st);
     flags[p] = f;
     sum += f;
   }
-  uint64_t total;
-  block_exclusive(sum, &total);
-  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) wave_sum[threadIdx.x / kWave] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t total = 0;
+#pragma unroll
+    for (int w = 0; w < kFlagThreads / kWave; ++w) total += wave_sum[w];
+    partials[blockIdx.x] = total;
+  }
 }
 
 __device__ __forceinline__ void pass_ranks(int64_t p, int s, Seg seg,
@@ -1436,7 +1445,7 @@ void kd_build_sequence(const KdBuildShape& sh, const float* d_cloud, int64_t str
     L(k_tile_partials<2, 2>, dim3(gpart), dim3(kBlock), 0, nparts, parts, st.lt, st.eq,
       int64_t(1), int64_t(0), gate);
     for (int pass = 1; pass <= 2; ++pass) {
-      L(k_pt_flags, dim3(kScanGrid), dim3(kScanThreads), 0, n, gate, pass,
+      L(k_pt_flags, dim3(kScanGrid), dim3(kFlagThreads), 0, n, gate, pass,
         static_cast<const int32_t*>(sc.pseg.p), static_cast<const float*>(sc.xs.p), sg, st,
         sc.flags.p, sc.partials64.p);
       exclusive_scan_from_partials<uint64_t>(L, sc.flags.p, sc.excl.p,
