@@ -12,7 +12,8 @@ outside the box), then checks one search type:
   and the TOUCH_STATISTICS count;
 - BRUTE_FORCE (IndexHeapSTL, brute_force_cpu.cpp:45-103): bit for bit;
 - BRUTE_FORCE_HIP: dists2 bit for bit, indices equal up to ties;
-- batched trees: every cloud's results bit for bit;
+- batched trees (epsilon and radius drawn too): every cloud's results bit
+  for bit;
 - large k (49-200) and high dimensions (up to 100), for the kd-tree's
   larger k-best forms and the brute force's index-order scan.
 The case's parameters are printed before it runs.
@@ -152,17 +153,19 @@ def test_batched_sweep(nabo, oracle, seed):
     clouds = [_cloud(rng, SHAPES[(seed + i) % len(SHAPES)], s, rows) for i, s in enumerate(sizes)]
     queries = [_queries(rng, c, int(rng.integers(1, 800)) + 4) for c in clouds]
     flags = int(rng.integers(0, 4))
+    eps = float(rng.choice([0.0, 0.0, 0.1]))
+    radius = float(rng.choice([np.inf, np.inf, 0.3, 2.0]))
     print(f"case sweep batch seed={seed} rows={rows} bucket={bucket} k={k} flags={flags} "
-          f"sizes={sizes}", flush=True)
+          f"eps={eps} radius={radius} sizes={sizes}", flush=True)
     off = np.concatenate([[0], np.cumsum([len(c) for c in clouds])]).astype(np.int64)
     qoff = np.concatenate([[0], np.cumsum([len(q) for q in queries])]).astype(np.int64)
     b = nabo.BatchSearch(torch.from_numpy(np.concatenate(clouds)).cuda(), off,
                          params={"bucketSize": bucket})
     gi, gd = b.knn(torch.from_numpy(np.concatenate(queries)).cuda(), torch.from_numpy(qoff).cuda(),
-                   k, optionFlags=flags)
+                   k, epsilon=eps, optionFlags=flags, maxRadius=radius)
     gi, gd = gi.cpu().numpy(), gd.cpu().numpy()
     for c, (cl, q) in enumerate(zip(clouds, queries)):
-        oi, od = oracle.kdtree(cl, bucket_size=bucket).knn(q, k, flags=flags, threads=8)
+        oi, od = oracle.kdtree(cl, bucket_size=bucket).knn(q, k, eps, flags, radius, threads=8)
         assert_bit_exact(gi[qoff[c]:qoff[c + 1]], gd[qoff[c]:qoff[c + 1]], oi, od, f"cloud {c}")
     b.close()
 
