@@ -5,7 +5,8 @@ bounds, kdtree_cpu.cpp:345-348), non-finite queries (NaN and infinite
 distances are never admitted, :392-394, and never open a far child,
 :415-417), a zero radius (dist <= 0 admits exact duplicates only), a cloud
 of one repeated point, k equal to the cloud size, empty or one-query calls,
-and a tree too deep for stacks in LDS. Every case compares indices, dists2 and the touched count with the
+a tree too deep for stacks in LDS, and clouds and queries with column
+strides above their row counts (host and device buffers). Every case compares indices, dists2 and the touched count with the
 oracle on the same tree.
 """
 import numpy as np
@@ -130,3 +131,55 @@ def test_unread_hip_error_of_another_call_is_not_reported(nabo, oracle):
     gi, gd = nns.knn(query, 4)
     oi, od = oracle.kdtree(cloud).knn(query, 4, threads=8)
     assert np.array_equal(gi, oi) and np.array_equal(gd.view(np.uint32), od.view(np.uint32))
+
+
+@pytest.mark.parametrize("stype", ["kd", "bf"])
+def test_strided_cloud_and_queries(nabo, oracle, stype):
+    # Eigen-style outer strides through the C ABI: a cloud whose points are 5
+    # floats apart, host queries 4 floats apart, and a strided device view;
+    # the extra floats hold values that must not be read.
+    import ctypes
+
+    import torch
+
+    from tests.helpers import assert_bit_exact
+
+    rng = np.random.default_rng(91)
+    n, m, k = 20_000, 3_000, 8
+    cloud = rng.random((n, 3)).astype(np.float32)
+    query = rng.random((m, 3)).astype(np.float32)
+    c5 = np.full((n, 5), 1e6, np.float32)
+    c5[:, :3] = cloud
+    q4 = np.full((m, 4), -1e6, np.float32)
+    q4[:, :3] = query
+    print(f"case strided {stype}", flush=True)
+    lib = nabo.load_library()
+    # brute force with libnabo's IndexHeapSTL (BRUTE_FORCE) is BRUTE_FORCE_HIP + the STL heap
+    p = nabo.make_params(heap="linear" if stype == "kd" else "stl")
+    st = nabo.SearchType.KDTREE_HIP if stype == "kd" else nabo.SearchType.BRUTE_FORCE_HIP
+    h = ctypes.c_void_p()
+    assert lib.nabo_hip_create(c5.ctypes.data_as(ctypes.c_void_p), n, 3, 5, 3, int(st), 0,
+                               ctypes.byref(p), ctypes.byref(h)) == 0, lib.nabo_hip_last_error()
+    try:
+        gi = np.empty((m, k), np.int32)
+        gd = np.empty((m, k), np.float32)
+        assert lib.nabo_hip_knn(h, q4.ctypes.data_as(ctypes.c_void_p), 3, 4, m, k, 0.0, 2,
+                                float("inf"), None, gi.ctypes.data_as(ctypes.c_void_p),
+                                gd.ctypes.data_as(ctypes.c_void_p), None) == 0, \
+            lib.nabo_hip_last_error()
+        if stype == "kd":
+            oi, od = oracle.kdtree(cloud, bucket_size=8).knn(query, k, flags=2, threads=8)
+        else:
+            oi, od = oracle.bf_knn(cloud, query, k, flags=2, threads=8)
+        assert_bit_exact(gi, gd, oi, od, "strided host queries")
+        dq = torch.from_numpy(q4).cuda()[:, :3]  # stride(0) == 4
+        di = torch.empty((m, k), dtype=torch.int32, device="cuda")
+        dd = torch.empty((m, k), dtype=torch.float32, device="cuda")
+        assert lib.nabo_hip_knn_device(h, ctypes.c_void_p(dq.data_ptr()), 3, 4, m, k, 0.0, 2,
+                                       float("inf"), None, ctypes.c_void_p(di.data_ptr()),
+                                       ctypes.c_void_p(dd.data_ptr()), None, None) == 0, \
+            lib.nabo_hip_last_error()
+        torch.cuda.synchronize()
+        assert_bit_exact(di.cpu().numpy(), dd.cpu().numpy(), oi, od, "strided device queries")
+    finally:
+        lib.nabo_hip_destroy(h)
