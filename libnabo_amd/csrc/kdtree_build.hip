@@ -827,24 +827,35 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
       }
       if (t == 0) s_next = 0;
       __syncthreads();
+      // This thread's points for the level, in registers: segment (-1:
+      // done), cut dimension, coordinate along it. The loads of all items
+      // are issued before the first LDS atomic, which would order them.
+      int g[kLocalItems];
+      uint32_t cdv[kLocalItems];
+      float x[kLocalItems];
+#pragma unroll
+      for (int j = 0; j < kLocalItems; ++j) g[j] = pseg[t * kLocalItems + j];
+#pragma unroll
+      for (int j = 0; j < kLocalItems; ++j) cdv[j] = g[j] >= 0 ? c_cd[g[j]] : 0u;
+#pragma unroll
+      for (int j = 0; j < kLocalItems; ++j) x[j] = rec_coord(rec[t * kLocalItems + j], cdv[j]);
       // min / max of the cut coordinate
       {
         int run = -1;
         uint32_t mn = 0xffffffffu, mx = 0u;
+#pragma unroll
         for (int j = 0; j < kLocalItems; ++j) {
-          const int p = t * kLocalItems + j;
-          const int g = pseg[p];
-          if (g != run) {
+          if (g[j] != run) {
             if (run >= 0) {
               atomicMin(&c_min[run], mn);
               atomicMax(&c_max[run], mx);
             }
-            run = g;
+            run = g[j];
             mn = 0xffffffffu;
             mx = 0u;
           }
-          if (g >= 0) {
-            const uint32_t key = float_key(rec_coord(rec[p], c_cd[g]));
+          if (g[j] >= 0) {
+            const uint32_t key = float_key(x[j]);
             mn = min(mn, key);
             mx = max(mx, key);
           }
@@ -855,32 +866,37 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
         }
       }
       __syncthreads();
-      for (int g = t; g < ns; g += kLocalBlock) {
-        const float pmin = key_float(c_min[g]), pmax = key_float(c_max[g]);
-        const float ideal = c_ideal[g];
-        c_cut[g] = ideal < pmin ? pmin : (ideal > pmax ? pmax : ideal);
+      for (int s = t; s < ns; s += kLocalBlock) {
+        const float pmin = key_float(c_min[s]), pmax = key_float(c_max[s]);
+        const float ideal = c_ideal[s];
+        c_cut[s] = ideal < pmin ? pmin : (ideal > pmax ? pmax : ideal);
       }
       __syncthreads();
       // points below / equal to the cut
+      float cut[kLocalItems];
+      int32_t sst[kLocalItems];
+#pragma unroll
+      for (int j = 0; j < kLocalItems; ++j) {
+        cut[j] = g[j] >= 0 ? c_cut[g[j]] : 0.f;
+        sst[j] = g[j] >= 0 ? c_start[g[j]] : 0;
+      }
       {
         int run = -1;
         uint32_t lt = 0u, eq = 0u;
+#pragma unroll
         for (int j = 0; j < kLocalItems; ++j) {
-          const int p = t * kLocalItems + j;
-          const int g = pseg[p];
-          if (g != run) {
+          if (g[j] != run) {
             if (run >= 0) {
               atomicAdd(&c_lt[run], lt);
               atomicAdd(&c_eq[run], eq);
             }
-            run = g;
+            run = g[j];
             lt = 0u;
             eq = 0u;
           }
-          if (g >= 0) {
-            const float x = rec_coord(rec[p], c_cd[g]);
-            lt += x < c_cut[g] ? 1u : 0u;
-            eq += x == c_cut[g] ? 1u : 0u;
+          if (g[j] >= 0) {
+            lt += x[j] < cut[j] ? 1u : 0u;
+            eq += x[j] == cut[j] ? 1u : 0u;
           }
         }
         if (run >= 0) {
@@ -889,37 +905,46 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
         }
       }
       __syncthreads();
+      int32_t br1[kLocalItems], br2[kLocalItems], scnt[kLocalItems];
+#pragma unroll
+      for (int j = 0; j < kLocalItems; ++j) {
+        br1[j] = g[j] >= 0 ? int32_t(c_lt[g[j]]) : 0;
+        br2[j] = br1[j] + (g[j] >= 0 ? int32_t(c_eq[g[j]]) : 0);
+        scnt[j] = g[j] >= 0 ? c_count[g[j]] : 0;
+      }
       // the two Hoare passes: flags (low half: belongs right but sits left,
       // high half: belongs left but sits right), ranks, pairs, swaps
       for (int pass = 1; pass <= 2; ++pass) {
+        if (pass == 2) {
+          // the first pass's swaps moved records between threads
+#pragma unroll
+          for (int j = 0; j < kLocalItems; ++j)
+            x[j] = rec_coord(rec[t * kLocalItems + j], cdv[j]);
+        }
         uint32_t f[kLocalItems];
         uint32_t sum = 0;
 #pragma unroll
         for (int j = 0; j < kLocalItems; ++j) {
-          const int p = t * kLocalItems + j;
-          const int g = pseg[p];
           f[j] = 0u;
-          if (g >= 0) {
-            const int32_t pos = p - c_start[g];
-            const float x = rec_coord(rec[p], c_cd[g]);
-            const float cut = c_cut[g];
-            const int32_t br1 = int32_t(c_lt[g]);
+          if (g[j] >= 0) {
+            const int32_t pos = t * kLocalItems + j - sst[j];
             bool fl, fr;
             if (pass == 1) {
-              fl = pos < br1 && !(x < cut);
-              fr = pos >= br1 && x < cut;
+              fl = pos < br1[j] && !(x[j] < cut[j]);
+              fr = pos >= br1[j] && x[j] < cut[j];
             } else {
-              const int32_t br2 = br1 + int32_t(c_eq[g]);
-              fl = pos >= br1 && pos < br2 && !(x == cut);
-              fr = pos >= br2 && x == cut;
+              fl = pos >= br1[j] && pos < br2[j] && !(x[j] == cut[j]);
+              fr = pos >= br2[j] && x[j] == cut[j];
             }
             f[j] = uint32_t(fl) | (uint32_t(fr) << 16);
           }
           sum += f[j];
         }
+        uint32_t ex[kLocalItems];
         uint32_t run = block_excl_sum(sum, wave_tot);
 #pragma unroll
         for (int j = 0; j < kLocalItems; ++j) {
+          ex[j] = run;
           excl[t * kLocalItems + j] = run;
           run += f[j];
         }
@@ -928,21 +953,17 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
 #pragma unroll
         for (int j = 0; j < kLocalItems; ++j) {
           if ((f[j] >> 16) == 0u) continue;
-          const int p = t * kLocalItems + j;
-          const int g = pseg[p];
-          const int32_t st = c_start[g];
-          const uint32_t base = excl[st] >> 16;
-          const uint32_t total_r = (excl[st + c_count[g]] >> 16) - base;
-          const uint32_t from_left = (excl[p] >> 16) - base;
-          slot_r[st + int32_t(total_r - 1u - from_left)] = int16_t(p);
+          const uint32_t base = excl[sst[j]] >> 16;
+          const uint32_t total_r = (excl[sst[j] + scnt[j]] >> 16) - base;
+          const uint32_t from_left = (ex[j] >> 16) - base;
+          slot_r[sst[j] + int32_t(total_r - 1u - from_left)] = int16_t(t * kLocalItems + j);
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kLocalItems; ++j) {
           if ((f[j] & 0xffffu) == 0u) continue;
           const int p = t * kLocalItems + j;
-          const int32_t st = c_start[pseg[p]];
-          const int q = slot_r[st + int32_t((excl[p] & 0xffffu) - (excl[st] & 0xffffu))];
+          const int q = slot_r[sst[j] + int32_t((ex[j] & 0xffffu) - (excl[sst[j]] & 0xffffu))];
           const float4 a = rec[p];
           rec[p] = rec[q];
           rec[q] = a;
@@ -950,12 +971,12 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
         __syncthreads();
       }
       // leftCount rule, node words, children
-      for (int g = t; g < ns; g += kLocalBlock) {
-        const int32_t cnt = c_count[g];
-        const float ideal = c_ideal[g];
-        const float pmin = key_float(c_min[g]), pmax = key_float(c_max[g]);
-        const int32_t br1 = int32_t(c_lt[g]);
-        const int32_t br2 = br1 + int32_t(c_eq[g]);
+      for (int s = t; s < ns; s += kLocalBlock) {
+        const int32_t cnt = c_count[s];
+        const float ideal = c_ideal[s];
+        const float pmin = key_float(c_min[s]), pmax = key_float(c_max[s]);
+        const int32_t br1 = int32_t(c_lt[s]);
+        const int32_t br2 = br1 + int32_t(c_eq[s]);
         int32_t left;
         if (ideal < pmin)
           left = 1;
@@ -968,19 +989,19 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
         else
           left = cnt / 2;
         left = min(max(left, 1), cnt - 1);
-        c_left[g] = left;
-        const uint32_t cd = c_cd[g];
-        const float cut = c_cut[g];
+        c_left[s] = left;
+        const uint32_t cd = c_cd[s];
+        const float cut = c_cut[s];
         const int64_t id0 = r0 + 2 * S0 + atomicAdd(&s_nodes, 2);
-        const int32_t node = c_node[g];
+        const int32_t node = c_node[s];
         bfs.cd[node] = cd;
         bfs.aux[node] = __float_as_uint(cut);
         bfs.left[node] = int32_t(id0);
         bfs.right[node] = int32_t(id0 + 1);
         for (int side = 0; side < 2; ++side) {
-          const int32_t cs = c_start[g] + (side ? left : 0);
+          const int32_t cs = c_start[s] + (side ? left : 0);
           const int32_t cc = side ? cnt - left : left;
-          const int32_t rc = c_rc[g] + side;
+          const int32_t rc = c_rc[s] + side;
           const int64_t id = id0 + side;
           bfs.start[id] = int32_t(S0 + cs);
           bfs.level[id] = level + 1;
@@ -989,7 +1010,7 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
             bfs.cd[id] = uint32_t(DIM);
             bfs.aux[id] = uint32_t(S0 + cs);
             bfs.count[id] = uint32_t(cc);
-            c_child[2 * g + side] = -1;
+            c_child[2 * s + side] = -1;
             atomicMax(&s_lmax, level + 1);
           } else {
             const int k = atomicAdd(&s_next, 1);
@@ -999,10 +1020,10 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
             n_node[k] = int32_t(id);
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-              n_lo[k * DIM + d] = (side && uint32_t(d) == cd) ? cut : c_lo[g * DIM + d];
-              n_hi[k * DIM + d] = (!side && uint32_t(d) == cd) ? cut : c_hi[g * DIM + d];
+              n_lo[k * DIM + d] = (side && uint32_t(d) == cd) ? cut : c_lo[s * DIM + d];
+              n_hi[k * DIM + d] = (!side && uint32_t(d) == cd) ? cut : c_hi[s * DIM + d];
             }
-            c_child[2 * g + side] = int16_t(k);
+            c_child[2 * s + side] = int16_t(k);
           }
         }
       }
@@ -1010,20 +1031,19 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
 #pragma unroll
       for (int j = 0; j < kLocalItems; ++j) {
         const int p = t * kLocalItems + j;
-        const int g = pseg[p];
-        if (g >= 0) pseg[p] = c_child[2 * g + (p - c_start[g] < c_left[g] ? 0 : 1)];
+        if (g[j] >= 0) pseg[p] = c_child[2 * g[j] + (p - sst[j] < c_left[g[j]] ? 0 : 1)];
       }
       ns = s_next;
       __syncthreads();
-      for (int g = t; g < ns; g += kLocalBlock) {
-        c_start[g] = n_start[g];
-        c_count[g] = n_count[g];
-        c_rc[g] = n_rc[g];
-        c_node[g] = n_node[g];
+      for (int s = t; s < ns; s += kLocalBlock) {
+        c_start[s] = n_start[s];
+        c_count[s] = n_count[s];
+        c_rc[s] = n_rc[s];
+        c_node[s] = n_node[s];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-          c_lo[g * DIM + d] = n_lo[g * DIM + d];
-          c_hi[g * DIM + d] = n_hi[g * DIM + d];
+          c_lo[s * DIM + d] = n_lo[s * DIM + d];
+          c_hi[s * DIM + d] = n_hi[s * DIM + d];
         }
       }
       ++level;
