@@ -724,7 +724,8 @@ __global__ void k_pt_reseg(int64_t n, const int32_t* __restrict__ gate,
 }
 
 // Exclusive prefix sum over the workgroup's kLocalBlock threads (wave
-// shuffles, then the waves' totals through LDS).
+// shuffles, then the waves' totals through LDS). One barrier: the caller
+// does not write wave_tot again before every thread has passed another.
 __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wave_tot) {
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   uint32_t incl = v;
@@ -737,7 +738,6 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wave_to
   __syncthreads();
   uint32_t add = 0;
   for (int i = 0; i < w; ++i) add += wave_tot[i];
-  __syncthreads();
   return add + incl - v;
 }
 
@@ -746,36 +746,70 @@ __device__ __forceinline__ float rec_coord(const float4& r, uint32_t cd) {
   return cd == 2u ? r.z : lo;
 }
 
+// The segments that split at one level of an LDS subtree.
+template <int DIM>
+struct LocalTable {
+  int32_t start[kLocalSegs], count[kLocalSegs], rc[kLocalSegs], node[kLocalSegs];
+  float lo[kLocalSegs * DIM], hi[kLocalSegs * DIM];
+  uint32_t cd[kLocalSegs], mn[kLocalSegs], mx[kLocalSegs], lt[kLocalSegs], eq[kLocalSegs];
+  float ideal[kLocalSegs];
+
+  // Entry k: position in the subtree's points, count, right-child count,
+  // node id and cell; its cut dimension (the first widest extent) and ideal
+  // cut (the cell's midpoint); min / max and counts cleared.
+  __device__ __forceinline__ void put(int k, int32_t st, int32_t cnt, int32_t r, int32_t nd,
+                                      const float* l, const float* h) {
+    start[k] = st;
+    count[k] = cnt;
+    rc[k] = r;
+    node[k] = nd;
+    float best = 0.f;
+    uint32_t axis = 0;
+    float mid = (h[0] + l[0]) * 0.5f;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      lo[k * DIM + d] = l[d];
+      hi[k * DIM + d] = h[d];
+      const float extent = h[d] - l[d];
+      if (extent > best) {
+        best = extent;
+        axis = uint32_t(d);
+        mid = (h[d] + l[d]) * 0.5f;
+      }
+    }
+    cd[k] = axis;
+    ideal[k] = mid;
+    mn[k] = 0xffffffffu;
+    mx[k] = 0u;
+    lt[k] = 0u;
+    eq[k] = 0u;
+  }
+};
+
 // Finishes segments of at most kLocalT points in LDS, one workgroup per
 // segment (taken from the LocalSeg list through a counter): level by level,
 // every splitting segment of the subtree at once, the steps of the level
 // loop above on LDS copies of the records (thread t owns points
-// [kLocalItems t, kLocalItems (t + 1)); per-segment min/max and counts by
-// runs of equal segments, one LDS atomic per run; each Hoare pass's ranks
-// from a workgroup scan of the packed out-of-place flags). The subtree's
-// nodes take ids r0 + 2 * start + j (j < 2 * count - 2, in creation order);
-// the segment's own node keeps its level-loop id.
+// [kLocalItems t, kLocalItems (t + 1)) and keeps their segments in
+// registers; per-segment min/max and counts by runs of equal segments, one
+// LDS atomic per run; each Hoare pass's ranks from a workgroup scan of the
+// packed out-of-place flags). A level reads its segments from one table and
+// writes the next level's into the other, with their cut dimensions and
+// ideal cuts, so a level takes ten workgroup barriers. The subtree's nodes
+// take ids r0 + 2 * start + j (j < 2 * count - 2, in creation order); the
+// segment's own node keeps its level-loop id.
 template <int DIM>
 __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
     const LocalSeg* __restrict__ list, BuildCounters* __restrict__ ctr, float4* __restrict__ recs,
     Bfs bfs, int64_t r0, uint32_t bucket) {
   __shared__ float4 rec[kLocalT];
-  __shared__ int16_t pseg[kLocalT];   // segment of the point in the level's table, -1 done
   __shared__ int16_t slot_r[kLocalT];
   __shared__ uint32_t excl[kLocalT + 1];
-  // the level's splitting segments
-  __shared__ int32_t c_start[kLocalSegs], c_count[kLocalSegs], c_rc[kLocalSegs], c_node[kLocalSegs];
-  __shared__ float c_lo[kLocalSegs * DIM], c_hi[kLocalSegs * DIM];
-  __shared__ uint32_t c_cd[kLocalSegs], c_min[kLocalSegs], c_max[kLocalSegs], c_lt[kLocalSegs],
-      c_eq[kLocalSegs];
-  __shared__ float c_ideal[kLocalSegs], c_cut[kLocalSegs];
+  __shared__ LocalTable<DIM> tab[2];
   __shared__ int32_t c_left[kLocalSegs];
   __shared__ int16_t c_child[2 * kLocalSegs];
-  // the next level's
-  __shared__ int32_t n_start[kLocalSegs], n_count[kLocalSegs], n_rc[kLocalSegs], n_node[kLocalSegs];
-  __shared__ float n_lo[kLocalSegs * DIM], n_hi[kLocalSegs * DIM];
-  __shared__ uint32_t wave_tot[kLocalBlock / kWave];
-  __shared__ int32_t s_entry, s_next, s_nodes, s_lmax;
+  __shared__ uint32_t wave_tot[2][kLocalBlock / kWave];  // one per Hoare pass
+  __shared__ int32_t s_entry, s_next[2], s_nodes, s_lmax;
   const int t = threadIdx.x;
   for (;;) {
     if (t == 0) s_entry = int32_t(atomicAdd(&ctr->claim, 1u));
@@ -787,56 +821,37 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
     const int32_t C = list[e].count;
     const int64_t S0 = list[e].start;
     int32_t level = list[e].level;
-    for (int p = t; p < kLocalT; p += kLocalBlock) {
-      if (p < C) rec[p] = recs[S0 + p];
-      pseg[p] = int16_t(p < C ? 0 : -1);
-    }
+    for (int p = t; p < C; p += kLocalBlock) rec[p] = recs[S0 + p];
+    // segment of each of this thread's points in the level's table; -1:
+    // in a leaf, or past the segment
+    int g[kLocalItems];
+#pragma unroll
+    for (int j = 0; j < kLocalItems; ++j) g[j] = t * kLocalItems + j < C ? 0 : -1;
     if (t == 0) {
-      c_start[0] = 0;
-      c_count[0] = C;
-      c_rc[0] = list[e].rc;
-      c_node[0] = list[e].node;
+      float lo[DIM], hi[DIM];
+#pragma unroll
       for (int d = 0; d < DIM; ++d) {
-        c_lo[d] = list[e].lo[d];
-        c_hi[d] = list[e].hi[d];
+        lo[d] = list[e].lo[d];
+        hi[d] = list[e].hi[d];
       }
+      tab[0].put(0, 0, C, list[e].rc, list[e].node, lo, hi);
+      s_next[0] = 0;
+      s_next[1] = 0;
       s_nodes = 0;
       s_lmax = level;
     }
     __syncthreads();
-    int ns = 1;
+    int ns = 1, cur = 0;
     while (ns > 0) {
-      // cut dimension: the first widest extent; ideal cut: the midpoint
-      for (int g = t; g < ns; g += kLocalBlock) {
-        float best = 0.f;
-        uint32_t axis = 0;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          const float extent = c_hi[g * DIM + d] - c_lo[g * DIM + d];
-          if (extent > best) {
-            best = extent;
-            axis = uint32_t(d);
-          }
-        }
-        c_cd[g] = axis;
-        c_ideal[g] = (c_hi[g * DIM + axis] + c_lo[g * DIM + axis]) * 0.5f;
-        c_min[g] = 0xffffffffu;
-        c_max[g] = 0u;
-        c_lt[g] = 0u;
-        c_eq[g] = 0u;
-      }
-      if (t == 0) s_next = 0;
-      __syncthreads();
-      // This thread's points for the level, in registers: segment (-1:
-      // done), cut dimension, coordinate along it. The loads of all items
-      // are issued before the first LDS atomic, which would order them.
-      int g[kLocalItems];
+      LocalTable<DIM>& ct = tab[cur];
+      LocalTable<DIM>& nt = tab[cur ^ 1];
+      // This thread's points for the level, in registers: cut dimension and
+      // coordinate along it. The loads of all items are issued before the
+      // first LDS atomic, which would order them.
       uint32_t cdv[kLocalItems];
       float x[kLocalItems];
 #pragma unroll
-      for (int j = 0; j < kLocalItems; ++j) g[j] = pseg[t * kLocalItems + j];
-#pragma unroll
-      for (int j = 0; j < kLocalItems; ++j) cdv[j] = g[j] >= 0 ? c_cd[g[j]] : 0u;
+      for (int j = 0; j < kLocalItems; ++j) cdv[j] = g[j] >= 0 ? ct.cd[g[j]] : 0u;
 #pragma unroll
       for (int j = 0; j < kLocalItems; ++j) x[j] = rec_coord(rec[t * kLocalItems + j], cdv[j]);
       // min / max of the cut coordinate
@@ -847,8 +862,8 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
         for (int j = 0; j < kLocalItems; ++j) {
           if (g[j] != run) {
             if (run >= 0) {
-              atomicMin(&c_min[run], mn);
-              atomicMax(&c_max[run], mx);
+              atomicMin(&ct.mn[run], mn);
+              atomicMax(&ct.mx[run], mx);
             }
             run = g[j];
             mn = 0xffffffffu;
@@ -861,24 +876,30 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
           }
         }
         if (run >= 0) {
-          atomicMin(&c_min[run], mn);
-          atomicMax(&c_max[run], mx);
+          atomicMin(&ct.mn[run], mn);
+          atomicMax(&ct.mx[run], mx);
         }
       }
       __syncthreads();
-      for (int s = t; s < ns; s += kLocalBlock) {
-        const float pmin = key_float(c_min[s]), pmax = key_float(c_max[s]);
-        const float ideal = c_ideal[s];
-        c_cut[s] = ideal < pmin ? pmin : (ideal > pmax ? pmax : ideal);
-      }
-      __syncthreads();
-      // points below / equal to the cut
+      // every thread has read this table's count: the counter of the table
+      // the next level fills can be cleared
+      if (t == 0) s_next[cur] = 0;
+      // the cut (the ideal cut clamped to the points' range), points below /
+      // equal to it
       float cut[kLocalItems];
-      int32_t sst[kLocalItems];
+      int32_t sst[kLocalItems], scnt[kLocalItems];
 #pragma unroll
       for (int j = 0; j < kLocalItems; ++j) {
-        cut[j] = g[j] >= 0 ? c_cut[g[j]] : 0.f;
-        sst[j] = g[j] >= 0 ? c_start[g[j]] : 0;
+        cut[j] = 0.f;
+        sst[j] = 0;
+        scnt[j] = 0;
+        if (g[j] >= 0) {
+          const float pmin = key_float(ct.mn[g[j]]), pmax = key_float(ct.mx[g[j]]);
+          const float ideal = ct.ideal[g[j]];
+          cut[j] = ideal < pmin ? pmin : (ideal > pmax ? pmax : ideal);
+          sst[j] = ct.start[g[j]];
+          scnt[j] = ct.count[g[j]];
+        }
       }
       {
         int run = -1;
@@ -887,8 +908,8 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
         for (int j = 0; j < kLocalItems; ++j) {
           if (g[j] != run) {
             if (run >= 0) {
-              atomicAdd(&c_lt[run], lt);
-              atomicAdd(&c_eq[run], eq);
+              atomicAdd(&ct.lt[run], lt);
+              atomicAdd(&ct.eq[run], eq);
             }
             run = g[j];
             lt = 0u;
@@ -900,17 +921,16 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
           }
         }
         if (run >= 0) {
-          atomicAdd(&c_lt[run], lt);
-          atomicAdd(&c_eq[run], eq);
+          atomicAdd(&ct.lt[run], lt);
+          atomicAdd(&ct.eq[run], eq);
         }
       }
       __syncthreads();
-      int32_t br1[kLocalItems], br2[kLocalItems], scnt[kLocalItems];
+      int32_t br1[kLocalItems], br2[kLocalItems];
 #pragma unroll
       for (int j = 0; j < kLocalItems; ++j) {
-        br1[j] = g[j] >= 0 ? int32_t(c_lt[g[j]]) : 0;
-        br2[j] = br1[j] + (g[j] >= 0 ? int32_t(c_eq[g[j]]) : 0);
-        scnt[j] = g[j] >= 0 ? c_count[g[j]] : 0;
+        br1[j] = g[j] >= 0 ? int32_t(ct.lt[g[j]]) : 0;
+        br2[j] = br1[j] + (g[j] >= 0 ? int32_t(ct.eq[g[j]]) : 0);
       }
       // the two Hoare passes: flags (low half: belongs right but sits left,
       // high half: belongs left but sits right), ranks, pairs, swaps
@@ -941,7 +961,7 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
           sum += f[j];
         }
         uint32_t ex[kLocalItems];
-        uint32_t run = block_excl_sum(sum, wave_tot);
+        uint32_t run = block_excl_sum(sum, wave_tot[pass - 1]);
 #pragma unroll
         for (int j = 0; j < kLocalItems; ++j) {
           ex[j] = run;
@@ -968,40 +988,42 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
           rec[p] = rec[q];
           rec[q] = a;
         }
-        __syncthreads();
+        // the second pass's swaps are read only after the barrier that
+        // ends the level
+        if (pass == 1) __syncthreads();
       }
-      // leftCount rule, node words, children
+      // leftCount rule, node words, children (the next level's table)
       for (int s = t; s < ns; s += kLocalBlock) {
-        const int32_t cnt = c_count[s];
-        const float ideal = c_ideal[s];
-        const float pmin = key_float(c_min[s]), pmax = key_float(c_max[s]);
-        const int32_t br1 = int32_t(c_lt[s]);
-        const int32_t br2 = br1 + int32_t(c_eq[s]);
+        const int32_t cnt = ct.count[s];
+        const float ideal = ct.ideal[s];
+        const float pmin = key_float(ct.mn[s]), pmax = key_float(ct.mx[s]);
+        const int32_t b1 = int32_t(ct.lt[s]);
+        const int32_t b2 = b1 + int32_t(ct.eq[s]);
         int32_t left;
         if (ideal < pmin)
           left = 1;
         else if (ideal > pmax)
           left = cnt - 1;
-        else if (br1 > cnt / 2)
-          left = br1;
-        else if (br2 < cnt / 2)
-          left = br2;
+        else if (b1 > cnt / 2)
+          left = b1;
+        else if (b2 < cnt / 2)
+          left = b2;
         else
           left = cnt / 2;
         left = min(max(left, 1), cnt - 1);
         c_left[s] = left;
-        const uint32_t cd = c_cd[s];
-        const float cut = c_cut[s];
+        const uint32_t cd = ct.cd[s];
+        const float cut_s = ideal < pmin ? pmin : (ideal > pmax ? pmax : ideal);
         const int64_t id0 = r0 + 2 * S0 + atomicAdd(&s_nodes, 2);
-        const int32_t node = c_node[s];
+        const int32_t node = ct.node[s];
         bfs.cd[node] = cd;
-        bfs.aux[node] = __float_as_uint(cut);
+        bfs.aux[node] = __float_as_uint(cut_s);
         bfs.left[node] = int32_t(id0);
         bfs.right[node] = int32_t(id0 + 1);
         for (int side = 0; side < 2; ++side) {
-          const int32_t cs = c_start[s] + (side ? left : 0);
+          const int32_t cs = ct.start[s] + (side ? left : 0);
           const int32_t cc = side ? cnt - left : left;
-          const int32_t rc = c_rc[s] + side;
+          const int32_t rc = ct.rc[s] + side;
           const int64_t id = id0 + side;
           bfs.start[id] = int32_t(S0 + cs);
           bfs.level[id] = level + 1;
@@ -1013,16 +1035,14 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
             c_child[2 * s + side] = -1;
             atomicMax(&s_lmax, level + 1);
           } else {
-            const int k = atomicAdd(&s_next, 1);
-            n_start[k] = cs;
-            n_count[k] = cc;
-            n_rc[k] = rc;
-            n_node[k] = int32_t(id);
+            float lo[DIM], hi[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-              n_lo[k * DIM + d] = (side && uint32_t(d) == cd) ? cut : c_lo[s * DIM + d];
-              n_hi[k * DIM + d] = (!side && uint32_t(d) == cd) ? cut : c_hi[s * DIM + d];
+              lo[d] = (side && uint32_t(d) == cd) ? cut_s : ct.lo[s * DIM + d];
+              hi[d] = (!side && uint32_t(d) == cd) ? cut_s : ct.hi[s * DIM + d];
             }
+            const int k = atomicAdd(&s_next[cur ^ 1], 1);
+            nt.put(k, cs, cc, rc, int32_t(id), lo, hi);
             c_child[2 * s + side] = int16_t(k);
           }
         }
@@ -1031,23 +1051,11 @@ __global__ __launch_bounds__(kLocalBlock) void k_local_finish(
 #pragma unroll
       for (int j = 0; j < kLocalItems; ++j) {
         const int p = t * kLocalItems + j;
-        if (g[j] >= 0) pseg[p] = c_child[2 * g[j] + (p - sst[j] < c_left[g[j]] ? 0 : 1)];
+        if (g[j] >= 0) g[j] = c_child[2 * g[j] + (p - sst[j] < c_left[g[j]] ? 0 : 1)];
       }
-      ns = s_next;
-      __syncthreads();
-      for (int s = t; s < ns; s += kLocalBlock) {
-        c_start[s] = n_start[s];
-        c_count[s] = n_count[s];
-        c_rc[s] = n_rc[s];
-        c_node[s] = n_node[s];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          c_lo[s * DIM + d] = n_lo[s * DIM + d];
-          c_hi[s * DIM + d] = n_hi[s * DIM + d];
-        }
-      }
+      ns = s_next[cur ^ 1];
+      cur ^= 1;
       ++level;
-      __syncthreads();
     }
     for (int p = t; p < C; p += kLocalBlock) recs[S0 + p] = rec[p];
     if (t == 0) {
