@@ -688,8 +688,28 @@ struct CurveBox {
   int axis_bits;
   float lo[3], sc[3];
 };
+
+// Bit b of the low 10 (spread3) or 16 (spread2) bits of v moved to bit 3b
+// (2b): the interleave of a Morton / Hilbert key with constant masks.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v ^ (v << 16)) & 0xff0000ffu;
+  v = (v ^ (v << 8)) & 0x0300f00fu;
+  v = (v ^ (v << 4)) & 0x030c30c3u;
+  v = (v ^ (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ uint32_t spread2(uint32_t v) {
+  v &= 0xffffu;
+  v = (v ^ (v << 8)) & 0x00ff00ffu;
+  v = (v ^ (v << 4)) & 0x0f0f0f0fu;
+  v = (v ^ (v << 2)) & 0x33333333u;
+  v = (v ^ (v << 1)) & 0x55555555u;
+  return v;
+}
 template <int DIM>
 __device__ __forceinline__ uint32_t hilbert_code(const float* qp, const CurveBox& box) {
+  static_assert(DIM == 2 || DIM == 3, "keys of 2-D and 3-D queries");
   const float top = float((1u << box.axis_bits) - 1u);
   uint32_t x[DIM];
 #pragma unroll
@@ -711,15 +731,21 @@ __device__ __forceinline__ uint32_t hilbert_code(const float* qp, const CurveBox
   }
 #pragma unroll
   for (int i = 1; i < DIM; ++i) x[i] ^= x[i - 1];
-  uint32_t flip = 0;
-  for (uint32_t q = msb; q > 1; q >>= 1)
-    if (x[DIM - 1] & q) flip ^= q - 1;
+  // flip = the xor of q - 1 over the bits q > 1 of the last coordinate:
+  // bit j is the parity of its bits above j (a suffix parity, shifted)
+  uint32_t y = x[DIM - 1];
+  y ^= y >> 1;
+  y ^= y >> 2;
+  y ^= y >> 4;
+  y ^= y >> 8;
+  y ^= y >> 16;
+  const uint32_t flip = y >> 1;
+  // bit b of coordinate i goes to bit b * DIM + (DIM - 1 - i)
   uint32_t code = 0;
 #pragma unroll
   for (int i = 0; i < DIM; ++i) {
     const uint32_t xi = x[i] ^ flip;
-    for (int b = 0; b < box.axis_bits; ++b)
-      code |= ((xi >> b) & 1u) << (b * DIM + (DIM - 1 - i));
+    code |= (DIM == 3 ? spread3(xi) : spread2(xi)) << (DIM - 1 - i);
   }
   return code;
 }
@@ -798,8 +824,7 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_scatter(const uint32_t* __r
                                                          int64_t m, int fine_bits,
                                                          int coarse_bins, int64_t tiles,
                                                          const uint32_t* __restrict__ starts,
-                                                         uint32_t* __restrict__ fkeys,
-                                                         int32_t* __restrict__ fidx) {
+                                                         uint2* __restrict__ fpairs) {
   __shared__ uint32_t cur[1 << kCoarseBits];
   const int64_t tile = blockIdx.x;
   for (int b = threadIdx.x; b < coarse_bins; b += kHistBlock)
@@ -810,13 +835,11 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_scatter(const uint32_t* __r
   for (int64_t q = tile * kSortTile + threadIdx.x; q < end; q += kHistBlock) {
     const uint32_t key = keys[q];
     const uint32_t pos = atomicAdd(&cur[key >> fine_bits], 1u);
-    fkeys[pos] = key & fine_mask;
-    fidx[pos] = int32_t(q);
+    fpairs[pos] = make_uint2(key & fine_mask, uint32_t(q));  // {fine key, query}
   }
 }
 
-__global__ __launch_bounds__(kHistBlock) void k_sort_fine(const uint32_t* __restrict__ fkeys,
-                                                      const int32_t* __restrict__ fidx,
+__global__ __launch_bounds__(kHistBlock) void k_sort_fine(const uint2* __restrict__ fpairs,
                                                       const uint32_t* __restrict__ starts,
                                                       int64_t tiles, int64_t m, int fine_bits,
                                                       int coarse_bins, int32_t* __restrict__ order) {
@@ -826,13 +849,13 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_fine(const uint32_t* __rest
   const uint32_t begin = starts[int64_t(bin) * tiles];
   const uint32_t end = bin + 1 < coarse_bins ? starts[int64_t(bin + 1) * tiles] : uint32_t(m);
   if (end - begin <= 1 || fine_bits == 0) {
-    for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock) order[e] = fidx[e];
+    for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock) order[e] = int32_t(fpairs[e].y);
     return;
   }
   const int bins = 1 << fine_bits;
   for (int b = threadIdx.x; b < bins; b += kHistBlock) h[b] = 0;
   __syncthreads();
-  for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock) atomicAdd(&h[fkeys[e]], 1u);
+  for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock) atomicAdd(&h[fpairs[e].x], 1u);
   __syncthreads();
   // Exclusive scan of the bins: each thread owns bins [t * per, (t + 1) * per).
   const int per = (bins + kHistBlock - 1) / kHistBlock;
@@ -858,8 +881,10 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_fine(const uint32_t* __rest
     run += c;
   }
   __syncthreads();
-  for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock)
-    order[atomicAdd(&h[fkeys[e]], 1u)] = fidx[e];
+  for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock) {
+    const uint2 f = fpairs[e];
+    order[atomicAdd(&h[f.x], 1u)] = int32_t(f.y);
+  }
 }
 
 // Persistent grid: as many workgroups as the device holds at once (from
@@ -1128,15 +1153,15 @@ void kd_knn_launch(const Index& ix, const KnnArgs& a, Launcher& L) {
 
   Scratch scratch(L.stream(), a.scratch ? a.scratch->base : nullptr,
                   a.scratch ? a.scratch->bytes : 0);
-  Span<uint32_t> counters, keys, counts, starts, fkeys, partials;
-  Span<int32_t> fidx, order;
+  Span<uint32_t> counters, keys, counts, starts, partials;
+  Span<uint2> fpairs;
+  Span<int32_t> order;
   scratch.want(counters, kXcds);
   if (sort) {
     scratch.want(keys, size_t(a.m));
     scratch.want(counts, size_t(ncounts));
     scratch.want(starts, size_t(ncounts));
-    scratch.want(fkeys, size_t(a.m));
-    scratch.want(fidx, size_t(a.m));
+    scratch.want(fpairs, size_t(a.m));
     scratch.want(order, size_t(a.m));
     scratch.want(partials, kScanGrid);
   }
@@ -1184,10 +1209,10 @@ void kd_knn_launch(const Index& ix, const KnnArgs& a, Launcher& L) {
                              partials.p, static_cast<uint32_t*>(nullptr));
     L(k_sort_scatter, dim3(unsigned(tiles)), dim3(kHistBlock), 0,
       static_cast<const uint32_t*>(keys.p), a.m, fine_bits, coarse_bins, tiles,
-      static_cast<const uint32_t*>(starts.p), fkeys.p, fidx.p);
+      static_cast<const uint32_t*>(starts.p), fpairs.p);
     L(k_sort_fine, dim3(unsigned(coarse_bins)), dim3(kHistBlock), 0,
-      static_cast<const uint32_t*>(fkeys.p), static_cast<const int32_t*>(fidx.p),
-      static_cast<const uint32_t*>(starts.p), tiles, a.m, fine_bits, coarse_bins, order.p);
+      static_cast<const uint2*>(fpairs.p), static_cast<const uint32_t*>(starts.p), tiles, a.m,
+      fine_bits, coarse_bins, order.p);
     p.order = order.p;
   }
 
