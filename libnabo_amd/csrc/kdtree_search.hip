@@ -771,6 +771,7 @@ __global__ void k_zero_counters(uint32_t* counters, int64_t n) {
 //      the key's low fine_bits.
 constexpr int kSortTile = 4096;
 constexpr int kHistBlock = 1024;  // sort kernels' workgroups (hist, scatter: 4 queries per thread)
+constexpr int kFineBlock = 512;   // fine pass: one workgroup per coarse bucket (~490 queries for 1M)
 constexpr int kCoarseBits = 11;
 constexpr int kFineBitsMax = 10;
 
@@ -839,26 +840,26 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_scatter(const uint32_t* __r
   }
 }
 
-__global__ __launch_bounds__(kHistBlock) void k_sort_fine(const uint2* __restrict__ fpairs,
+__global__ __launch_bounds__(kFineBlock) void k_sort_fine(const uint2* __restrict__ fpairs,
                                                       const uint32_t* __restrict__ starts,
                                                       int64_t tiles, int64_t m, int fine_bits,
                                                       int coarse_bins, int32_t* __restrict__ order) {
   __shared__ uint32_t h[1 << kFineBitsMax];
-  __shared__ uint32_t wave_sum[kHistBlock / kWave];
+  __shared__ uint32_t wave_sum[kFineBlock / kWave];
   const int bin = blockIdx.x;
   const uint32_t begin = starts[int64_t(bin) * tiles];
   const uint32_t end = bin + 1 < coarse_bins ? starts[int64_t(bin + 1) * tiles] : uint32_t(m);
   if (end - begin <= 1 || fine_bits == 0) {
-    for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock) order[e] = int32_t(fpairs[e].y);
+    for (uint32_t e = begin + threadIdx.x; e < end; e += kFineBlock) order[e] = int32_t(fpairs[e].y);
     return;
   }
   const int bins = 1 << fine_bits;
-  for (int b = threadIdx.x; b < bins; b += kHistBlock) h[b] = 0;
+  for (int b = threadIdx.x; b < bins; b += kFineBlock) h[b] = 0;
   __syncthreads();
-  for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock) atomicAdd(&h[fpairs[e].x], 1u);
+  for (uint32_t e = begin + threadIdx.x; e < end; e += kFineBlock) atomicAdd(&h[fpairs[e].x], 1u);
   __syncthreads();
   // Exclusive scan of the bins: each thread owns bins [t * per, (t + 1) * per).
-  const int per = (bins + kHistBlock - 1) / kHistBlock;
+  const int per = (bins + kFineBlock - 1) / kFineBlock;
   const int b0 = threadIdx.x * per;
   uint32_t sum = 0;
   for (int b = b0; b < min(bins, b0 + per); ++b) sum += h[b];
@@ -881,7 +882,7 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_fine(const uint2* __restric
     run += c;
   }
   __syncthreads();
-  for (uint32_t e = begin + threadIdx.x; e < end; e += kHistBlock) {
+  for (uint32_t e = begin + threadIdx.x; e < end; e += kFineBlock) {
     const uint2 f = fpairs[e];
     order[atomicAdd(&h[f.x], 1u)] = int32_t(f.y);
   }
@@ -1210,7 +1211,7 @@ void kd_knn_launch(const Index& ix, const KnnArgs& a, Launcher& L) {
     L(k_sort_scatter, dim3(unsigned(tiles)), dim3(kHistBlock), 0,
       static_cast<const uint32_t*>(keys.p), a.m, fine_bits, coarse_bins, tiles,
       static_cast<const uint32_t*>(starts.p), fpairs.p);
-    L(k_sort_fine, dim3(unsigned(coarse_bins)), dim3(kHistBlock), 0,
+    L(k_sort_fine, dim3(unsigned(coarse_bins)), dim3(kFineBlock), 0,
       static_cast<const uint2*>(fpairs.p), static_cast<const uint32_t*>(starts.p), tiles, a.m,
       fine_bits, coarse_bins, order.p);
     p.order = order.p;
