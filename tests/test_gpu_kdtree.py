@@ -175,6 +175,26 @@ def test_knn_dim_subset(nabo, oracle, rng):
     assert_bit_exact(gi, gd, oi, od, "dim subset")
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_knn_query_order_full_key_width(nabo, oracle, dim):
+    """600k queries: the query order's Hilbert keys at their full width (10
+    bits per axis in 2-D, 7 in 3-D, the coarse pass's 11 bits and the fine
+    pass's 9 or 10), queries partly outside the cloud's box (clamped to its
+    edge cells); results bit-exact with the oracle, which searches the
+    queries in their given order."""
+    rng = np.random.default_rng(20 + dim)
+    scale = np.array([50.0, 20.0, 8.0][:dim])
+    cloud = (rng.random((120_000, dim)) * scale).astype(np.float32)
+    query = (rng.random((600_000, dim)) * (scale + 2.0) - 1.0).astype(np.float32)
+    nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, dim,
+                                      {"bucketSize": 8, "querySort": 1})
+    otree = oracle.kdtree(cloud, bucket_size=8)
+    print(f"case knn full-width query order dim={dim}", flush=True)
+    gi, gd = nns.knn(query, 4, 0.0, 0)
+    oi, od = otree.knn(query, 4, 0.0, 0)
+    assert_bit_exact(gi, gd, oi, od, f"full-width keys dim={dim}")
+
+
 def test_knn_no_query_sort(nabo, oracle):
     cloud, query = datasets.uniform_cube(10_000, 4_000, seed=3)
     nns = nabo.NearestNeighbourSearch(cloud, nabo.SearchType.KDTREE_HIP, 3,
